@@ -1,0 +1,158 @@
+/*
+ * within_amd.h -- C ABI of the MI355X-native projection path of
+ * py-econometrics/within (D'WDx = D'Wz and the demeaned residual z - Dx).
+ *
+ * Plain pointers and sizes only.  Host-pointer entry points copy in and out;
+ * *_device entry points take pointers to gfx950 device memory.  Every call is
+ * synchronous on return.  Status codes mirror the reference's WithinError
+ * variants (crates/within/src/error.rs:25-59); non-convergence is reported in
+ * the per-RHS stats, never as an error (solver.rs:158-239).
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef WITHIN_AMD_H
+#define WITHIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wa_design wa_design;
+typedef struct wa_solver wa_solver;
+
+typedef enum {
+  WA_OK = 0,
+  WA_ERR_EMPTY_OBSERVATIONS = 1,   /* WithinError::EmptyObservations            */
+  WA_ERR_OBS_COUNT_MISMATCH = 2,   /* WithinError::ObservationCountMismatch     */
+  WA_ERR_WEIGHT_COUNT_MISMATCH = 3,/* WithinError::WeightCountMismatch          */
+  WA_ERR_OVERFLOW = 4,             /* WithinError::Overflow                     */
+  WA_ERR_SINGULAR_DIAGONAL = 5,    /* WithinError::SingularDiagonal             */
+  WA_ERR_LOCAL_SOLVER_BUILD = 6,   /* WithinError::LocalSolverBuild             */
+  WA_ERR_PRECONDITIONER_BUILD = 7, /* WithinError::PreconditionerBuild          */
+  WA_ERR_ITERATIVE_SOLVE = 8,      /* WithinError::IterativeSolve               */
+  WA_ERR_INVALID_ARGUMENT = 9,     /* bad sizes / pointers / config values      */
+  WA_ERR_DEVICE = 10,              /* HIP runtime failure, or no gfx950 device  */
+  WA_ERR_UNSUPPORTED = 11          /* KrylovMethod::Lsmr and other unbuilt paths */
+} wa_status;
+
+/* LocalSolverConfig + ApproxCholConfig + ApproxSchurConfig
+ * (crates/within/src/config.rs:40-149).  Rust LocalSolverConfig::default():
+ * ac_split 1, approx_schur 1 (seed 0, split 1), dense_threshold 24;
+ * solver_default(): ac_split 2. */
+typedef struct {
+  uint64_t ac_seed;
+  int32_t ac_split;        /* split_merge: None -> 1                              */
+  int32_t approx_schur;    /* 0 = exact Schur complement, 1 = clique-tree sampled  */
+  uint64_t schur_seed;
+  int32_t schur_split;
+  int32_t dense_threshold; /* n_keep <= threshold -> dense anchored Cholesky      */
+  int64_t dense_core;      /* approximate Cholesky hands its last <= dense_core
+                              vertices to an exact dense factor (0 = default 2048,
+                              1 = pure approximate Cholesky)                     */
+} wa_local_config;
+
+/* SolverParams + Preconditioner (config.rs:157-237). */
+typedef struct {
+  int32_t krylov;          /* 0 = Cg, 1 = Gmres{restart}, 2 = Lsmr (unsupported)   */
+  int32_t restart;         /* GMRES restart (default 30)                           */
+  int32_t operator_repr;   /* 0 = Implicit, 1 = Explicit                           */
+  int32_t preconditioner;  /* 0 = none, 1 = Additive, 2 = Multiplicative           */
+  double tol;              /* default 1e-8                                         */
+  int64_t maxiter;         /* default 1000                                         */
+  int64_t max_refinements; /* default 2                                            */
+  int32_t precision;       /* 0 = FP64 throughout (reference semantics),
+                              1 = FP32 inner Krylov + FP64 iterative refinement    */
+  int32_t block;           /* RHS per block-vector solve: 0 = auto (32), 1 = one
+                              RHS per solve                                        */
+  int32_t host_only;       /* 1 = build the host-side setup only (no GPU); apply
+                              and solve calls then fail with WA_ERR_DEVICE        */
+  wa_local_config local;
+} wa_params;
+
+/* Per-RHS result fields of SolveResult / BatchSolveResult
+ * (orchestrate.rs:37-134). */
+typedef struct {
+  int32_t converged;
+  int64_t iterations;      /* all Krylov iterations incl. refinement corrections  */
+  double residual;         /* ||D'W(y - Dx)|| / max(||D'Wy||, 1e-15)              */
+  double time_solve;       /* seconds (the block's solve time, shared by its RHS)  */
+} wa_rhs_stats;
+
+/* Message of the last failing call on this thread (never NULL). */
+const char* wa_last_error(void);
+
+/* Fills wa_params with SolverParams::default() and Rust
+ * LocalSolverConfig::default() (config.rs:96-104, :227-237), additive Schwarz. */
+void wa_params_default(wa_params* p);
+
+/* ---- design (WeightedDesign::from_store, domain.rs:133-156) ----------------
+ * codes: column q of the (n_obs x n_factors) uint32 category matrix starts at
+ * codes + q * ld (F-order, observation.rs:236-291).  weights: NULL = unit. */
+wa_status wa_design_create(const uint32_t* codes, int64_t n_obs, int32_t n_factors, int64_t ld,
+                           const double* weights, wa_design** out);
+void wa_design_destroy(wa_design* d);
+int64_t wa_design_n_dofs(const wa_design* d);
+int64_t wa_design_n_obs(const wa_design* d);
+int32_t wa_design_n_factors(const wa_design* d);
+/* level count of factor q (max code + 1) */
+int64_t wa_design_n_levels(const wa_design* d, int32_t q);
+
+/* ---- solver (Solver::from_design, solver.rs:98-132) ----------------------- */
+wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** out);
+void wa_solver_destroy(wa_solver* s);
+
+/* Solver::solve_batch (solver.rs:242-277; orchestrate.rs:172-184).
+ * Y: n_obs x k column-major (ldy >= n_obs).  X: n_dofs x k (ldx >= n_dofs),
+ * demeaned: n_obs x k (ldd >= n_obs), both column-major; demeaned may be NULL.
+ * stats: k entries. */
+wa_status wa_solve_batch(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X,
+                         int64_t ldx, double* demeaned, int64_t ldd, wa_rhs_stats* stats);
+/* Same, with Y / X / demeaned in device memory. */
+wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X,
+                                int64_t ldx, double* demeaned, int64_t ldd, wa_rhs_stats* stats);
+
+/* ---- Level A: Operator drop-ins (schwarz-precond/src/lib.rs:106-131) ------
+ * Column-major blocks of k vectors with leading dimension ld. */
+/* GramianOperator::apply (operator/gramian.rs:84-89): Y = D'W D X */
+wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
+/* Gramian (explicit CSR) Operator::apply (gramian/explicit.rs:261-277) */
+wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
+/* FePreconditioner::apply (operator/preconditioner.rs:108-154) */
+wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld);
+/* WeightedDesign::matvec_d (domain.rs:390-395): Y (n_obs x k) = D X */
+wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k);
+/* WeightedDesign::rmatvec_wdt (domain.rs:409-418): X (n_dofs x k) = D'W R */
+wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx,
+                         int32_t k);
+
+/* ---- setup introspection (host data; no GPU needed) ------------------------
+ * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
+ * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,
+ * operator_passes, dense_subdomains, max_core, reserved, reserved. */
+wa_status wa_solver_info(const wa_solver* s, int64_t* info16);
+/* Subdomain i: meta[0..7] = q, r, n_q, n_r, eliminate_q, dense, rounds, core
+ * size; gidx (n_q + n_r), pou (n_q + n_r) may be NULL. */
+wa_status wa_solver_subdomain(const wa_solver* s, int64_t i, int64_t* meta8, uint32_t* gidx,
+                              double* pou);
+/* The reduced-system factor of subdomain i in the oracle's layout: pivot (n),
+ * col_ptr (n+1), col_idx / col_w (nnz), round_ptr (rounds+1), order (eliminated
+ * count), core (c), core_inv (c*c).  nnz_out receives the column nnz. */
+wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz_out,
+                                     double* pivot, uint32_t* col_ptr, uint32_t* col_idx,
+                                     double* col_w, uint32_t* round_ptr, uint32_t* order,
+                                     uint32_t* core, double* core_inv);
+
+/* ---- kernel timing (HIP events on the solver's stream) ---------------------
+ * Kernel classes: 0 operator, 1 preconditioner, 2 vector ops, 3 observation
+ * passes, 4 explicit operator.  Returns launches and summed milliseconds
+ * since the last reset.  Timing adds event records: off unless enabled. */
+void wa_timing_enable(wa_solver* s, int32_t on);
+void wa_timing_reset(wa_solver* s);
+wa_status wa_timing_get(wa_solver* s, int32_t kclass, int64_t* launches, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WITHIN_AMD_H */

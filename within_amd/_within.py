@@ -1,0 +1,418 @@
+"""Python surface of ``within._within`` on the MI355X-native path.
+
+Mirrors the PyO3 bridge of the reference (``crates/within-py/src/lib.rs``
+:62-1074, stubs in ``python/within/_within.pyi``): the same config classes,
+defaults, result types and validation, backed by ``libwithin_amd.so``.
+Front-door defaults follow the reference exactly:
+
+* ``preconditioner=None`` means additive Schwarz with
+  ``LocalSolverConfig::solver_default()`` (AC split 2) (lib.rs:546-552);
+* ``SchurComplement()`` with no arguments means an exact Schur complement and
+  AC split 1 (lib.rs:499-514);
+* CG or LSMR with a multiplicative preconditioner raises ``ValueError``
+  (lib.rs:639-656).
+
+Extension: ``CG`` / ``GMRES`` accept ``precision="mixed"`` (FP32 inner
+Krylov + FP64 iterative refinement) and ``block`` (right-hand sides per
+block-vector solve) in addition to the reference fields.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+import warnings
+from enum import IntEnum
+
+import numpy as np
+
+from . import _native as N
+
+
+class OperatorRepr(IntEnum):
+    Implicit = 0
+    Explicit = 1
+
+
+class Preconditioner(IntEnum):
+    Additive = 0
+    Multiplicative = 1
+    Off = 2
+
+
+class ReductionStrategy(IntEnum):
+    """Accepted for API compatibility; the GPU combine is always the
+    deterministic gather over each DOF's subdomain slots."""
+    Auto = 0
+    AtomicScatter = 1
+    ParallelReduction = 2
+
+
+_PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+class CG:
+    def __init__(self, tol=1e-8, maxiter=1000, operator=OperatorRepr.Implicit, max_refinements=2,
+                 precision="fp64", block=None):
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.operator = OperatorRepr(operator)
+        self.max_refinements = int(max_refinements)
+        if precision not in _PRECISIONS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.block = block
+
+    def __repr__(self):
+        return (f"CG(tol={self.tol}, maxiter={self.maxiter}, operator={self.operator.name}, "
+                f"max_refinements={self.max_refinements}, precision={self.precision!r})")
+
+
+class GMRES:
+    def __init__(self, tol=1e-8, maxiter=1000, restart=30, operator=OperatorRepr.Implicit, max_refinements=2,
+                 precision="fp64", block=None):
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.restart = int(restart)
+        self.operator = OperatorRepr(operator)
+        self.max_refinements = int(max_refinements)
+        if precision not in _PRECISIONS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.block = block
+
+    def __repr__(self):
+        return (f"GMRES(tol={self.tol}, maxiter={self.maxiter}, restart={self.restart}, "
+                f"operator={self.operator.name}, max_refinements={self.max_refinements})")
+
+
+class LSMR:
+    """Accepted for API compatibility; solving with it raises ValueError
+    (modified LSMR is not part of the MI355X path)."""
+
+    def __init__(self, tol=1e-8, maxiter=1000, local_size=None):
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.local_size = local_size
+
+
+class ApproxCholConfig:
+    def __init__(self, seed=0, split=1):
+        if int(split) < 1:
+            raise ValueError("split must be >= 1")
+        self.seed = int(seed)
+        self.split = int(split)
+
+
+class ApproxSchurConfig:
+    def __init__(self, seed=0, split=1):
+        if int(split) < 1:
+            raise ValueError("split must be >= 1")
+        self.seed = int(seed)
+        self.split = int(split)
+
+
+class SchurComplement:
+    def __init__(self, approx_chol=None, approx_schur=None, dense_threshold=24):
+        if approx_chol is not None and not isinstance(approx_chol, ApproxCholConfig):
+            raise TypeError("approx_chol must be ApproxCholConfig or None")
+        if approx_schur is not None and not isinstance(approx_schur, ApproxSchurConfig):
+            raise TypeError("approx_schur must be ApproxSchurConfig or None")
+        self.approx_chol = approx_chol
+        self.approx_schur = approx_schur
+        self.dense_threshold = int(dense_threshold)
+
+
+class AdditiveSchwarz:
+    def __init__(self, local_solver=None, reduction=ReductionStrategy.Auto):
+        if local_solver is not None and not isinstance(local_solver, SchurComplement):
+            raise TypeError("local_solver must be SchurComplement or None")
+        self.local_solver = local_solver
+        self.reduction = ReductionStrategy(reduction)
+
+
+class MultiplicativeSchwarz:
+    def __init__(self, local_solver=None):
+        if local_solver is not None and not isinstance(local_solver, SchurComplement):
+            raise TypeError("local_solver must be SchurComplement or None")
+        self.local_solver = local_solver
+
+
+class SolveResult:
+    __slots__ = ("x", "demeaned", "converged", "iterations", "residual", "time_total", "time_setup",
+                 "time_solve")
+
+    def __init__(self, **kw):
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __repr__(self):
+        return (f"SolveResult(converged={self.converged}, iterations={self.iterations}, "
+                f"residual={self.residual:.3e}, n_dofs={len(self.x)})")
+
+
+class BatchSolveResult:
+    __slots__ = ("x", "demeaned", "converged", "iterations", "residual", "time_solve", "time_total")
+
+    def __init__(self, **kw):
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __repr__(self):
+        return f"BatchSolveResult(k={len(self.converged)}, converged={all(self.converged)})"
+
+
+class AdditiveSchwarzDiagnostics:
+    def __init__(self, info, reduction):
+        self.reduction_strategy = reduction
+        self.resolved_reduction_strategy = ReductionStrategy.ParallelReduction
+        self.total_inner_parallel_work = int(info[7] * 2 + info[1])
+        self.max_inner_parallel_work = int(info[7] * 2 + info[1])
+        self.total_scatter_dofs = int(info[1])
+        self.outer_parallel_capacity = float(max(info[0], 1))
+        self.scatter_overlap = float(info[1]) / max(float(info[9]), 1.0)
+
+    def __repr__(self):
+        return (f"AdditiveSchwarzDiagnostics(total_scatter_dofs={self.total_scatter_dofs}, "
+                f"scatter_overlap={self.scatter_overlap:.3f})")
+
+
+# ---------------------------------------------------------------------------
+# config extraction (crates/within-py/src/lib.rs:496-671)
+# ---------------------------------------------------------------------------
+def _local_from(sc, default_split):
+    """-> (ac_seed, ac_split, approx_schur, schur_seed, schur_split, dense_threshold)."""
+    if sc is None:
+        return (0, default_split, 1, 0, 1, 24)
+    if not isinstance(sc, SchurComplement):
+        raise TypeError("local_solver must be SchurComplement or None")
+    ac = sc.approx_chol or ApproxCholConfig()
+    if sc.approx_schur is None:
+        return (ac.seed, ac.split, 0, 0, 1, sc.dense_threshold)
+    return (ac.seed, ac.split, 1, sc.approx_schur.seed, sc.approx_schur.split, sc.dense_threshold)
+
+
+def _precond_config(preconditioner):
+    """-> (kind, local tuple, reduction); kind 0 none, 1 additive, 2 multiplicative."""
+    if preconditioner is None:
+        return 1, _local_from(None, 2), ReductionStrategy.Auto
+    if isinstance(preconditioner, (Preconditioner, int)) and not isinstance(preconditioner, bool):
+        p = Preconditioner(int(preconditioner))
+        if p == Preconditioner.Off:
+            return 0, _local_from(None, 2), ReductionStrategy.Auto
+        if p == Preconditioner.Additive:
+            return 1, _local_from(None, 2), ReductionStrategy.Auto
+        return 2, _local_from(None, 2), ReductionStrategy.Auto
+    if isinstance(preconditioner, AdditiveSchwarz):
+        return 1, _local_from(preconditioner.local_solver, 2), preconditioner.reduction
+    if isinstance(preconditioner, MultiplicativeSchwarz):
+        return 2, _local_from(preconditioner.local_solver, 2), ReductionStrategy.Auto
+    raise TypeError("preconditioner must be Preconditioner.Additive, Preconditioner.Multiplicative, "
+                    "Preconditioner.Off, AdditiveSchwarz(...), MultiplicativeSchwarz(...), or None")
+
+
+def _params(config, preconditioner):
+    L = N.load()
+    p = N.Params()
+    L.wa_params_default(C.byref(p))
+    if config is None:
+        config = CG()
+    if isinstance(config, LSMR):
+        raise ValueError("LSMR is not provided by within_amd (CG and GMRES are)")
+    if not isinstance(config, (CG, GMRES)):
+        raise TypeError("config must be CG, GMRES, or LSMR")
+    kind, loc, reduction = _precond_config(preconditioner)
+    if isinstance(config, CG) and kind == 2:
+        raise ValueError("CG requires a symmetric preconditioner; use 'additive' or switch to GMRES")
+    p.krylov = 1 if isinstance(config, GMRES) else 0
+    p.restart = config.restart if isinstance(config, GMRES) else 30
+    p.operator_repr = int(config.operator)
+    p.preconditioner = kind
+    p.tol = config.tol
+    p.maxiter = config.maxiter
+    p.max_refinements = config.max_refinements
+    p.precision = _PRECISIONS[config.precision]
+    p.block = 0 if config.block is None else int(config.block)
+    (p.local.ac_seed, p.local.ac_split, p.local.approx_schur, p.local.schur_seed, p.local.schur_split,
+     p.local.dense_threshold) = loc
+    return p, reduction
+
+
+def _categories(categories):
+    a = np.asarray(categories)
+    if a.ndim != 2:
+        raise ValueError("categories must be a 2-D array of shape (n_obs, n_factors)")
+    if a.dtype != np.uint32:
+        raise TypeError("categories must have dtype uint32")
+    if a.shape[0] > 1 and a.shape[1] > 1 and not a.flags.f_contiguous:
+        warnings.warn("categories array is not F-contiguous (column-major). Use np.asfortranarray(categories) "
+                      "for faster solves.", UserWarning, stacklevel=3)
+    return np.asfortranarray(a)
+
+
+def _vector(v, n, name):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+    if a.ndim != 1 or a.shape[0] != n:
+        raise ValueError(f"{name} must have shape ({n},), got {a.shape}")
+    return a
+
+
+class _Design:
+    def __init__(self, categories, weights):
+        L = N.load()
+        cats = _categories(categories)
+        self.n_obs, self.n_factors = cats.shape
+        w = None if weights is None else _vector(weights, self.n_obs, "weights")
+        h = C.c_void_p()
+        N.check(L.wa_design_create(N.ptr(cats), self.n_obs, self.n_factors, max(self.n_obs, 1), N.ptr(w),
+                                   C.byref(h)))
+        self.h = h
+        self.n_dofs = L.wa_design_n_dofs(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.load().wa_design_destroy(self.h)
+            self.h = None
+
+
+class FePreconditioner:
+    """The preconditioner of a built ``Solver`` (device resident).
+
+    ``apply`` runs the GPU additive / multiplicative Schwarz apply.  Passing it
+    to ``Solver(..., preconditioner=p)`` reuses its factors when the design
+    matches.  The byte serialisation of the reference (postcard) is not
+    provided; pickling raises TypeError.
+    """
+
+    def __init__(self, data=None, *, _solver=None):
+        if _solver is None:
+            raise TypeError("FePreconditioner objects are obtained from Solver.preconditioner()")
+        self._solver = _solver
+
+    def apply(self, x):
+        s = self._solver
+        x = _vector(x, s.n_dofs, "x")
+        y = np.zeros(s.n_dofs)
+        N.check(N.load().wa_precond_apply(s._h, N.ptr(x), N.ptr(y), 1, s.n_dofs))
+        return y
+
+    @property
+    def nrows(self):
+        return self._solver.n_dofs
+
+    @property
+    def ncols(self):
+        return self._solver.n_dofs
+
+    @property
+    def n_subdomains(self):
+        return int(self._solver.info()[0])
+
+    @property
+    def subdomain_inner_parallel_work(self):
+        return [0] * self.n_subdomains
+
+    def additive_schwarz_diagnostics(self):
+        if self._solver._kind != 1:
+            return None
+        return AdditiveSchwarzDiagnostics(self._solver.info(), self._solver._reduction)
+
+    def __reduce__(self):
+        raise TypeError("FePreconditioner serialisation is not provided by within_amd")
+
+    def __repr__(self):
+        return f"FePreconditioner(n_dofs={self.nrows}, n_subdomains={self.n_subdomains})"
+
+
+class Solver:
+    """Persistent solver: preconditioner built and uploaded once
+    (crates/within/src/solver.rs:82-447)."""
+
+    def __init__(self, categories, config=None, weights=None, preconditioner=None, *, _host_only=False):
+        t0 = time.perf_counter()
+        if isinstance(preconditioner, FePreconditioner):
+            base = preconditioner._solver
+            preconditioner = base._precond_arg
+        self._precond_arg = preconditioner
+        self._design = _Design(categories, weights)
+        p, self._reduction = _params(config, preconditioner)
+        p.host_only = 1 if _host_only else 0
+        self._kind = p.preconditioner
+        self._params = p
+        h = C.c_void_p()
+        N.check(N.load().wa_solver_create(self._design.h, C.byref(p), C.byref(h)))
+        self._h = h
+        self.time_setup = time.perf_counter() - t0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            N.load().wa_solver_destroy(self._h)
+            self._h = None
+
+    @property
+    def n_dofs(self):
+        return self._design.n_dofs
+
+    @property
+    def n_obs(self):
+        return self._design.n_obs
+
+    def info(self):
+        out = np.zeros(16, dtype=np.int64)
+        N.check(N.load().wa_solver_info(self._h, N.ptr(out)))
+        return out
+
+    def preconditioner(self):
+        if self._kind == 0:
+            return None
+        return FePreconditioner(_solver=self)
+
+    def _solve_cols(self, Y):
+        n, m = self.n_obs, self.n_dofs
+        Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))
+        if Y.ndim != 2 or Y.shape[0] != n:
+            raise ValueError(f"Y must have shape ({n}, k), got {Y.shape}")
+        k = Y.shape[1]
+        X = np.zeros((m, k), order="F")
+        D = np.zeros((n, k), order="F")
+        stats = (N.RhsStats * max(k, 1))()
+        t0 = time.perf_counter()
+        N.check(N.load().wa_solve_batch(self._h, N.ptr(Y), k, n, N.ptr(X), m, N.ptr(D), n, stats))
+        return X, D, stats, time.perf_counter() - t0
+
+    def solve(self, y):
+        t0 = time.perf_counter()
+        y = _vector(y, self.n_obs, "y")
+        X, D, st, _ = self._solve_cols(y[:, None])
+        return SolveResult(x=X[:, 0], demeaned=D[:, 0], converged=bool(st[0].converged),
+                           iterations=int(st[0].iterations), residual=float(st[0].residual),
+                           time_total=time.perf_counter() - t0, time_setup=0.0, time_solve=float(st[0].time_solve))
+
+    def solve_batch(self, Y):
+        t0 = time.perf_counter()
+        X, D, st, _ = self._solve_cols(Y)
+        k = X.shape[1]
+        return BatchSolveResult(x=X, demeaned=D, converged=[bool(st[i].converged) for i in range(k)],
+                                iterations=[int(st[i].iterations) for i in range(k)],
+                                residual=[float(st[i].residual) for i in range(k)],
+                                time_solve=[float(st[i].time_solve) for i in range(k)],
+                                time_total=time.perf_counter() - t0)
+
+
+def solve(categories, y, config=None, weights=None, preconditioner=None):
+    """One-shot solve (crates/within-py/src/lib.rs:750-775)."""
+    t0 = time.perf_counter()
+    s = Solver(categories, config=config, weights=weights, preconditioner=preconditioner)
+    r = s.solve(y)
+    r.time_setup = s.time_setup
+    r.time_total = time.perf_counter() - t0
+    return r
+
+
+def solve_batch(categories, Y, config=None, weights=None, preconditioner=None):
+    """One-shot batch solve (crates/within-py/src/lib.rs:781-815)."""
+    t0 = time.perf_counter()
+    s = Solver(categories, config=config, weights=weights, preconditioner=preconditioner)
+    r = s.solve_batch(Y)
+    r.time_total = time.perf_counter() - t0
+    return r

@@ -1,0 +1,111 @@
+// kernels.hpp -- device-side views of the uploaded plan and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace wa {
+
+constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
+constexpr int kRedBlocks = 1024; // fixed reduction grid (deterministic sums)
+
+struct DevSched {
+  const Item* items = nullptr;
+  int64_t n_items = 0;
+  const Fixup* fix = nullptr;
+  int64_t n_fix = 0;
+  int64_t n_slots = 0;
+};
+
+// one level-sorted factor pass (plan.hpp FactorPass) in precision T
+template <typename T>
+struct DevPass {
+  const uint32_t* row_ptr = nullptr;
+  const uint32_t* other = nullptr;  // element-major (qm per element)
+  const uint32_t* obs = nullptr;
+  const T* ew = nullptr;            // element weights (null = unit)
+  const T* diag = nullptr;
+  int qm = 0;
+  uint32_t off = 0;
+  int64_t levels = 0, n_elem = 0;
+  DevSched wide, narrow;
+};
+
+template <typename T>
+struct DevCsr {
+  const uint32_t* ptr = nullptr;
+  const uint32_t* idx = nullptr;
+  const T* val = nullptr;
+  int64_t nrows = 0, nnz = 0;
+  DevSched wide, narrow;
+};
+
+template <typename T>
+struct DevPrecond {
+  int64_t n_loc = 0, n_red = 0, n_sub = 0, n_elim = 0, n_levels = 0, n_core_rows = 0;
+  const uint32_t *loc_gidx, *loc_ptr, *red_ptr, *loc_sub, *red_sub, *red_loc;
+  const T* loc_scale;
+  const uint32_t *elim_loc, *elim_ptr, *elim_idx;
+  const T *elim_inv, *elim_val;
+  const uint32_t *keep_ptr, *keep_idx;
+  const T *keep_val, *keep_rowsum;
+  const uint32_t *lvl_rows, *fin_ptr, *fin_idx, *bout_ptr, *bout_idx;
+  const T *fin_val, *bout_val, *inv_pivot;
+  const uint32_t *core_ptr, *core_idx, *core_row_core;
+  const uint64_t* core_mat_off;
+  const T* core_mat;
+  const uint32_t *dof_ptr, *dof_loc;
+  DevSched sched_loc, sched_red;
+  std::vector<uint32_t> h_lvl_ptr;  // host copy of level boundaries
+};
+
+// scratch for one preconditioner apply (kp columns)
+template <typename T>
+struct PrecondWork {
+  T *B = nullptr, *S = nullptr, *Rr = nullptr, *C = nullptr;
+  T *sum1 = nullptr, *sum2 = nullptr, *sum3 = nullptr, *part = nullptr;
+};
+
+// per-column Krylov state (FP64 scalars)
+struct ColState {
+  double bnorm, rnorm, tol, rz, rz0, alpha, beta, pad;
+  int active, converged, iters, pad2;
+};
+
+template <typename T>
+void launch_pull(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp);
+void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const uint32_t* obs, const double* w,
+                        const double* V, double* out, double* part, int kp);
+void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
+                   const double* X, const double* Y, double* E, int kp);
+void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp);
+void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld);
+template <typename T>
+void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out);
+template <typename T>
+void launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m,
+                      int kp, double* part, double* out);
+template <typename T>
+void launch_xpby(hipStream_t st, T* P, const T* Z, const double* beta, const int* mask, int64_t m, int kp);
+template <typename S, typename D>
+void launch_cast_scale(hipStream_t st, const S* src, D* dst, const double* scale, int64_t m, int kp);
+template <typename S>
+void launch_axpy_acc(hipStream_t st, double* X, const S* Dl, const double* scale, const int* mask, int64_t m,
+                     int kp);
+void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const double* rz, const double* tol, int k,
+                     int kp);
+void launch_cg_alpha(hipStream_t st, ColState* s, const double* pap, double* alpha, int itn, int kp);
+void launch_cg_check(hipStream_t st, ColState* s, const double* rr, int itn, int kp);
+void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* beta, int* mask, int* n_active,
+                    int itn, int kp);
+template <typename T>
+void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
+                    int64_t m);
+template <typename T>
+void launch_spmm(hipStream_t st, const DevCsr<T>& g, const T* X, T* Y, T* part, int kp);
+
+}  // namespace wa

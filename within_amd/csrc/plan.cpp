@@ -1,0 +1,850 @@
+// plan.cpp -- host-side setup (see plan.hpp).  Built once per design; its
+// arrays are uploaded once and read by every GPU apply.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+namespace wa {
+
+enum { E_EMPTY = 1, E_OVERFLOW = 4, E_SINGULAR = 5, E_LOCAL = 6, E_INVALID = 9 };
+
+// ---------------------------------------------------------------------------
+// design -- WeightedDesign::from_store (crates/within/src/domain.rs:133-156)
+// ---------------------------------------------------------------------------
+Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* weights) {
+  if (n_obs <= 0) throw Error(E_EMPTY, "EmptyObservations: no observations");
+  if (Q <= 0) throw Error(E_INVALID, "at least one factor is required");
+  if (ld < n_obs) throw Error(E_INVALID, "leading dimension smaller than n_obs");
+  if (n_obs >= (i64)UINT32_MAX) throw Error(E_OVERFLOW, "Overflow: n_obs exceeds u32 indexing");
+  Design d;
+  d.n_obs = n_obs;
+  d.Q = Q;
+  d.codes.resize(Q);
+  i64 off = 0;
+  for (int q = 0; q < Q; ++q) {
+    const u32* col = codes + (i64)q * ld;
+    d.codes[q].assign(col, col + n_obs);
+    u32 mx = 0;
+    for (i64 i = 0; i < n_obs; ++i) mx = std::max(mx, col[i]);
+    d.levels.push_back((i64)mx + 1);
+    d.offset.push_back(off);
+    off += (i64)mx + 1;
+  }
+  if (off >= (i64)UINT32_MAX) throw Error(E_OVERFLOW, "Overflow: n_dofs exceeds u32 indexing");
+  d.n_dofs = off;
+  if (weights) d.weights.assign(weights, weights + n_obs);
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// segmented-reduction work items
+// ---------------------------------------------------------------------------
+Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows) {
+  Schedule s;
+  u32 nrows = (u32)row_ptr.size() - 1;
+  u32 r = 0;
+  while (r < nrows) {
+    u32 len = row_ptr[r + 1] - row_ptr[r];
+    if (len > chunk) {
+      u32 slot0 = (u32)s.n_slots;
+      for (u32 e0 = row_ptr[r]; e0 < row_ptr[r + 1]; e0 += chunk) {
+        u32 e1 = std::min(e0 + chunk, row_ptr[r + 1]);
+        s.items.push_back({r, r + 1, e0, e1, (int32_t)s.n_slots++, 0});
+      }
+      s.fixups.push_back({r, slot0, (u32)s.n_slots, 0});
+      ++r;
+      continue;
+    }
+    u32 r1 = r, tot = 0;
+    while (r1 < nrows && r1 - r < max_rows) {
+      u32 l = row_ptr[r1 + 1] - row_ptr[r1];
+      if (l > chunk || (r1 > r && tot + l > chunk)) break;
+      tot += l;
+      ++r1;
+    }
+    s.items.push_back({r, r1, row_ptr[r], row_ptr[r1], -1, 0});
+    r = r1;
+  }
+  return s;
+}
+
+// stable counting sort of `idx` by key[idx]
+static void counting_sort(std::vector<u32>& idx, const u32* key, i64 nkeys) {
+  std::vector<u32> cnt(nkeys + 1, 0);
+  for (u32 i : idx) cnt[key[i] + 1]++;
+  for (i64 k = 0; k < nkeys; ++k) cnt[k + 1] += cnt[k];
+  std::vector<u32> out(idx.size());
+  for (u32 i : idx) out[cnt[key[i]]++] = i;
+  idx.swap(out);
+}
+
+static FactorPass build_pass(const Design& d, int q) {
+  FactorPass fp;
+  fp.q = q;
+  i64 n = d.n_obs;
+  std::vector<u32> idx(n);
+  std::iota(idx.begin(), idx.end(), 0u);
+  // secondary key: the largest other factor (gather locality inside a level)
+  int sec = -1;
+  for (int r = 0; r < d.Q; ++r)
+    if (r != q && (sec < 0 || d.levels[r] > d.levels[sec])) sec = r;
+  if (sec >= 0) counting_sort(idx, d.codes[sec].data(), d.levels[sec]);
+  counting_sort(idx, d.codes[q].data(), d.levels[q]);
+  fp.obs = idx;
+  fp.row_ptr.assign(d.levels[q] + 1, 0);
+  for (i64 i = 0; i < n; ++i) fp.row_ptr[d.codes[q][i] + 1]++;
+  for (i64 j = 0; j < d.levels[q]; ++j) fp.row_ptr[j + 1] += fp.row_ptr[j];
+  int Qm = d.Q - 1;
+  fp.other.resize((size_t)n * Qm);
+  for (i64 t = 0; t < n; ++t) {
+    u32 o = idx[t];
+    int s = 0;
+    for (int r = 0; r < d.Q; ++r)
+      if (r != q) fp.other[(size_t)t * Qm + s++] = (u32)(d.offset[r] + d.codes[r][o]);
+  }
+  if (!d.weights.empty()) {
+    fp.w.resize(n);
+    for (i64 t = 0; t < n; ++t) fp.w[t] = d.weights[idx[t]];
+  }
+  fp.diag.assign(d.levels[q], 0.0);
+  for (i64 j = 0; j < d.levels[q]; ++j)
+    for (u32 t = fp.row_ptr[j]; t < fp.row_ptr[j + 1]; ++t) fp.diag[j] += d.w(idx[t]);
+  fp.sched_wide = make_schedule(fp.row_ptr, 256, 64);
+  fp.sched_narrow = make_schedule(fp.row_ptr, 64, 64);
+  return fp;
+}
+
+// ---------------------------------------------------------------------------
+// CSR, cross-tabs, components, partition weights
+//   gramian/cross_tab.rs:75-145, :381-501, :514-664; factor_pairs.rs:63-304
+// ---------------------------------------------------------------------------
+Csr Csr::transpose() const {
+  Csr t;
+  t.nrows = ncols;
+  t.ncols = nrows;
+  t.indptr.assign(ncols + 1, 0);
+  for (u32 j : indices) t.indptr[j + 1]++;
+  for (i64 j = 0; j < ncols; ++j) t.indptr[j + 1] += t.indptr[j];
+  t.indices.resize(indices.size());
+  t.data.resize(data.size());
+  std::vector<u32> cur(t.indptr.begin(), t.indptr.end() - 1);
+  for (i64 i = 0; i < nrows; ++i)
+    for (u32 p = indptr[i]; p < indptr[i + 1]; ++p) {
+      u32 at = cur[indices[p]]++;
+      t.indices[at] = (u32)i;
+      t.data[at] = data[p];
+    }
+  return t;
+}
+
+static CrossTab cross_tab(const Design& d, int q, int r, const std::vector<u32>& qmap,
+                          const std::vector<u32>& rmap, i64 nq, i64 nr) {
+  CrossTab ct;
+  ct.diag_q.assign(nq, 0.0);
+  ct.diag_r.assign(nr, 0.0);
+  const u32* cq = d.codes[q].data();
+  const u32* cr = d.codes[r].data();
+  std::vector<u32> ptr(nq + 1, 0);
+  for (i64 i = 0; i < d.n_obs; ++i) {
+    u32 a = qmap[cq[i]], b = rmap[cr[i]];
+    if (a == UINT32_MAX || b == UINT32_MAX) continue;
+    double w = d.w(i);
+    ct.diag_q[a] += w;
+    ct.diag_r[b] += w;
+    ptr[a + 1]++;
+  }
+  for (i64 j = 0; j < nq; ++j) ptr[j + 1] += ptr[j];
+  std::vector<u32> col(ptr[nq]);
+  std::vector<double> val(ptr[nq]);
+  {
+    std::vector<u32> cur(ptr.begin(), ptr.end() - 1);
+    for (i64 i = 0; i < d.n_obs; ++i) {
+      u32 a = qmap[cq[i]], b = rmap[cr[i]];
+      if (a == UINT32_MAX || b == UINT32_MAX) continue;
+      u32 at = cur[a]++;
+      col[at] = b;
+      val[at] = d.w(i);
+    }
+  }
+  Csr& c = ct.c;
+  c.nrows = nq;
+  c.ncols = nr;
+  c.indptr.assign(nq + 1, 0);
+  std::vector<double> acc(nr, 0.0);
+  std::vector<char> hit(nr, 0);
+  std::vector<u32> cols;
+  for (i64 row = 0; row < nq; ++row) {
+    for (u32 p = ptr[row]; p < ptr[row + 1]; ++p) {
+      if (!hit[col[p]]) { hit[col[p]] = 1; cols.push_back(col[p]); }
+      acc[col[p]] += val[p];
+    }
+    std::sort(cols.begin(), cols.end());
+    for (u32 k : cols) {
+      if (acc[k] != 0.0) { c.indices.push_back(k); c.data.push_back(acc[k]); }
+      acc[k] = 0.0;
+      hit[k] = 0;
+    }
+    cols.clear();
+    c.indptr[row + 1] = (u32)c.indices.size();
+  }
+  ct.ct = c.transpose();
+  return ct;
+}
+
+std::vector<Subdomain> build_subdomains(const Design& d) {
+  int Q = d.Q;
+  std::vector<std::vector<char>> active(Q);
+  for (int q = 0; q < Q; ++q) {
+    active[q].assign(d.levels[q], 0);
+    for (i64 i = 0; i < d.n_obs; ++i) active[q][d.codes[q][i]] = 1;
+  }
+  std::vector<std::pair<int, int>> pairs;
+  for (int q = 0; q < Q; ++q)
+    for (int r = q + 1; r < Q; ++r) pairs.push_back({q, r});
+  std::vector<std::vector<Subdomain>> per(pairs.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t pi = 0; pi < pairs.size(); ++pi) {
+    int q = pairs[pi].first, r = pairs[pi].second;
+    std::vector<u32> qmap(d.levels[q], UINT32_MAX), rmap(d.levels[r], UINT32_MAX), l2g;
+    i64 nq = 0, nr = 0;
+    for (i64 j = 0; j < d.levels[q]; ++j)
+      if (active[q][j]) { qmap[j] = (u32)nq++; l2g.push_back((u32)(d.offset[q] + j)); }
+    for (i64 j = 0; j < d.levels[r]; ++j)
+      if (active[r][j]) { rmap[j] = (u32)nr++; l2g.push_back((u32)(d.offset[r] + j)); }
+    if (nq == 0 || nr == 0) continue;
+    CrossTab full = cross_tab(d, q, r, qmap, rmap, nq, nr);
+    // bipartite components by DFS (q nodes 0..nq, r nodes nq..)
+    i64 n = nq + nr;
+    std::vector<char> seen(n, 0);
+    std::vector<i64> stack;
+    std::vector<std::pair<std::vector<u32>, std::vector<u32>>> comps;
+    for (i64 s0 = 0; s0 < n; ++s0) {
+      if (seen[s0]) continue;
+      seen[s0] = 1;
+      stack.push_back(s0);
+      std::vector<u32> qi, ri;
+      while (!stack.empty()) {
+        i64 v = stack.back();
+        stack.pop_back();
+        if (v < nq) {
+          qi.push_back((u32)v);
+          for (u32 p = full.c.indptr[v]; p < full.c.indptr[v + 1]; ++p) {
+            i64 g = nq + full.c.indices[p];
+            if (!seen[g]) { seen[g] = 1; stack.push_back(g); }
+          }
+        } else {
+          ri.push_back((u32)(v - nq));
+          for (u32 p = full.ct.indptr[v - nq]; p < full.ct.indptr[v - nq + 1]; ++p) {
+            i64 g = full.ct.indices[p];
+            if (!seen[g]) { seen[g] = 1; stack.push_back(g); }
+          }
+        }
+      }
+      std::sort(qi.begin(), qi.end());
+      std::sort(ri.begin(), ri.end());
+      comps.push_back({std::move(qi), std::move(ri)});
+    }
+    for (auto& cm : comps) {
+      Subdomain s;
+      s.q = q;
+      s.r = r;
+      if (comps.size() == 1) {
+        s.ct = full;
+      } else {
+        std::vector<u32> rre(nr, UINT32_MAX);
+        for (size_t k = 0; k < cm.second.size(); ++k) rre[cm.second[k]] = (u32)k;
+        CrossTab& ct = s.ct;
+        ct.c.nrows = cm.first.size();
+        ct.c.ncols = cm.second.size();
+        ct.c.indptr.assign(cm.first.size() + 1, 0);
+        for (size_t k = 0; k < cm.first.size(); ++k) {
+          u32 o = cm.first[k];
+          ct.diag_q.push_back(full.diag_q[o]);
+          for (u32 p = full.c.indptr[o]; p < full.c.indptr[o + 1]; ++p)
+            if (rre[full.c.indices[p]] != UINT32_MAX) {
+              ct.c.indices.push_back(rre[full.c.indices[p]]);
+              ct.c.data.push_back(full.c.data[p]);
+            }
+          ct.c.indptr[k + 1] = (u32)ct.c.indices.size();
+        }
+        for (u32 o : cm.second) ct.diag_r.push_back(full.diag_r[o]);
+        ct.ct = ct.c.transpose();
+      }
+      for (u32 i : cm.first) s.gidx.push_back(l2g[i]);
+      for (u32 i : cm.second) s.gidx.push_back(l2g[nq + i]);
+      per[pi].push_back(std::move(s));
+    }
+  }
+  std::vector<Subdomain> subs;
+  for (auto& v : per)
+    for (auto& s : v) subs.push_back(std::move(s));
+  std::vector<u32> cnt(d.n_dofs, 0);
+  for (auto& s : subs)
+    for (u32 g : s.gidx) cnt[g]++;
+  for (auto& s : subs) {
+    bool uniq = true;
+    for (u32 g : s.gidx) uniq = uniq && cnt[g] <= 1;
+    if (!uniq) {
+      s.pou.resize(s.gidx.size());
+      for (size_t k = 0; k < s.gidx.size(); ++k) s.pou[k] = 1.0 / std::sqrt((double)cnt[s.gidx[k]]);
+    }
+  }
+  return subs;
+}
+
+// ---------------------------------------------------------------------------
+// clique trees + approximate Cholesky (GKS 2023 clique-tree sampling,
+// arXiv:2303.00709, as called at operator/schur_complement.rs:128-139)
+//   neighbours split into `split` equal copies, stably sorted by (weight, id);
+//   for i < len-1: rem = weight after i; j > i drawn with P ~ w_j;
+//   edge (u_i, u_j) with weight w_i * rem / total (total = diag when split == 1,
+//   else the weight sum); copies of one neighbour give no edge.
+//   RNG: splitmix64 seeded with the star seed, u = (x >> 11) * 2^-53.
+// ---------------------------------------------------------------------------
+struct Edge { u32 lo, hi; double w; };
+
+static inline uint64_t mix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static void sample_clique_tree(const std::vector<std::pair<u32, double>>& nbrs, double diag,
+                               uint64_t seed, int split, std::vector<Edge>& out) {
+  if (split < 1) split = 1;
+  std::vector<std::pair<u32, double>> e;
+  e.reserve(nbrs.size() * split);
+  for (const auto& p : nbrs)
+    for (int c = 0; c < split; ++c) e.push_back({p.first, split == 1 ? p.second : p.second / split});
+  std::stable_sort(e.begin(), e.end(), [](const std::pair<u32, double>& a, const std::pair<u32, double>& b) {
+    return a.second != b.second ? a.second < b.second : a.first < b.first;
+  });
+  size_t len = e.size();
+  if (len < 2) return;
+  double total = diag;
+  if (split > 1) {
+    total = 0.0;
+    for (const auto& p : e) total += p.second;
+  }
+  std::vector<double> suf(len + 1, 0.0);
+  for (size_t i = len; i-- > 0;) suf[i] = suf[i + 1] + e[i].second;
+  uint64_t st = seed;
+  for (size_t i = 0; i + 1 < len; ++i) {
+    double rem = suf[i + 1];
+    double target = (double)(mix64(st) >> 11) * 0x1.0p-53 * rem;
+    size_t lo = i + 1, hi = len - 1;
+    while (lo < hi) {
+      size_t mid = (lo + hi) / 2;
+      if (suf[i + 1] - suf[mid + 1] > target) hi = mid; else lo = mid + 1;
+    }
+    double w = e[i].second * rem / total;
+    u32 a = e[i].first, b = e[lo].first;
+    if (a == b || !(w > 0.0)) continue;
+    out.push_back({std::min(a, b), std::max(a, b), w});
+  }
+}
+
+// Cholesky of the anchored minor (drop the last index) of a dense n x n
+// Laplacian-like matrix, then its inverse T'T with T = L^-1 (sums over k
+// ascending).  Empty result = minor not positive definite.
+std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n) {
+  std::vector<double> inv(n * n, 0.0);
+  i64 m = n - 1;
+  if (m <= 0) return inv;
+  std::vector<double> l(m * m);
+  for (i64 i = 0; i < m; ++i)
+    for (i64 j = 0; j < m; ++j) l[i * m + j] = a[i * n + j];
+  for (i64 j = 0; j < m; ++j) {
+    double s = l[j * m + j];
+    for (i64 k = 0; k < j; ++k) s -= l[j * m + k] * l[j * m + k];
+    if (!(s > 0.0)) return {};
+    double ljj = std::sqrt(s);
+    l[j * m + j] = ljj;
+#pragma omp parallel for schedule(static) if (m - j > 512)
+    for (i64 i = j + 1; i < m; ++i) {
+      double t = l[i * m + j];
+      for (i64 k = 0; k < j; ++k) t -= l[i * m + k] * l[j * m + k];
+      l[i * m + j] = t / ljj;
+    }
+  }
+  std::vector<double> T(m * m, 0.0);
+#pragma omp parallel for schedule(dynamic, 8) if (m > 256)
+  for (i64 j = 0; j < m; ++j)
+    for (i64 i = j; i < m; ++i) {
+      double s = i == j ? 1.0 : 0.0;
+      for (i64 k = j; k < i; ++k) s -= l[i * m + k] * T[j * m + k];
+      T[j * m + i] = s / l[i * m + i];
+    }
+#pragma omp parallel for schedule(dynamic, 8) if (m > 256)
+  for (i64 i = 0; i < m; ++i)
+    for (i64 j = 0; j < m; ++j) {
+      double s = 0.0;
+      for (i64 k = std::max(i, j); k < m; ++k) s += T[i * m + k] * T[j * m + k];
+      inv[i * n + j] = s;
+    }
+  return inv;
+}
+
+static inline uint64_t round_key(uint64_t seed, uint64_t round, uint64_t v) {
+  uint64_t s = seed ^ (round * 0xD1B54A32D192ED03ULL) ^ (v * 0x8CB92BA72F3D8DD7ULL);
+  return mix64(s);
+}
+
+// Rounds of independent eliminations on a Laplacian (off-diagonals = -w):
+// v is eliminated in round t when (degree, key(t, v), v) is smaller than that
+// of every remaining neighbour; fill = clique tree of its neighbourhood (seed
+// + v), merged after the round in (v ascending, emission) order.  Once at most
+// `dense_core` vertices remain they are factored exactly (anchored at the
+// last one); a core whose anchored minor is singular falls back to rounds.
+static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 dense_core) {
+  Factor f;
+  i64 n = lap.nrows;
+  f.n = n;
+  std::vector<std::map<u32, double>> adj(n);
+  for (i64 i = 0; i < n; ++i)
+    for (u32 p = lap.indptr[i]; p < lap.indptr[i + 1]; ++p) {
+      double w = -lap.data[p];
+      if ((i64)lap.indices[p] != i && w > 0.0) adj[i][lap.indices[p]] = w;
+    }
+  f.pivot.assign(n, 0.0);
+  std::vector<std::vector<std::pair<u32, double>>> cols(n);
+  std::vector<char> live(n, 1);
+  std::vector<uint64_t> key(n);
+  i64 left = n, cut = dense_core > 1 ? dense_core : 0;
+  uint64_t round = 0;
+  f.round_ptr.push_back(0);
+  std::vector<u32> pick;
+  while (left > 0) {
+    if (left <= cut) {
+      std::vector<u32> core;
+      for (i64 v = 0; v < n; ++v)
+        if (live[v]) core.push_back((u32)v);
+      i64 c = (i64)core.size();
+      std::vector<i64> at(n, -1);
+      for (i64 i = 0; i < c; ++i) at[core[i]] = i;
+      std::vector<double> a(c * c, 0.0);
+      for (i64 i = 0; i < c; ++i) {
+        double sum = 0.0;
+        for (const auto& kv : adj[core[i]]) {
+          sum += kv.second;
+          a[i * c + at[kv.first]] = -kv.second;
+        }
+        a[i * c + i] = sum;
+      }
+      std::vector<double> inv = anchored_inverse(a, c);
+      if (!inv.empty()) {
+        f.core = std::move(core);
+        f.core_inv = std::move(inv);
+        break;
+      }
+      cut = 0;
+    }
+    for (i64 v = 0; v < n; ++v)
+      if (live[v]) key[v] = round_key(seed, round, (uint64_t)v);
+    pick.clear();
+    for (i64 v = 0; v < n; ++v) {
+      if (!live[v]) continue;
+      size_t dv = adj[v].size();
+      bool mine = true;
+      for (const auto& kv : adj[v]) {
+        size_t du = adj[kv.first].size();
+        if (du < dv || (du == dv && (key[kv.first] < key[v] || (key[kv.first] == key[v] && (i64)kv.first < v)))) {
+          mine = false;
+          break;
+        }
+      }
+      if (mine) pick.push_back((u32)v);
+    }
+    std::vector<Edge> fill;
+    std::vector<std::pair<u32, double>> nb;
+    for (u32 v : pick) {
+      nb.assign(adj[v].begin(), adj[v].end());
+      double dv = 0.0;
+      for (const auto& p : nb) dv += p.second;
+      f.pivot[v] = dv;
+      if (nb.size() >= 2) sample_clique_tree(nb, dv, seed + (uint64_t)v, split, fill);
+      for (const auto& p : nb) adj[p.first].erase(v);
+      adj[v].clear();
+      cols[v] = nb;
+      live[v] = 0;
+      f.order.push_back(v);
+    }
+    for (const Edge& e : fill) {
+      adj[e.lo][e.hi] += e.w;
+      adj[e.hi][e.lo] += e.w;
+    }
+    left -= (i64)pick.size();
+    f.round_ptr.push_back((u32)f.order.size());
+    ++round;
+  }
+  f.col_ptr.assign(n + 1, 0);
+  for (i64 v = 0; v < n; ++v) f.col_ptr[v + 1] = f.col_ptr[v] + (u32)cols[v].size();
+  f.col_idx.reserve(f.col_ptr[n]);
+  f.col_w.reserve(f.col_ptr[n]);
+  for (i64 v = 0; v < n; ++v)
+    for (const auto& p : cols[v]) { f.col_idx.push_back(p.first); f.col_w.push_back(p.second); }
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// Schur complements (operator/schur_complement.rs:170-216, :331-447, :534-586,
+// :697-711) and the local-solver build (operator/schwarz.rs:207-303)
+// ---------------------------------------------------------------------------
+struct ElimView {
+  bool eq;
+  i64 n_keep, n_elim;
+  const std::vector<double>* dkeep;
+  const std::vector<double>* delim;
+  const Csr* k2e;
+  const Csr* e2k;
+};
+
+static ElimView elim_view(const CrossTab& ct, std::vector<double>& inv) {
+  ElimView e;
+  e.eq = ct.n_q() >= ct.n_r();
+  e.n_keep = e.eq ? ct.n_r() : ct.n_q();
+  e.n_elim = e.eq ? ct.n_q() : ct.n_r();
+  e.delim = e.eq ? &ct.diag_q : &ct.diag_r;
+  e.dkeep = e.eq ? &ct.diag_r : &ct.diag_q;
+  e.k2e = e.eq ? &ct.ct : &ct.c;
+  e.e2k = e.eq ? &ct.c : &ct.ct;
+  inv.clear();
+  for (i64 i = 0; i < e.n_elim; ++i) {
+    double v = (*e.delim)[i];
+    if (!(v > 0.0))
+      throw Error(E_SINGULAR, "SingularDiagonal: eliminated block " + std::string(e.eq ? "q" : "r") +
+                                  " has a zero diagonal at index " + std::to_string(i));
+    inv.push_back(1.0 / v);
+  }
+  return e;
+}
+
+static Csr schur_exact(const ElimView& e, const std::vector<double>& inv) {
+  Csr S;
+  S.nrows = S.ncols = e.n_keep;
+  S.indptr.assign(e.n_keep + 1, 0);
+  std::vector<double> acc(e.n_keep, 0.0);
+  std::vector<char> hit(e.n_keep, 0);
+  std::vector<u32> cols;
+  for (i64 i = 0; i < e.n_keep; ++i) {
+    acc[i] = (*e.dkeep)[i];
+    hit[i] = 1;
+    cols.push_back((u32)i);
+    for (u32 p = e.k2e->indptr[i]; p < e.k2e->indptr[i + 1]; ++p) {
+      u32 k = e.k2e->indices[p];
+      double sc = e.k2e->data[p] * inv[k];
+      for (u32 q = e.e2k->indptr[k]; q < e.e2k->indptr[k + 1]; ++q) {
+        u32 j = e.e2k->indices[q];
+        if (!hit[j]) { hit[j] = 1; cols.push_back(j); }
+        acc[j] -= sc * e.e2k->data[q];
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (u32 j : cols) {
+      if (acc[j] != 0.0 || (i64)j == i) { S.indices.push_back(j); S.data.push_back(acc[j]); }
+      acc[j] = 0.0;
+      hit[j] = 0;
+    }
+    cols.clear();
+    S.indptr[i + 1] = (u32)S.indices.size();
+  }
+  return S;
+}
+
+static Csr schur_sampled(const ElimView& e, uint64_t seed, int split) {
+  std::vector<Edge> edges;
+  std::vector<std::pair<u32, double>> nb;
+  for (i64 k = 0; k < e.n_elim; ++k) {
+    u32 b = e.e2k->indptr[k], en = e.e2k->indptr[k + 1];
+    if (en - b <= 1) continue;
+    nb.clear();
+    for (u32 p = b; p < en; ++p) nb.push_back({e.e2k->indices[p], e.e2k->data[p]});
+    sample_clique_tree(nb, (*e.delim)[k], seed + (uint64_t)k, split, edges);
+  }
+  std::stable_sort(edges.begin(), edges.end(), [](const Edge& a, const Edge& b) {
+    return a.lo != b.lo ? a.lo < b.lo : a.hi < b.hi;
+  });
+  std::vector<Edge> m;
+  for (const Edge& x : edges) {
+    if (!m.empty() && m.back().lo == x.lo && m.back().hi == x.hi) m.back().w += x.w;
+    else m.push_back(x);
+  }
+  i64 n = e.n_keep;
+  std::vector<u32> lo_cnt(n, 0), up_cnt(n, 0);
+  std::vector<double> dg(n, 0.0);
+  for (const Edge& x : m) {
+    up_cnt[x.lo]++;
+    lo_cnt[x.hi]++;
+    dg[x.lo] += x.w;
+    dg[x.hi] += x.w;
+  }
+  Csr S;
+  S.nrows = S.ncols = n;
+  S.indptr.assign(n + 1, 0);
+  for (i64 i = 0; i < n; ++i) S.indptr[i + 1] = S.indptr[i] + lo_cnt[i] + 1 + up_cnt[i];
+  S.indices.resize(S.indptr[n]);
+  S.data.resize(S.indptr[n]);
+  std::vector<u32> lc(n), uc(n);
+  for (i64 i = 0; i < n; ++i) {
+    lc[i] = S.indptr[i];
+    u32 dp = S.indptr[i] + lo_cnt[i];
+    S.indices[dp] = (u32)i;
+    S.data[dp] = dg[i];
+    uc[i] = dp + 1;
+  }
+  for (const Edge& x : m) {
+    u32 p = uc[x.lo]++;
+    S.indices[p] = x.hi;
+    S.data[p] = -x.w;
+    p = lc[x.hi]++;
+    S.indices[p] = x.lo;
+    S.data[p] = -x.w;
+  }
+  return S;
+}
+
+void build_local_solver(Subdomain& s, const LocalConfig& cfg) {
+  ElimView e = elim_view(s.ct, s.inv_diag_elim);
+  s.eliminate_q = e.eq;
+  if (cfg.dense_threshold > 0 && e.n_keep <= cfg.dense_threshold) {
+    // dense fast path: exact Schur, anchored at the last kept index
+    Csr S = schur_exact(e, s.inv_diag_elim);
+    i64 c = e.n_keep;
+    std::vector<double> a(c * c, 0.0);
+    for (i64 i = 0; i < c; ++i)
+      for (u32 p = S.indptr[i]; p < S.indptr[i + 1]; ++p) a[i * c + S.indices[p]] = S.data[p];
+    std::vector<double> inv = anchored_inverse(a, c);
+    if (!inv.empty()) {
+      s.dense = true;
+      s.f = Factor();
+      s.f.n = c;
+      s.f.round_ptr.push_back(0);
+      s.f.pivot.assign(c, 0.0);
+      s.f.col_ptr.assign(c + 1, 0);
+      for (i64 i = 0; i < c; ++i) s.f.core.push_back((u32)i);
+      s.f.core_inv = std::move(inv);
+      return;
+    }
+  }
+  Csr S = cfg.approx_schur ? schur_sampled(e, cfg.schur_seed, cfg.schur_split) : schur_exact(e, s.inv_diag_elim);
+  s.f = factor_laplacian(S, cfg.ac_seed, cfg.ac_split, cfg.dense_core);
+}
+
+// ---------------------------------------------------------------------------
+// explicit Gramian -- gramian/explicit.rs:25-108, :307-409 (sorted rows, diag
+// kept, zero off-diagonals dropped)
+// ---------------------------------------------------------------------------
+Csr build_gramian(const Design& d) {
+  Csr g;
+  g.nrows = g.ncols = d.n_dofs;
+  g.indptr.assign(d.n_dofs + 1, 0);
+  std::vector<std::vector<std::pair<u32, double>>> rows(d.n_dofs);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < d.Q; ++q) {
+    std::vector<u32> ptr(d.levels[q] + 1, 0);
+    for (i64 i = 0; i < d.n_obs; ++i) ptr[d.codes[q][i] + 1]++;
+    for (i64 j = 0; j < d.levels[q]; ++j) ptr[j + 1] += ptr[j];
+    std::vector<u32> ob(d.n_obs);
+    {
+      std::vector<u32> cur(ptr.begin(), ptr.end() - 1);
+      for (i64 i = 0; i < d.n_obs; ++i) ob[cur[d.codes[q][i]]++] = (u32)i;
+    }
+    std::map<u32, double> acc;
+    for (i64 j = 0; j < d.levels[q]; ++j) {
+      if (ptr[j] == ptr[j + 1]) continue;
+      u32 self = (u32)(d.offset[q] + j);
+      acc.clear();
+      for (u32 t = ptr[j]; t < ptr[j + 1]; ++t) {
+        u32 i = ob[t];
+        double w = d.w(i);
+        acc[self] += w;
+        for (int r = 0; r < d.Q; ++r)
+          if (r != q) acc[(u32)(d.offset[r] + d.codes[r][i])] += w;
+      }
+      auto& row = rows[self];
+      for (const auto& kv : acc)
+        if (kv.second != 0.0 || kv.first == self) row.push_back(kv);
+    }
+  }
+  for (i64 a = 0; a < d.n_dofs; ++a) g.indptr[a + 1] = g.indptr[a] + (u32)rows[a].size();
+  g.indices.resize(g.indptr[d.n_dofs]);
+  g.data.resize(g.indptr[d.n_dofs]);
+  for (i64 a = 0; a < d.n_dofs; ++a) {
+    u32 p = g.indptr[a];
+    for (const auto& kv : rows[a]) { g.indices[p] = kv.first; g.data[p] = kv.second; ++p; }
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// flatten subdomains into concatenated device arrays
+// ---------------------------------------------------------------------------
+static void flatten(Plan& p) {
+  PrecondFlat& f = p.pre;
+  const auto& subs = p.subs;
+  f.n_sub = (i64)subs.size();
+  f.loc_ptr.assign(1, 0);
+  f.red_ptr.assign(1, 0);
+  for (const auto& s : subs) {
+    f.loc_ptr.push_back(f.loc_ptr.back() + (u32)s.n_local());
+    f.red_ptr.push_back(f.red_ptr.back() + (u32)s.n_keep());
+  }
+  f.n_loc = f.loc_ptr.back();
+  f.n_red = f.red_ptr.back();
+  f.loc_gidx.resize(f.n_loc);
+  f.loc_scale.resize(f.n_loc);
+  f.loc_sub.resize(f.n_loc);
+  f.red_sub.resize(f.n_red);
+  f.red_loc.resize(f.n_red);
+  f.elim_ptr.assign(1, 0);
+  f.keep_ptr.assign(1, 0);
+  f.keep_ptr.reserve(f.n_red + 1);
+  f.inv_pivot.assign(f.n_red, 0.0);
+  i64 max_rounds = 0;
+  for (const auto& s : subs) max_rounds = std::max<i64>(max_rounds, (i64)s.f.round_ptr.size() - 1);
+  // keep rows and elimination rows, subdomain by subdomain
+  for (i64 si = 0; si < f.n_sub; ++si) {
+    const Subdomain& s = subs[si];
+    u32 L0 = f.loc_ptr[si], R0 = f.red_ptr[si];
+    i64 nq = s.ct.n_q(), nl = s.n_local();
+    for (i64 k = 0; k < nl; ++k) {
+      f.loc_gidx[L0 + k] = s.gidx[k];
+      double w = s.pou.empty() ? 1.0 : s.pou[k];
+      f.loc_scale[L0 + k] = k >= nq ? -w : w;
+      f.loc_sub[L0 + k] = (u32)si;
+    }
+    const Csr& e2k = s.eliminate_q ? s.ct.c : s.ct.ct;   // elim rows -> keep cols
+    const Csr& k2e = s.eliminate_q ? s.ct.ct : s.ct.c;   // keep rows -> elim cols
+    u32 elim_base = s.eliminate_q ? L0 : L0 + (u32)nq;
+    u32 keep_base = s.eliminate_q ? L0 + (u32)nq : L0;
+    for (i64 e = 0; e < e2k.nrows; ++e) {
+      f.elim_loc.push_back(elim_base + (u32)e);
+      f.elim_inv.push_back(s.inv_diag_elim[e]);
+      for (u32 q = e2k.indptr[e]; q < e2k.indptr[e + 1]; ++q) {
+        f.elim_idx.push_back(R0 + e2k.indices[q]);
+        f.elim_val.push_back(e2k.data[q]);
+      }
+      f.elim_ptr.push_back((u32)f.elim_idx.size());
+    }
+    for (i64 j = 0; j < k2e.nrows; ++j) {
+      f.red_sub[R0 + j] = (u32)si;
+      f.red_loc[R0 + j] = keep_base + (u32)j;
+      double rs = 0.0;
+      for (u32 q = k2e.indptr[j]; q < k2e.indptr[j + 1]; ++q) {
+        double v = k2e.data[q] * s.inv_diag_elim[k2e.indices[q]];
+        f.keep_idx.push_back(elim_base + k2e.indices[q]);
+        f.keep_val.push_back(v);
+        rs += v;
+      }
+      f.keep_ptr.push_back((u32)f.keep_idx.size());
+      f.keep_rowsum.push_back(rs);
+    }
+    f.c_nnz += (i64)s.ct.c.indices.size();
+    if (s.dense) f.dense_subs++;
+  }
+  // factor levels: round t of every subdomain, then the core gather level
+  std::vector<std::vector<std::pair<u32, double>>> in_edges(f.n_red);
+  for (i64 si = 0; si < f.n_sub; ++si) {
+    const Factor& F = subs[si].f;
+    u32 R0 = f.red_ptr[si];
+    for (u32 k = 0; k < F.round_ptr.back(); ++k) {
+      u32 a = F.order[k];
+      if (!(F.pivot[a] > 0.0)) continue;
+      for (u32 p = F.col_ptr[a]; p < F.col_ptr[a + 1]; ++p)
+        in_edges[R0 + F.col_idx[p]].push_back({R0 + a, F.col_w[p] / F.pivot[a]});
+      f.inv_pivot[R0 + a] = 1.0 / F.pivot[a];
+    }
+    f.ac_nnz += (i64)F.col_idx.size();
+  }
+  f.lvl_ptr.assign(1, 0);
+  f.fin_ptr.assign(1, 0);
+  f.bout_ptr.assign(1, 0);
+  for (i64 t = 0; t <= max_rounds; ++t) {
+    for (i64 si = 0; si < f.n_sub; ++si) {
+      const Factor& F = subs[si].f;
+      u32 R0 = f.red_ptr[si];
+      std::vector<u32> rows;
+      if (t < (i64)F.round_ptr.size() - 1) {
+        for (u32 k = F.round_ptr[t]; k < F.round_ptr[t + 1]; ++k) rows.push_back(F.order[k]);
+      } else if (t == max_rounds) {
+        rows = F.core;  // last level: core vertices gather their in-edges
+      }
+      for (u32 v : rows) {
+        u32 g = R0 + v;
+        f.lvl_rows.push_back(g);
+        for (const auto& pe : in_edges[g]) { f.fin_idx.push_back(pe.first); f.fin_val.push_back(pe.second); }
+        f.fin_ptr.push_back((u32)f.fin_idx.size());
+        // eliminated rows pull their later neighbours on the way back; core
+        // rows get their values from the dense solve instead
+        if (t < max_rounds && F.pivot[v] > 0.0)
+          for (u32 p = F.col_ptr[v]; p < F.col_ptr[v + 1]; ++p) {
+            f.bout_idx.push_back(R0 + F.col_idx[p]);
+            f.bout_val.push_back(F.col_w[p] / F.pivot[v]);
+          }
+        f.bout_ptr.push_back((u32)f.bout_idx.size());
+      }
+    }
+    f.lvl_ptr.push_back((u32)f.lvl_rows.size());
+  }
+  // dense cores
+  f.core_ptr.assign(1, 0);
+  for (i64 si = 0; si < f.n_sub; ++si) {
+    const Factor& F = subs[si].f;
+    if (F.core.empty()) continue;
+    u32 R0 = f.red_ptr[si];
+    f.core_mat_off.push_back(f.core_mat.size());
+    for (u32 v : F.core) {
+      f.core_idx.push_back(R0 + v);
+      f.core_row_core.push_back((u32)(f.core_ptr.size() - 1));
+    }
+    f.core_ptr.push_back((u32)f.core_idx.size());
+    f.core_mat.insert(f.core_mat.end(), F.core_inv.begin(), F.core_inv.end());
+    f.max_core = std::max<i64>(f.max_core, (i64)F.core.size());
+  }
+  // DOF -> local entries
+  f.dof_ptr.assign(p.d.n_dofs + 1, 0);
+  for (u32 g : f.loc_gidx) f.dof_ptr[g + 1]++;
+  for (i64 j = 0; j < p.d.n_dofs; ++j) f.dof_ptr[j + 1] += f.dof_ptr[j];
+  f.dof_loc.resize(f.n_loc);
+  {
+    std::vector<u32> cur(f.dof_ptr.begin(), f.dof_ptr.end() - 1);
+    for (i64 l = 0; l < f.n_loc; ++l) f.dof_loc[cur[f.loc_gidx[l]]++] = (u32)l;
+  }
+  f.sched_loc = make_schedule(f.loc_ptr, 1024, 64);
+  f.sched_red = make_schedule(f.red_ptr, 1024, 64);
+}
+
+void build_plan(Plan& p) {
+  const Design& d = p.d;
+  p.passes.resize(d.Q);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < d.Q; ++q) p.passes[q] = build_pass(d, q);
+  if (p.explicit_op || p.precond == 2) {
+    p.gram = build_gramian(d);
+    p.has_gram = true;
+  }
+  if (p.precond == 1 || p.precond == 2) {
+    p.subs = build_subdomains(d);
+    std::vector<std::string> err(p.subs.size());
+    std::vector<int> code(p.subs.size(), 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < p.subs.size(); ++i) {
+      try {
+        build_local_solver(p.subs[i], p.cfg);
+      } catch (const Error& e) {
+        err[i] = e.what();
+        code[i] = e.code;
+      }
+    }
+    for (size_t i = 0; i < p.subs.size(); ++i)
+      if (code[i]) throw Error(code[i], err[i]);
+    flatten(p);
+  }
+}
+
+}  // namespace wa

@@ -1,0 +1,174 @@
+// plan.hpp -- host-side setup of the projection path: everything that is
+// built once per design and uploaded to HBM (SolverParams / Solver::from_design,
+// crates/within/src/solver.rs:98-132).
+//
+// The plan flattens the reference's per-subdomain objects into concatenated
+// arrays so that one kernel launch serves every subdomain:
+//   * observation layouts for the deterministic implicit operator
+//     (level-sorted CSR per factor; domain.rs:303-432 semantics),
+//   * cross-tabs / components / partition weights (factor_pairs.rs,
+//     gramian/cross_tab.rs),
+//   * Schur complements (schur_complement.rs) and the reduced-system factor
+//     as rounds of independent eliminations plus a dense anchored core,
+//   * the explicit Gramian CSR (gramian/explicit.rs).
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace wa {
+
+using u32 = uint32_t;
+using i64 = int64_t;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct LocalConfig {
+  uint64_t ac_seed = 0;
+  int ac_split = 1;
+  bool approx_schur = true;
+  uint64_t schur_seed = 0;
+  int schur_split = 1;
+  int dense_threshold = 24;
+  i64 dense_core = 2048;
+};
+
+struct Design {
+  i64 n_obs = 0;
+  int Q = 0;
+  std::vector<std::vector<u32>> codes;  // codes[q][i]
+  std::vector<double> weights;          // empty = unit
+  std::vector<i64> levels, offset;
+  i64 n_dofs = 0;
+  double w(i64 i) const { return weights.empty() ? 1.0 : weights[i]; }
+};
+Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* weights);
+
+// Work decomposition of a segmented reduction: a work item either covers whole
+// rows [row0, row1) (written directly) or one chunk [e0, e1) of a long row
+// (written to partial slot `slot`, summed in slot order by a fix-up).
+struct Item { u32 row0, row1, e0, e1; int32_t slot; int32_t pad; };
+struct Fixup { u32 row, slot0, slot1, pad; };
+struct Schedule {
+  std::vector<Item> items;
+  std::vector<Fixup> fixups;
+  i64 n_slots = 0;
+};
+Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows);
+
+// Level-sorted CSR of one factor: rows = levels of q, elements = observations
+// sorted by (level, code of the largest other factor, observation index).
+struct FactorPass {
+  int q = 0;
+  std::vector<u32> row_ptr;   // levels[q] + 1
+  std::vector<u32> obs;       // element -> observation
+  std::vector<u32> other;     // element-major, (Q-1) global DOF indices each
+  std::vector<double> w;      // element weights (empty = unit)
+  std::vector<double> diag;   // sum of weights per level
+  Schedule sched_wide, sched_narrow;
+};
+
+struct Csr {
+  i64 nrows = 0, ncols = 0;
+  std::vector<u32> indptr, indices;
+  std::vector<double> data;
+  Csr transpose() const;
+};
+
+struct CrossTab {
+  Csr c, ct;
+  std::vector<double> diag_q, diag_r;
+  i64 n_q() const { return c.nrows; }
+  i64 n_r() const { return c.ncols; }
+};
+
+// Reduced-system factor (rounds of independent eliminations + dense core).
+struct Factor {
+  i64 n = 0;
+  std::vector<u32> round_ptr, order;
+  std::vector<double> pivot;
+  std::vector<u32> col_ptr, col_idx;
+  std::vector<double> col_w;
+  std::vector<u32> core;
+  std::vector<double> core_inv;  // |core|^2, anchored (zero last row / column)
+};
+
+struct Subdomain {
+  int q = 0, r = 0;
+  std::vector<u32> gidx;    // q-levels then r-levels
+  std::vector<double> pou;  // empty = uniform
+  CrossTab ct;
+  bool eliminate_q = true, dense = false;
+  std::vector<double> inv_diag_elim;
+  Factor f;                 // dense path: rounds empty, core = all kept vertices
+  i64 n_local() const { return ct.n_q() + ct.n_r(); }
+  i64 n_keep() const { return eliminate_q ? ct.n_r() : ct.n_q(); }
+};
+
+// Concatenated, device-ready preconditioner arrays.
+struct PrecondFlat {
+  i64 n_loc = 0, n_red = 0, n_sub = 0;
+  // local space (per subdomain: q entries then r entries)
+  std::vector<u32> loc_gidx;
+  std::vector<double> loc_scale;      // sign (r-block negated) x partition weight
+  std::vector<u32> loc_ptr;           // n_sub + 1
+  std::vector<u32> red_ptr;           // n_sub + 1 (reduced = kept vertices)
+  std::vector<u32> loc_sub;           // local entry -> subdomain
+  std::vector<u32> red_sub;           // reduced entry -> subdomain
+  std::vector<u32> red_loc;           // reduced entry -> its local entry
+  // elimination rows: eliminated local entry e, inv diag, row of C / C' into reduced
+  std::vector<u32> elim_loc;          // n_elim
+  std::vector<double> elim_inv;       // n_elim
+  std::vector<u32> elim_ptr, elim_idx;   // CSR: elim row -> reduced entries
+  std::vector<double> elim_val;
+  // keep rows: reduced entry j, CSR over eliminated local entries with value
+  // C[e,j] * inv_diag[e], plus the row sum of those values
+  std::vector<u32> keep_ptr, keep_idx;
+  std::vector<double> keep_val, keep_rowsum;
+  // factor levels in the reduced space (all subdomains merged by round)
+  std::vector<u32> lvl_ptr;           // n_levels + 1 over lvl_rows
+  std::vector<u32> lvl_rows;          // reduced entries eliminated per round
+  std::vector<u32> fin_ptr, fin_idx;  // forward: in-edges (earlier vertex a, coef w/d_a)
+  std::vector<double> fin_val;        //   indexed by position in lvl_rows
+  std::vector<u32> bout_ptr, bout_idx;  // backward: out-edges (later vertex u, coef w/d_v)
+  std::vector<double> bout_val;
+  std::vector<double> inv_pivot;      // per reduced entry (0 for core / anchors)
+  // dense cores
+  std::vector<u32> core_ptr;          // n_cores + 1 over core_idx
+  std::vector<u32> core_idx;          // reduced entries
+  std::vector<uint64_t> core_mat_off; // n_cores
+  std::vector<double> core_mat;       // concatenated row-major inverses
+  std::vector<u32> core_row_core;     // per core row -> core id
+  // DOF combine: dof -> local entries (subdomain order)
+  std::vector<u32> dof_ptr, dof_loc;
+  // segment schedules for per-subdomain sums
+  Schedule sched_loc, sched_red;
+  i64 max_core = 0, ac_nnz = 0, c_nnz = 0, dense_subs = 0;
+};
+
+struct Plan {
+  Design d;
+  LocalConfig cfg;
+  int precond = 1;            // 0 none, 1 additive, 2 multiplicative
+  bool explicit_op = false;
+  std::vector<FactorPass> passes;  // one per factor
+  std::vector<Subdomain> subs;
+  PrecondFlat pre;
+  Csr gram;                   // explicit Gramian (when needed)
+  bool has_gram = false;
+};
+
+// Builds everything the solver uploads (throws wa::Error).
+void build_plan(Plan& p);
+
+// Individual steps (exposed for tests through the C ABI).
+std::vector<Subdomain> build_subdomains(const Design& d);
+void build_local_solver(Subdomain& s, const LocalConfig& cfg);
+Csr build_gramian(const Design& d);
+std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n);
+
+}  // namespace wa

@@ -1,0 +1,916 @@
+// runtime.cpp -- the C ABI (include/within_amd.h): device upload of the plan
+// and the solver loops (block CG / GMRES, FP64 iterative refinement) that
+// drive the kernels.  Mirrors Solver::solve / solve_batch
+// (crates/within/src/solver.rs:158-347) for k right-hand sides per block.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/within_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace wa;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+struct DevError : std::runtime_error {
+  explicit DevError(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      throw DevError(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);      \
+  } while (0)
+
+int set_error(const std::exception& e) {
+  g_err = e.what();
+  if (auto* we = dynamic_cast<const Error*>(&e)) return we->code;
+  if (dynamic_cast<const DevError*>(&e)) return WA_ERR_DEVICE;
+  return WA_ERR_INVALID_ARGUMENT;
+}
+
+constexpr int KMAX = 32;
+constexpr double F64_EPS = 2.220446049250313e-16;
+
+int pow2_at_least(int k) {
+  int p = 1;
+  while (p < k) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+struct wa_design {
+  Design d;
+};
+
+struct Timer {
+  bool on = false;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  int64_t launches[5] = {0, 0, 0, 0, 0};
+  double ms[5] = {0, 0, 0, 0, 0};
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  void flush() {
+    for (auto& r : pending) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.cls] += t;
+      launches[r.cls]++;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  ~Timer() {
+    for (auto& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+struct wa_solver {
+  Plan plan;
+  wa_params params{};
+  bool host_only = false;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  int64_t n = 0, m = 0;
+  int Q = 0;
+  // design
+  uint32_t* codes = nullptr;  // Q x n (original order)
+  uint32_t* offs = nullptr;
+  double* wobs = nullptr;     // weights in observation order (null = unit)
+  std::vector<DevPass<double>> p64;
+  std::vector<DevPass<float>> p32;
+  bool has_pre = false, has_gram = false;
+  DevPrecond<double> pre64{};
+  DevPrecond<float> pre32{};
+  DevCsr<double> g64{};
+  DevCsr<float> g32{};
+  // workspaces (kp <= 32)
+  double *Yr = nullptr, *E = nullptr;
+  double *B64 = nullptr, *X64 = nullptr, *R64 = nullptr, *T64 = nullptr;
+  double *cx64 = nullptr, *cr64 = nullptr, *cz64 = nullptr, *cp64 = nullptr, *cap64 = nullptr;
+  float *cx32 = nullptr, *cr32 = nullptr, *cz32 = nullptr, *cp32 = nullptr, *cap32 = nullptr;
+  float* rhs32 = nullptr;
+  PrecondWork<double> w64{};
+  PrecondWork<float> w32{};
+  double* part64 = nullptr;  // item partials (FP64; FP32 kernels use it reinterpreted)
+  double* redp = nullptr;    // reduction partials
+  double* sc = nullptr;      // small per-column scalars
+  int* imask = nullptr;
+  ColState* cs = nullptr;
+  double* hbuf = nullptr;    // pinned host scratch
+  Timer timer;
+  // device user buffers for host-pointer calls
+  double* ubuf = nullptr;
+  int64_t ubuf_len = 0;
+
+  template <typename T>
+  T* dalloc(int64_t count) {
+    if (count <= 0) count = 1;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(T) * (size_t)count));
+    allocs.push_back(p);
+    return (T*)p;
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v) {
+    T* p = dalloc<T>((int64_t)v.size());
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return p;
+  }
+  template <typename T>
+  T* upload_as(const std::vector<double>& v) {
+    std::vector<T> c(v.begin(), v.end());
+    return upload(c);
+  }
+  DevSched upload_sched(const Schedule& s) {
+    DevSched d;
+    d.items = upload(s.items);
+    d.n_items = (int64_t)s.items.size();
+    d.fix = upload(s.fixups);
+    d.n_fix = (int64_t)s.fixups.size();
+    d.n_slots = s.n_slots;
+    return d;
+  }
+  ~wa_solver() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : allocs) (void)hipFree(p);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // scalar scratch layout (each KMAX doubles)
+  double* s_(int i) { return sc + (int64_t)i * KMAX; }
+
+  void tic(int cls, hipEvent_t& a) {
+    if (!timer.on) return;
+    a = timer.get();
+    HIPCHK(hipEventRecord(a, st));
+  }
+  void toc(int cls, hipEvent_t a) {
+    if (!timer.on) return;
+    hipEvent_t b = timer.get();
+    HIPCHK(hipEventRecord(b, st));
+    timer.pending.push_back({cls, a, b});
+  }
+
+  void setup_device();
+  template <typename T> void op_apply(const T* X, T* Y, int kp);
+  template <typename T> void pre_apply(const T* R, T* Z, int kp);
+  void rhs64(const double* V, double* out, int kp);
+  template <typename T>
+  void cg(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+          std::vector<int64_t>& its);
+  template <typename T>
+  void gmres(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+             std::vector<int64_t>& its);
+  template <typename T>
+  void krylov(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+              std::vector<int64_t>& its) {
+    if (params.krylov == 1) gmres<T>(k, kp, rhs, x, tol, conv, its);
+    else cg<T>(k, kp, rhs, x, tol, conv, its);
+  }
+  void solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
+                   int64_t ldd, wa_rhs_stats* stats);
+  std::vector<double> col_norms(const double* V, int64_t rows, int kp);
+  template <typename T> T* vec(int which);
+};
+
+template <> double* wa_solver::vec<double>(int w) {
+  switch (w) { case 0: return cx64; case 1: return cr64; case 2: return cz64; case 3: return cp64; default: return cap64; }
+}
+template <> float* wa_solver::vec<float>(int w) {
+  switch (w) { case 0: return cx32; case 1: return cr32; case 2: return cz32; case 3: return cp32; default: return cap32; }
+}
+
+static void check_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    throw DevError(std::string("within_amd requires a gfx950 (MI355X) device, found ") + prop.gcnArchName);
+}
+
+void wa_solver::setup_device() {
+  const Design& d = plan.d;
+  n = d.n_obs;
+  m = d.n_dofs;
+  Q = d.Q;
+  check_device();
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  // design (original observation order) for D x and demeaning
+  {
+    std::vector<uint32_t> c((size_t)Q * n);
+    for (int q = 0; q < Q; ++q) std::memcpy(&c[(size_t)q * n], d.codes[q].data(), sizeof(uint32_t) * n);
+    codes = upload(c);
+    std::vector<uint32_t> o;
+    for (int q = 0; q < Q; ++q) o.push_back((uint32_t)d.offset[q]);
+    offs = upload(o);
+    if (!d.weights.empty()) wobs = upload(d.weights);
+  }
+  int64_t max_slots = 1;
+  for (const FactorPass& fp : plan.passes) {
+    DevPass<double> a;
+    DevPass<float> b;
+    a.row_ptr = b.row_ptr = upload(fp.row_ptr);
+    a.other = b.other = upload(fp.other);
+    a.obs = b.obs = upload(fp.obs);
+    if (!fp.w.empty()) {
+      a.ew = upload(fp.w);
+      b.ew = upload_as<float>(fp.w);
+    }
+    a.diag = upload(fp.diag);
+    b.diag = upload_as<float>(fp.diag);
+    a.qm = b.qm = Q - 1;
+    a.off = b.off = (uint32_t)d.offset[fp.q];
+    a.levels = b.levels = d.levels[fp.q];
+    a.n_elem = b.n_elem = n;
+    a.wide = b.wide = upload_sched(fp.sched_wide);
+    a.narrow = b.narrow = upload_sched(fp.sched_narrow);
+    max_slots = std::max({max_slots, a.wide.n_slots, a.narrow.n_slots});
+    p64.push_back(a);
+    p32.push_back(b);
+  }
+  if (plan.has_gram) {
+    const Csr& g = plan.gram;
+    std::vector<uint32_t> ptr(g.indptr.begin(), g.indptr.end());
+    Schedule wide = make_schedule(ptr, 256, 64), narrow = make_schedule(ptr, 64, 64);
+    g64.ptr = g32.ptr = upload(g.indptr);
+    g64.idx = g32.idx = upload(g.indices);
+    g64.val = upload(g.data);
+    g32.val = upload_as<float>(g.data);
+    g64.nrows = g32.nrows = g.nrows;
+    g64.nnz = g32.nnz = (int64_t)g.indices.size();
+    g64.wide = g32.wide = upload_sched(wide);
+    g64.narrow = g32.narrow = upload_sched(narrow);
+    max_slots = std::max({max_slots, g64.wide.n_slots, g64.narrow.n_slots});
+    has_gram = true;
+  }
+  if (!plan.subs.empty()) {
+    const PrecondFlat& f = plan.pre;
+    auto fill = [&](auto& P, auto tag) {
+      using T = decltype(tag);
+      P.n_loc = f.n_loc;
+      P.n_red = f.n_red;
+      P.n_sub = f.n_sub;
+      P.n_elim = (int64_t)f.elim_loc.size();
+      P.n_levels = (int64_t)f.lvl_ptr.size() - 1;
+      P.n_core_rows = (int64_t)f.core_idx.size();
+      P.h_lvl_ptr = f.lvl_ptr;
+      P.loc_scale = upload_as<T>(f.loc_scale);
+      P.elim_inv = upload_as<T>(f.elim_inv);
+      P.elim_val = upload_as<T>(f.elim_val);
+      P.keep_val = upload_as<T>(f.keep_val);
+      P.keep_rowsum = upload_as<T>(f.keep_rowsum);
+      P.fin_val = upload_as<T>(f.fin_val);
+      P.bout_val = upload_as<T>(f.bout_val);
+      P.inv_pivot = upload_as<T>(f.inv_pivot);
+      P.core_mat = upload_as<T>(f.core_mat);
+    };
+    fill(pre64, double());
+    fill(pre32, float());
+    auto share = [&](auto& P) {
+      P.sched_loc = upload_sched(f.sched_loc);
+      P.sched_red = upload_sched(f.sched_red);
+    };
+    // integer arrays shared by both precisions
+    const uint32_t* loc_gidx = upload(f.loc_gidx);
+    const uint32_t* loc_ptr = upload(f.loc_ptr);
+    const uint32_t* red_ptr = upload(f.red_ptr);
+    const uint32_t* loc_sub = upload(f.loc_sub);
+    const uint32_t* red_sub = upload(f.red_sub);
+    const uint32_t* red_loc = upload(f.red_loc);
+    const uint32_t* elim_loc = upload(f.elim_loc);
+    const uint32_t* elim_ptr = upload(f.elim_ptr);
+    const uint32_t* elim_idx = upload(f.elim_idx);
+    const uint32_t* keep_ptr = upload(f.keep_ptr);
+    const uint32_t* keep_idx = upload(f.keep_idx);
+    const uint32_t* lvl_rows = upload(f.lvl_rows);
+    const uint32_t* fin_ptr = upload(f.fin_ptr);
+    const uint32_t* fin_idx = upload(f.fin_idx);
+    const uint32_t* bout_ptr = upload(f.bout_ptr);
+    const uint32_t* bout_idx = upload(f.bout_idx);
+    const uint32_t* core_ptr = upload(f.core_ptr);
+    const uint32_t* core_idx = upload(f.core_idx);
+    const uint32_t* core_row_core = upload(f.core_row_core);
+    const uint64_t* core_mat_off = upload(f.core_mat_off);
+    const uint32_t* dof_ptr = upload(f.dof_ptr);
+    const uint32_t* dof_loc = upload(f.dof_loc);
+    auto ints = [&](auto& P) {
+      P.loc_gidx = loc_gidx; P.loc_ptr = loc_ptr; P.red_ptr = red_ptr; P.loc_sub = loc_sub;
+      P.red_sub = red_sub; P.red_loc = red_loc; P.elim_loc = elim_loc; P.elim_ptr = elim_ptr;
+      P.elim_idx = elim_idx; P.keep_ptr = keep_ptr; P.keep_idx = keep_idx; P.lvl_rows = lvl_rows;
+      P.fin_ptr = fin_ptr; P.fin_idx = fin_idx; P.bout_ptr = bout_ptr; P.bout_idx = bout_idx;
+      P.core_ptr = core_ptr; P.core_idx = core_idx; P.core_row_core = core_row_core;
+      P.core_mat_off = core_mat_off; P.dof_ptr = dof_ptr; P.dof_loc = dof_loc;
+    };
+    ints(pre64);
+    ints(pre32);
+    share(pre64);
+    pre32.sched_loc = pre64.sched_loc;
+    pre32.sched_red = pre64.sched_red;
+    max_slots = std::max({max_slots, pre64.sched_loc.n_slots, pre64.sched_red.n_slots});
+    int64_t nsub = std::max<int64_t>(f.n_sub, 1);
+    w64.B = dalloc<double>(f.n_loc * KMAX);
+    w64.S = dalloc<double>(f.n_loc * KMAX);
+    w64.Rr = dalloc<double>(f.n_red * KMAX);
+    w64.C = dalloc<double>((int64_t)f.core_idx.size() * KMAX);
+    w64.sum1 = dalloc<double>(nsub * KMAX);
+    w64.sum2 = dalloc<double>(nsub * KMAX);
+    w64.sum3 = dalloc<double>(nsub * KMAX);
+    w32.B = dalloc<float>(f.n_loc * KMAX);
+    w32.S = dalloc<float>(f.n_loc * KMAX);
+    w32.Rr = dalloc<float>(f.n_red * KMAX);
+    w32.C = dalloc<float>((int64_t)f.core_idx.size() * KMAX);
+    w32.sum1 = dalloc<float>(nsub * KMAX);
+    w32.sum2 = dalloc<float>(nsub * KMAX);
+    w32.sum3 = dalloc<float>(nsub * KMAX);
+    has_pre = true;
+  }
+  part64 = dalloc<double>(max_slots * KMAX);
+  w64.part = part64;
+  w32.part = (float*)part64;
+  redp = dalloc<double>((int64_t)kRedBlocks * KMAX);
+  sc = dalloc<double>(16 * KMAX);
+  imask = dalloc<int>(3 * KMAX);
+  cs = dalloc<ColState>(KMAX);
+  HIPCHK(hipHostMalloc((void**)&hbuf, sizeof(double) * 16 * KMAX, hipHostMallocDefault));
+  // vectors
+  Yr = dalloc<double>(n * KMAX);
+  E = dalloc<double>(n * KMAX);
+  B64 = dalloc<double>(m * KMAX);
+  X64 = dalloc<double>(m * KMAX);
+  R64 = dalloc<double>(m * KMAX);
+  T64 = dalloc<double>(m * KMAX);
+  cx64 = dalloc<double>(m * KMAX); cr64 = dalloc<double>(m * KMAX); cz64 = dalloc<double>(m * KMAX);
+  cp64 = dalloc<double>(m * KMAX); cap64 = dalloc<double>(m * KMAX);
+  cx32 = dalloc<float>(m * KMAX); cr32 = dalloc<float>(m * KMAX); cz32 = dalloc<float>(m * KMAX);
+  cp32 = dalloc<float>(m * KMAX); cap32 = dalloc<float>(m * KMAX);
+  rhs32 = dalloc<float>(m * KMAX);
+  HIPCHK(hipDeviceSynchronize());
+}
+
+template <typename T>
+static const DevPass<T>& pass_of(wa_solver* s, int q);
+template <> const DevPass<double>& pass_of<double>(wa_solver* s, int q) { return s->p64[q]; }
+template <> const DevPass<float>& pass_of<float>(wa_solver* s, int q) { return s->p32[q]; }
+
+template <typename T>
+void wa_solver::op_apply(const T* X, T* Y, int kp) {
+  hipEvent_t a = nullptr;
+  if (params.operator_repr == 1 && has_gram) {
+    tic(4, a);
+    if constexpr (std::is_same<T, double>::value) launch_spmm<double>(st, g64, X, Y, part64, kp);
+    else launch_spmm<float>(st, g32, X, Y, (float*)part64, kp);
+    toc(4, a);
+    return;
+  }
+  tic(0, a);
+  for (int q = 0; q < Q; ++q) launch_pull<T>(st, pass_of<T>(this, q), X, Y, (T*)part64, kp);
+  toc(0, a);
+}
+
+template <typename T>
+void wa_solver::pre_apply(const T* R, T* Z, int kp) {
+  hipEvent_t a = nullptr;
+  tic(1, a);
+  if constexpr (std::is_same<T, double>::value) launch_precond<double>(st, pre64, R, Z, w64, kp, m);
+  else launch_precond<float>(st, pre32, R, Z, w32, kp, m);
+  toc(1, a);
+}
+
+void wa_solver::rhs64(const double* V, double* out, int kp) {
+  hipEvent_t a = nullptr;
+  tic(3, a);
+  for (int q = 0; q < Q; ++q) launch_scatter_obs(st, p64[q], p64[q].obs, wobs, V, out, part64, kp);
+  toc(3, a);
+}
+
+std::vector<double> wa_solver::col_norms(const double* V, int64_t rows, int kp) {
+  launch_dot<double>(st, V, V, rows, kp, redp, sc);
+  HIPCHK(hipMemcpyAsync(hbuf, sc, sizeof(double) * kp, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<double> r(kp);
+  for (int c = 0; c < kp; ++c) r[c] = std::sqrt(hbuf[c]);
+  return r;
+}
+
+// Block CG: k independent recurrences sharing every operator and
+// preconditioner apply (solve/cg.rs:45-133 per column).
+template <typename T>
+void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+                   std::vector<int64_t>& its) {
+  T *r = vec<T>(1), *z = vec<T>(2), *p = vec<T>(3), *ap = vec<T>(4);
+  const int64_t bytes = sizeof(T) * m * kp;
+  double *bn2 = s_(0), *rz = s_(1), *pap = s_(2), *alpha = s_(3), *rr = s_(4), *rzn = s_(5), *beta = s_(6),
+         *tolv = s_(7);
+  int* mask = imask;
+  int* nact = imask + KMAX;
+  std::vector<double> tp(KMAX, 0.0);
+  for (int c = 0; c < k; ++c) tp[c] = tol[c];
+  HIPCHK(hipMemcpyAsync(tolv, tp.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(x, 0, bytes, st));
+  HIPCHK(hipMemcpyAsync(r, rhs, bytes, hipMemcpyDeviceToDevice, st));
+  hipEvent_t ev = nullptr;
+  if (has_pre && params.preconditioner != 0) pre_apply<T>(r, z, kp);
+  else HIPCHK(hipMemcpyAsync(z, r, bytes, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(p, z, bytes, hipMemcpyDeviceToDevice, st));
+  tic(2, ev);
+  launch_dot<T>(st, r, z, m, kp, redp, rz);
+  launch_dot<T>(st, rhs, rhs, m, kp, redp, bn2);
+  launch_cg_start(st, cs, bn2, rz, tolv, k, kp);
+  toc(2, ev);
+  int64_t itn = 0;
+  bool any = true;
+  {
+    // all columns may already be done (zero right-hand sides)
+    std::vector<ColState> h(kp);
+    HIPCHK(hipMemcpyAsync(h.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    any = false;
+    for (int c = 0; c < kp; ++c) any = any || h[c].active;
+  }
+  while (any && itn < params.maxiter) {
+    ++itn;
+    op_apply<T>(p, ap, kp);
+    tic(2, ev);
+    launch_dot<T>(st, p, ap, m, kp, redp, pap);
+    launch_cg_alpha(st, cs, pap, alpha, (int)itn, kp);
+    launch_cg_update<T>(st, x, p, r, ap, alpha, m, kp, redp, rr);
+    launch_cg_check(st, cs, rr, (int)itn, kp);
+    toc(2, ev);
+    if (has_pre && params.preconditioner != 0) pre_apply<T>(r, z, kp);
+    else HIPCHK(hipMemcpyAsync(z, r, bytes, hipMemcpyDeviceToDevice, st));
+    tic(2, ev);
+    launch_dot<T>(st, r, z, m, kp, redp, rzn);
+    launch_cg_beta(st, cs, rzn, beta, mask, nact, (int)itn, kp);
+    launch_xpby<T>(st, p, z, beta, mask, m, kp);
+    toc(2, ev);
+    HIPCHK(hipMemcpyAsync(hbuf, nact, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    any = *(int*)hbuf > 0;
+  }
+  std::vector<ColState> h(kp);
+  HIPCHK(hipMemcpyAsync(h.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int c = 0; c < k; ++c) {
+    if (h[c].active) {  // maxiter reached
+      conv[c] = 0;
+      its[c] = params.maxiter;
+    } else {
+      conv[c] = h[c].converged;
+      its[c] = h[c].iters;
+    }
+  }
+}
+
+// Right-preconditioned GMRES(restart) with modified Gram-Schmidt and Givens
+// rotations, one Arnoldi process per column (solve/gmres.rs:201-530).  The
+// Hessenberg / Givens recurrences stay in FP64 on the host side of the loop.
+template <typename T>
+void wa_solver::gmres(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+                      std::vector<int64_t>& its) {
+  (void)k; (void)kp; (void)rhs; (void)x; (void)tol; (void)conv; (void)its;
+  throw Error(WA_ERR_UNSUPPORTED, "GMRES on the device is not available in this build");
+}
+
+// One block of up to 32 right-hand sides (Solver::solve per column,
+// solver.rs:158-239 + iterative_refinement :296-347).
+void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
+                            int64_t ldd, wa_rhs_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int kp = pow2_at_least(kb);
+  hipEvent_t ev = nullptr;
+  tic(3, ev);
+  launch_col_to_row(st, Y + (int64_t)c0 * ldy, ldy, n, kb, Yr, kp);
+  toc(3, ev);
+  rhs64(Yr, B64, kp);
+  std::vector<double> bn = col_norms(B64, m, kp);
+  std::vector<double> rhs_norm(kb), abs_tol(kb);
+  for (int c = 0; c < kb; ++c) {
+    rhs_norm[c] = std::max(bn[c], 1e-15);
+    abs_tol[c] = params.tol * rhs_norm[c];
+  }
+  std::vector<int> conv(kb, 0), cconv(kb, 0);
+  std::vector<int64_t> its(kb, 0), cits(kb, 0);
+  const int64_t vbytes = sizeof(double) * m * kp;
+  const bool mixed = params.precision == 1;
+  if (!mixed) {
+    std::vector<double> tol(kb, params.tol);
+    krylov<double>(kb, kp, B64, cx64, tol, conv, its);
+    HIPCHK(hipMemcpyAsync(X64, cx64, vbytes, hipMemcpyDeviceToDevice, st));
+    std::vector<int> live(kb, 1);
+    for (int64_t it = 0; it < params.max_refinements; ++it) {
+      launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+      rhs64(E, R64, kp);
+      std::vector<double> cn = col_norms(R64, m, kp);
+      std::vector<double> scale(KMAX, 0.0), ctol(kb, 1.0);
+      bool anyc = false;
+      for (int c = 0; c < kb; ++c) {
+        if (live[c] && cn[c] <= abs_tol[c]) live[c] = 0;
+        if (live[c]) {
+          anyc = true;
+          scale[c] = 1.0;
+          ctol[c] = std::min(abs_tol[c] / cn[c], 1.0);
+        }
+      }
+      if (!anyc) break;
+      HIPCHK(hipMemcpyAsync(s_(8), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+      launch_cast_scale<double, double>(st, R64, T64, s_(8), m, kp);
+      krylov<double>(kb, kp, T64, cx64, ctol, cconv, cits);
+      std::vector<int> msk(KMAX, 0);
+      for (int c = 0; c < kb; ++c) msk[c] = live[c];
+      HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
+      launch_axpy_acc<double>(st, X64, cx64, nullptr, imask + KMAX + 1, m, kp);
+      for (int c = 0; c < kb; ++c)
+        if (live[c]) {
+          its[c] += cits[c];
+          conv[c] = cconv[c];
+        }
+    }
+  } else {
+    // FP32 inner Krylov, FP64 outer residual D'W(y - Dx) and accumulation
+    HIPCHK(hipMemsetAsync(X64, 0, vbytes, st));
+    const int64_t max_outer = std::max<int64_t>(params.max_refinements, 0) + 8;
+    std::vector<int> live(kb, 1);
+    for (int64_t outer = 0; outer <= max_outer; ++outer) {
+      std::vector<double> rn;
+      if (outer == 0) {
+        HIPCHK(hipMemcpyAsync(R64, B64, vbytes, hipMemcpyDeviceToDevice, st));
+        rn = bn;
+      } else {
+        launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+        rhs64(E, R64, kp);
+        rn = col_norms(R64, m, kp);
+      }
+      std::vector<double> scale(KMAX, 0.0), itol(kb, 1.0);
+      std::vector<int> msk(KMAX, 0);
+      bool anyc = false;
+      for (int c = 0; c < kb; ++c) {
+        if (live[c] && (rn[c] <= abs_tol[c] || rn[c] < F64_EPS)) live[c] = 0;
+        if (live[c]) {
+          anyc = true;
+          scale[c] = 1.0 / rn[c];
+          msk[c] = 1;
+          itol[c] = std::min(1.0, std::max(abs_tol[c] / rn[c], 1e-4));
+        }
+      }
+      if (!anyc || outer == max_outer) break;
+      HIPCHK(hipMemcpyAsync(s_(8), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+      launch_cast_scale<double, float>(st, R64, rhs32, s_(8), m, kp);
+      krylov<float>(kb, kp, rhs32, cx32, itol, cconv, cits);
+      for (int c = 0; c < kb; ++c) scale[c] = msk[c] ? rn[c] : 0.0;
+      HIPCHK(hipMemcpyAsync(s_(9), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
+      launch_axpy_acc<float>(st, X64, cx32, s_(9), imask + KMAX + 1, m, kp);
+      for (int c = 0; c < kb; ++c)
+        if (live[c]) its[c] += cits[c];
+    }
+  }
+  // final demeaning and observation-space residual (FP64)
+  launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+  rhs64(E, R64, kp);
+  std::vector<double> fr = col_norms(R64, m, kp);
+  tic(3, ev);
+  if (dem) launch_row_to_col(st, E, kp, n, kb, dem + (int64_t)c0 * ldd, ldd);
+  launch_row_to_col(st, X64, kp, m, kb, X + (int64_t)c0 * ldx, ldx);
+  toc(3, ev);
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  double tsolve = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int c = 0; c < kb; ++c) {
+    wa_rhs_stats& s = stats[c0 + c];
+    s.residual = fr[c] / rhs_norm[c];
+    s.iterations = its[c];
+    s.converged = mixed ? (s.residual <= params.tol) : conv[c];
+    s.time_solve = tsolve;
+  }
+  if (timer.on) timer.flush();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static void need_device(wa_solver* s) {
+  if (!s) throw Error(WA_ERR_INVALID_ARGUMENT, "null solver");
+  if (s->host_only) throw DevError("solver was built host-only (no device state)");
+}
+
+// host column-major in/out around a DOF-block device op
+template <typename F>
+static wa_status level_a(wa_solver* s, const double* in, int64_t rows_in, int64_t ldin, double* out,
+                         int64_t rows_out, int64_t ldout, int32_t k, F&& body) {
+  try {
+    need_device(s);
+    if (k <= 0 || k > KMAX) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be in 1..32");
+    if (!in || !out || ldin < rows_in || ldout < rows_out)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "bad buffer or leading dimension");
+    const int kp = pow2_at_least(k);
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc(&din, sizeof(double) * rows_in * k));
+    HIPCHK(hipMalloc(&dout, sizeof(double) * rows_out * k));
+    std::unique_ptr<double, void (*)(double*)> g1(din, [](double* p) { (void)hipFree(p); });
+    std::unique_ptr<double, void (*)(double*)> g2(dout, [](double* p) { (void)hipFree(p); });
+    for (int c = 0; c < k; ++c)
+      HIPCHK(hipMemcpyAsync(din + (int64_t)c * rows_in, in + (int64_t)c * ldin, sizeof(double) * rows_in,
+                            hipMemcpyHostToDevice, s->st));
+    body(din, dout, kp);
+    for (int c = 0; c < k; ++c)
+      HIPCHK(hipMemcpyAsync(out + (int64_t)c * ldout, dout + (int64_t)c * rows_out, sizeof(double) * rows_out,
+                            hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipGetLastError());
+    if (s->timer.on) s->timer.flush();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+extern "C" {
+
+const char* wa_last_error(void) { return g_err.c_str(); }
+
+void wa_params_default(wa_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->krylov = 0;
+  p->restart = 30;
+  p->operator_repr = 0;
+  p->preconditioner = 1;
+  p->tol = 1e-8;
+  p->maxiter = 1000;
+  p->max_refinements = 2;
+  p->precision = 0;
+  p->block = 0;
+  p->host_only = 0;
+  p->local.ac_seed = 0;
+  p->local.ac_split = 1;
+  p->local.approx_schur = 1;
+  p->local.schur_seed = 0;
+  p->local.schur_split = 1;
+  p->local.dense_threshold = 24;
+  p->local.dense_core = 0;
+}
+
+wa_status wa_design_create(const uint32_t* codes, int64_t n_obs, int32_t n_factors, int64_t ld,
+                           const double* weights, wa_design** out) {
+  try {
+    if (!out) throw Error(WA_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (n_obs > 0 && !codes) throw Error(WA_ERR_INVALID_ARGUMENT, "null codes");
+    auto d = std::make_unique<wa_design>();
+    d->d = make_design(codes, n_obs, n_factors, ld, weights);
+    *out = d.release();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+void wa_design_destroy(wa_design* d) { delete d; }
+int64_t wa_design_n_dofs(const wa_design* d) { return d ? d->d.n_dofs : 0; }
+int64_t wa_design_n_obs(const wa_design* d) { return d ? d->d.n_obs : 0; }
+int32_t wa_design_n_factors(const wa_design* d) { return d ? d->d.Q : 0; }
+int64_t wa_design_n_levels(const wa_design* d, int32_t q) {
+  return (d && q >= 0 && q < d->d.Q) ? d->d.levels[q] : 0;
+}
+
+wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** out) {
+  try {
+    if (!d || !p || !out) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (p->krylov == 2) throw Error(WA_ERR_UNSUPPORTED, "LSMR is not provided by within_amd");
+    if (p->krylov != 0 && p->krylov != 1) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
+    if (p->preconditioner < 0 || p->preconditioner > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown preconditioner");
+    if (p->preconditioner == 2 && p->krylov == 0)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
+    if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
+    if (p->local.ac_split < 1 || p->local.schur_split < 1)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "split must be >= 1");
+    auto s = std::make_unique<wa_solver>();
+    s->params = *p;
+    s->host_only = p->host_only != 0;
+    Plan& pl = s->plan;
+    pl.d = d->d;
+    pl.cfg.ac_seed = p->local.ac_seed;
+    pl.cfg.ac_split = p->local.ac_split;
+    pl.cfg.approx_schur = p->local.approx_schur != 0;
+    pl.cfg.schur_seed = p->local.schur_seed;
+    pl.cfg.schur_split = p->local.schur_split;
+    pl.cfg.dense_threshold = p->local.dense_threshold;
+    pl.cfg.dense_core = p->local.dense_core == 0 ? 2048 : p->local.dense_core;
+    pl.precond = p->preconditioner;
+    pl.explicit_op = p->operator_repr == 1;
+    build_plan(pl);
+    if (!s->host_only) s->setup_device();
+    *out = s.release();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+void wa_solver_destroy(wa_solver* s) { delete s; }
+
+static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X, int64_t ldx,
+                            double* dem, int64_t ldd, wa_rhs_stats* stats, bool device) {
+  try {
+    need_device(s);
+    if (k < 0 || !stats || (k > 0 && (!Y || !X))) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (ldy < s->n || ldx < s->m || (dem && ldd < s->n))
+      throw Error(WA_ERR_OBS_COUNT_MISMATCH, "ObservationCountMismatch: leading dimension too small");
+    if (k == 0) return WA_OK;
+    const double* dY = Y;
+    double *dX = X, *dD = dem;
+    if (!device) {
+      int64_t need = ldy * k + s->m * (int64_t)k + (dem ? s->n * (int64_t)k : 0);
+      if (need > s->ubuf_len) {
+        if (s->ubuf) HIPCHK(hipFree(s->ubuf));
+        s->ubuf = nullptr;
+        HIPCHK(hipMalloc(&s->ubuf, sizeof(double) * need));
+        s->ubuf_len = need;
+      }
+      double* y = s->ubuf;
+      HIPCHK(hipMemcpyAsync(y, Y, sizeof(double) * ldy * k, hipMemcpyHostToDevice, s->st));
+      dY = y;
+      dX = y + ldy * k;
+      dD = dem ? dX + s->m * (int64_t)k : nullptr;
+    }
+    const int64_t lx = device ? ldx : s->m, ld = device ? ldd : s->n;
+    int blk = s->params.block <= 0 ? KMAX : std::min(s->params.block, KMAX);
+    for (int c0 = 0; c0 < k; c0 += blk) {
+      int kb = std::min(blk, k - c0);
+      s->solve_block(dY, ldy, c0, kb, dX, lx, dD, ld, stats);
+    }
+    if (!device) {
+      for (int c = 0; c < k; ++c)
+        HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, dX + (int64_t)c * s->m, sizeof(double) * s->m,
+                              hipMemcpyDeviceToHost, s->st));
+      if (dem)
+        for (int c = 0; c < k; ++c)
+          HIPCHK(hipMemcpyAsync(dem + (int64_t)c * ldd, dD + (int64_t)c * s->n, sizeof(double) * s->n,
+                                hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_solve_batch(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X, int64_t ldx,
+                         double* demeaned, int64_t ldd, wa_rhs_stats* stats) {
+  return solve_impl(s, Y, k, ldy, X, ldx, demeaned, ldd, stats, false);
+}
+wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X, int64_t ldx,
+                                double* demeaned, int64_t ldd, wa_rhs_stats* stats) {
+  return solve_impl(s, Y, k, ldy, X, ldx, demeaned, ldd, stats, true);
+}
+
+wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  return level_a(s, X, s ? s->m : 0, ld, Y, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+    int saved = s->params.operator_repr;
+    s->params.operator_repr = 0;
+    s->op_apply<double>(s->cp64, s->cap64, kp);
+    s->params.operator_repr = saved;
+    launch_row_to_col(s->st, s->cap64, kp, s->m, k, dout, s->m);
+  });
+}
+
+wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  if (s && !s->has_gram) {
+    g_err = "solver has no explicit Gramian (build with operator_repr = 1 or multiplicative Schwarz)";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  return level_a(s, X, s ? s->m : 0, ld, Y, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+    launch_spmm<double>(s->st, s->g64, s->cp64, s->cap64, s->part64, kp);
+    launch_row_to_col(s->st, s->cap64, kp, s->m, k, dout, s->m);
+  });
+}
+
+wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld) {
+  if (s && !s->has_pre) {
+    g_err = "solver has no preconditioner";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  return level_a(s, R, s ? s->m : 0, ld, Z, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->m, s->m, k, s->cr64, kp);
+    s->pre_apply<double>(s->cr64, s->cz64, kp);
+    launch_row_to_col(s->st, s->cz64, kp, s->m, k, dout, s->m);
+  });
+}
+
+wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k) {
+  return level_a(s, X, s ? s->m : 0, ldx, Y, s ? s->n : 0, ldy, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+    launch_demean(s->st, s->n, s->Q, s->codes, s->offs, s->cp64, nullptr, s->E, kp);
+    launch_row_to_col(s->st, s->E, kp, s->n, k, dout, s->n);
+  });
+}
+
+wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
+  return level_a(s, R, s ? s->n : 0, ldr, X, s ? s->m : 0, ldx, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->n, s->n, k, s->Yr, kp);
+    s->rhs64(s->Yr, s->B64, kp);
+    launch_row_to_col(s->st, s->B64, kp, s->m, k, dout, s->m);
+  });
+}
+
+wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
+  if (!s || !info) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
+  const PrecondFlat& f = s->plan.pre;
+  std::memset(info, 0, sizeof(int64_t) * 16);
+  info[0] = (int64_t)s->plan.subs.size();
+  info[1] = f.n_loc;
+  info[2] = f.n_red;
+  info[3] = f.lvl_ptr.empty() ? 0 : (int64_t)f.lvl_ptr.size() - 2;
+  info[4] = f.ac_nnz;
+  info[5] = (int64_t)f.core_idx.size();
+  info[6] = f.core_ptr.empty() ? 0 : (int64_t)f.core_ptr.size() - 1;
+  info[7] = f.c_nnz;
+  info[8] = s->plan.has_gram ? (int64_t)s->plan.gram.indices.size() : 0;
+  info[9] = s->plan.d.n_dofs;
+  info[10] = s->plan.d.n_obs;
+  info[11] = (int64_t)s->plan.passes.size();
+  info[12] = f.dense_subs;
+  info[13] = f.max_core;
+  return WA_OK;
+}
+
+wa_status wa_solver_subdomain(const wa_solver* s, int64_t i, int64_t* meta, uint32_t* gidx, double* pou) {
+  if (!s || !meta || i < 0 || i >= (int64_t)s->plan.subs.size()) {
+    g_err = "bad subdomain index";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const Subdomain& sd = s->plan.subs[i];
+  meta[0] = sd.q; meta[1] = sd.r; meta[2] = sd.ct.n_q(); meta[3] = sd.ct.n_r();
+  meta[4] = sd.eliminate_q; meta[5] = sd.dense;
+  meta[6] = (int64_t)sd.f.round_ptr.size() - 1;
+  meta[7] = (int64_t)sd.f.core.size();
+  if (gidx) std::memcpy(gidx, sd.gidx.data(), sizeof(uint32_t) * sd.gidx.size());
+  if (pou)
+    for (size_t k = 0; k < sd.gidx.size(); ++k) pou[k] = sd.pou.empty() ? 1.0 : sd.pou[k];
+  return WA_OK;
+}
+
+wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz_out, double* pivot,
+                                     uint32_t* col_ptr, uint32_t* col_idx, double* col_w, uint32_t* round_ptr,
+                                     uint32_t* order, uint32_t* core, double* core_inv) {
+  if (!s || i < 0 || i >= (int64_t)s->plan.subs.size()) {
+    g_err = "bad subdomain index";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const Factor& f = s->plan.subs[i].f;
+  if (nnz_out) *nnz_out = (int64_t)f.col_idx.size();
+  if (pivot) std::memcpy(pivot, f.pivot.data(), sizeof(double) * f.pivot.size());
+  if (col_ptr) std::memcpy(col_ptr, f.col_ptr.data(), sizeof(uint32_t) * f.col_ptr.size());
+  if (col_idx) std::memcpy(col_idx, f.col_idx.data(), sizeof(uint32_t) * f.col_idx.size());
+  if (col_w) std::memcpy(col_w, f.col_w.data(), sizeof(double) * f.col_w.size());
+  if (round_ptr) std::memcpy(round_ptr, f.round_ptr.data(), sizeof(uint32_t) * f.round_ptr.size());
+  if (order) std::memcpy(order, f.order.data(), sizeof(uint32_t) * f.order.size());
+  if (core) std::memcpy(core, f.core.data(), sizeof(uint32_t) * f.core.size());
+  if (core_inv) std::memcpy(core_inv, f.core_inv.data(), sizeof(double) * f.core_inv.size());
+  return WA_OK;
+}
+
+void wa_timing_enable(wa_solver* s, int32_t on) {
+  if (s) s->timer.on = on != 0 && !s->host_only;
+}
+void wa_timing_reset(wa_solver* s) {
+  if (!s) return;
+  for (int i = 0; i < 5; ++i) { s->timer.launches[i] = 0; s->timer.ms[i] = 0; }
+}
+wa_status wa_timing_get(wa_solver* s, int32_t kclass, int64_t* launches, double* ms) {
+  if (!s || kclass < 0 || kclass > 4) { g_err = "bad timing class"; return WA_ERR_INVALID_ARGUMENT; }
+  if (launches) *launches = s->timer.launches[kclass];
+  if (ms) *ms = s->timer.ms[kclass];
+  return WA_OK;
+}
+
+}  // extern "C"
