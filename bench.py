@@ -63,13 +63,18 @@ def device_Y(p, k, torch, seed=21):
     return Y
 
 
+def rust_default_precond(W):
+    """Rust LocalSolverConfig::default() (config.rs:96-104): AC split 1, sampled Schur."""
+    return W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+
+
 class Run:
     def __init__(self, p, cfg):
         import within_amd as W
 
         self.W = W
         t0 = time.perf_counter()
-        self.s = W.Solver(p["codes"], config=cfg)
+        self.s = W.Solver(p["codes"], config=cfg, preconditioner=rust_default_precond(W))
         self.setup_s = time.perf_counter() - t0
         self.n, self.m = self.s.n_obs, self.s.n_dofs
 
@@ -96,11 +101,12 @@ def cpu_baseline(args):
     Y = synth.akm_response(p, k=args.cpu_k, beta_cols=args.cpu_k - 1)
     threads = os.cpu_count() or 1
     threads = min(threads, 16)
-    o = oracle.OracleSolver(p["codes"], ac_split=2)
+    o = oracle.OracleSolver(p["codes"], ac_split=1)
     t, conv = o.time_solve_batch(Y, threads=threads)
     return {"value": round(args.cpu_k / t, 4), "unit": "RHS/s", "cores": threads, "kind": "port",
             "sample": f"same AKM generator at n={p['codes'].shape[0]} ({scale:g} of 20M), k={args.cpu_k}, "
-                      f"additive Schwarz + CG FP64 (oracle/), setup excluded, {t:.2f} s",
+                      f"additive Schwarz (Rust default local solver) + CG FP64 (oracle/), setup excluded, "
+                      f"{t:.2f} s",
             "converged": int(conv.sum())}
 
 
@@ -166,18 +172,20 @@ def main():
 
     # --- FP64 reference solution (tight) for the 1e-8 check ---------------------
     log("bench: FP64 reference solve (tol 1e-11) ...")
-    ref = Run(p, W.CG(tol=1e-11, max_refinements=4))
+    ref = fast
+    ref.s.set_config(W.CG(tol=1e-11, max_refinements=4))
     Dref = torch.zeros((k, n), device="cuda", dtype=torch.float64)
     Xr = torch.zeros((k, m), device="cuda", dtype=torch.float64)
     _, st_ref = ref.solve(Y, k, torch, Xr, Dref)
     err = (torch.linalg.vector_norm(D_fast - Dref, dim=1) / torch.linalg.vector_norm(Dref, dim=1)).cpu().numpy()
     solved = [bool(st[j][0]) and err[j] <= 1e-8 for j in range(k)]
     log(f"bench: max rel err vs FP64 reference {err.max():.2e}; solved {sum(solved)}/{k}")
-    del ref, Xr
+    del Xr
 
     # --- plain FP64 Schwarz-PCG, one RHS per solve -------------------------------
     log("bench: plain FP64 k=1 baseline ...")
-    base = Run(p, W.CG(precision="fp64", block=1))
+    base = fast
+    base.s.set_config(W.CG(precision="fp64", block=1))
     nb = min(args.baseline_rhs, k)
     Xb = torch.zeros((nb, m), device="cuda", dtype=torch.float64)
     Db = torch.zeros((nb, n), device="cuda", dtype=torch.float64)
@@ -187,7 +195,6 @@ def main():
     solved_b = sum(bool(stb[j][0]) and errb[j] <= 1e-8 for j in range(nb))
     base_rate = solved_b / tb
     log(f"bench: baseline {nb} RHS in {tb:.2f}s ({base_rate:.2f} RHS/s), iters={[s[1] for s in stb]}")
-    del base
 
     value = sum(solved) / t_step
     n_op, ms_op = cls["operator"]

@@ -102,6 +102,11 @@ int64_t wa_design_n_levels(const wa_design* d, int32_t q);
 /* ---- solver (Solver::from_design, solver.rs:98-132) ----------------------- */
 wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** out);
 void wa_solver_destroy(wa_solver* s);
+/* Change the per-solve settings (Krylov method, tolerances, precision, block
+ * size) of a built solver; preconditioner settings are fixed at build.  No
+ * reference counterpart: it lets one built preconditioner serve several
+ * SolverParams (the reference rebuilds a Solver per SolverParams). */
+wa_status wa_solver_set_params(wa_solver* s, const wa_params* p);
 
 /* Solver::solve_batch (solver.rs:242-277; orchestrate.rs:172-184).
  * Y: n_obs x k column-major (ldy >= n_obs).  X: n_dofs x k (ldx >= n_dofs),

@@ -433,6 +433,7 @@ ACFactor ac_factor(const Csr& lap, uint64_t seed, int split, i64 dense_core) {
   uint64_t round = 0;
   std::vector<u32> sel;
   std::vector<uint64_t> hkey(n);
+  std::vector<uint64_t> blocked(n, 0);
   i64 cut = dense_core > 1 ? dense_core : 0;
   while (remaining > 0) {
     if (remaining <= cut) {
@@ -460,23 +461,24 @@ ACFactor ac_factor(const Csr& lap, uint64_t seed, int split, i64 dense_core) {
       }
       cut = 0;  // disconnected core: finish with plain rounds
     }
+    // greedy maximal independent set over alive vertices in (degree, hash, id) order
     sel.clear();
+    std::vector<u32> cand;
     for (i64 v = 0; v < n; ++v)
-      if (alive[v]) hkey[v] = mis_hash(seed, round, (uint64_t)v);
-    for (i64 v = 0; v < n; ++v) {
-      if (!alive[v]) continue;
-      size_t dv = adj[v].size();
-      bool ok = true;
-      for (auto& kv : adj[v]) {
-        u32 u = kv.first;
-        size_t du = adj[u].size();
-        if (du < dv || (du == dv && (hkey[u] < hkey[v] || (hkey[u] == hkey[v] && (i64)u < v)))) {
-          ok = false;
-          break;
-        }
-      }
-      if (ok) sel.push_back((u32)v);
+      if (alive[v]) { hkey[v] = mis_hash(seed, round, (uint64_t)v); cand.push_back((u32)v); }
+    std::sort(cand.begin(), cand.end(), [&](u32 a, u32 b) {
+      size_t da = adj[a].size(), db = adj[b].size();
+      if (da != db) return da < db;
+      if (hkey[a] != hkey[b]) return hkey[a] < hkey[b];
+      return a < b;
+    });
+    for (u32 v : cand) {
+      if (blocked[v] == round + 1) continue;
+      sel.push_back(v);
+      blocked[v] = round + 1;
+      for (auto& kv : adj[v]) blocked[kv.first] = round + 1;
     }
+    std::sort(sel.begin(), sel.end());
     std::vector<Edge> fill;
     std::vector<std::pair<u32, double>> nb;
     for (u32 v : sel) {

@@ -58,7 +58,7 @@ class RhsStats(C.Structure):
 
 EXPORTS = [
     "wa_last_error", "wa_params_default", "wa_design_create", "wa_design_destroy", "wa_design_n_dofs",
-    "wa_design_n_obs", "wa_design_n_factors", "wa_design_n_levels", "wa_solver_create", "wa_solver_destroy",
+    "wa_design_n_obs", "wa_design_n_factors", "wa_design_n_levels", "wa_solver_create", "wa_solver_destroy", "wa_solver_set_params",
     "wa_solve_batch", "wa_solve_batch_device", "wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply",
     "wa_matvec_d", "wa_rmatvec_wdt", "wa_solver_info", "wa_solver_subdomain", "wa_solver_subdomain_factor",
     "wa_timing_enable", "wa_timing_reset", "wa_timing_get",
@@ -93,6 +93,7 @@ def load():
     L.wa_design_n_levels.argtypes = [P, i32]
     L.wa_design_n_levels.restype = i64
     L.wa_solver_create.argtypes = [P, C.POINTER(Params), C.POINTER(P)]
+    L.wa_solver_set_params.argtypes = [P, C.POINTER(Params)]
     L.wa_solver_destroy.argtypes = [P]
     L.wa_solver_destroy.restype = None
     for nm in ("wa_solve_batch", "wa_solve_batch_device"):

@@ -349,6 +349,13 @@ class Solver:
             N.load().wa_solver_destroy(self._h)
             self._h = None
 
+    def set_config(self, config):
+        """Switch the Krylov settings (CG/GMRES, tol, maxiter, refinements,
+        precision, block) of this built solver; the preconditioner stays."""
+        p, _ = _params(config, self._precond_arg)
+        N.check(N.load().wa_solver_set_params(self._h, C.byref(p)))
+        self._params = p
+
     @property
     def n_dofs(self):
         return self._design.n_dofs

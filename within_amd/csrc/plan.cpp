@@ -3,12 +3,32 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
 
 namespace wa {
+
+// phase timings on stderr when WITHIN_AMD_VERBOSE is set
+static bool verbose() {
+  static int v = -1;
+  if (v < 0) v = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+  return v == 1;
+}
+struct PhaseTimer {
+  const char* name;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit PhaseTimer(const char* n) : name(n) {}
+  ~PhaseTimer() {
+    if (verbose())
+      std::fprintf(stderr, "[within_amd] %s: %.3f s\n", name,
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
+};
 
 enum { E_EMPTY = 1, E_OVERFLOW = 4, E_SINGULAR = 5, E_LOCAL = 6, E_INVALID = 9 };
 
@@ -396,8 +416,8 @@ static inline uint64_t round_key(uint64_t seed, uint64_t round, uint64_t v) {
 }
 
 // Rounds of independent eliminations on a Laplacian (off-diagonals = -w):
-// v is eliminated in round t when (degree, key(t, v), v) is smaller than that
-// of every remaining neighbour; fill = clique tree of its neighbourhood (seed
+// round t takes a greedy maximal independent set, visiting vertices in
+// (degree, key(t, v), v) order; fill = clique tree of its neighbourhood (seed
 // + v), merged after the round in (v ascending, emission) order.  Once at most
 // `dense_core` vertices remain they are factored exactly (anchored at the
 // last one); a core whose anchored minor is singular falls back to rounds.
@@ -414,7 +434,7 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
   f.pivot.assign(n, 0.0);
   std::vector<std::vector<std::pair<u32, double>>> cols(n);
   std::vector<char> live(n, 1);
-  std::vector<uint64_t> key(n);
+  std::vector<uint64_t> key(n), taken(n, 0);
   i64 left = n, cut = dense_core > 1 ? dense_core : 0;
   uint64_t round = 0;
   f.round_ptr.push_back(0);
@@ -444,22 +464,22 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       }
       cut = 0;
     }
-    for (i64 v = 0; v < n; ++v)
-      if (live[v]) key[v] = round_key(seed, round, (uint64_t)v);
+    // greedy maximal independent set in (degree, key, id) order
     pick.clear();
-    for (i64 v = 0; v < n; ++v) {
-      if (!live[v]) continue;
-      size_t dv = adj[v].size();
-      bool mine = true;
-      for (const auto& kv : adj[v]) {
-        size_t du = adj[kv.first].size();
-        if (du < dv || (du == dv && (key[kv.first] < key[v] || (key[kv.first] == key[v] && (i64)kv.first < v)))) {
-          mine = false;
-          break;
-        }
-      }
-      if (mine) pick.push_back((u32)v);
+    std::vector<u32> order;
+    for (i64 v = 0; v < n; ++v)
+      if (live[v]) { key[v] = round_key(seed, round, (uint64_t)v); order.push_back((u32)v); }
+    std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+      size_t da = adj[a].size(), db = adj[b].size();
+      return da != db ? da < db : (key[a] != key[b] ? key[a] < key[b] : a < b);
+    });
+    for (u32 v : order) {
+      if (taken[v] == round + 1) continue;
+      pick.push_back(v);
+      taken[v] = round + 1;
+      for (const auto& kv : adj[v]) taken[kv.first] = round + 1;
     }
+    std::sort(pick.begin(), pick.end());
     std::vector<Edge> fill;
     std::vector<std::pair<u32, double>> nb;
     for (u32 v : pick) {
@@ -822,20 +842,34 @@ static void flatten(Plan& p) {
 void build_plan(Plan& p) {
   const Design& d = p.d;
   p.passes.resize(d.Q);
+  {
+    PhaseTimer t("operator passes");
 #pragma omp parallel for schedule(dynamic, 1)
-  for (int q = 0; q < d.Q; ++q) p.passes[q] = build_pass(d, q);
+    for (int q = 0; q < d.Q; ++q) p.passes[q] = build_pass(d, q);
+  }
   if (p.explicit_op || p.precond == 2) {
+    PhaseTimer t("explicit gramian");
     p.gram = build_gramian(d);
     p.has_gram = true;
   }
   if (p.precond == 1 || p.precond == 2) {
-    p.subs = build_subdomains(d);
+    {
+      PhaseTimer t("subdomains");
+      p.subs = build_subdomains(d);
+    }
+    PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (size_t i = 0; i < p.subs.size(); ++i) {
       try {
+        auto t0 = std::chrono::steady_clock::now();
         build_local_solver(p.subs[i], p.cfg);
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (verbose() && dt > 0.05)
+          std::fprintf(stderr, "[within_amd]   subdomain %zu (n_q=%lld n_r=%lld): %.3f s, rounds %zu core %zu\n", i,
+                       (long long)p.subs[i].ct.n_q(), (long long)p.subs[i].ct.n_r(), dt,
+                       p.subs[i].f.round_ptr.size() - 1, p.subs[i].f.core.size());
       } catch (const Error& e) {
         err[i] = e.what();
         code[i] = e.code;
@@ -843,6 +877,7 @@ void build_plan(Plan& p) {
     }
     for (size_t i = 0; i < p.subs.size(); ++i)
       if (code[i]) throw Error(code[i], err[i]);
+    PhaseTimer tf("flatten");
     flatten(p);
   }
 }

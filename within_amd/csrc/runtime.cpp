@@ -736,6 +736,25 @@ wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** o
 
 void wa_solver_destroy(wa_solver* s) { delete s; }
 
+wa_status wa_solver_set_params(wa_solver* s, const wa_params* p) {
+  try {
+    if (!s || !p) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    const wa_params& o = s->params;
+    if (p->preconditioner != o.preconditioner || std::memcmp(&p->local, &o.local, sizeof(wa_local_config)) != 0)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner settings are fixed when the solver is built");
+    if (p->operator_repr == 1 && !s->has_gram)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no explicit Gramian");
+    if (p->krylov == 2) throw Error(WA_ERR_UNSUPPORTED, "LSMR is not provided by within_amd");
+    if (p->krylov == 0 && p->preconditioner == 2)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner");
+    if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
+    s->params = *p;
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
 static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X, int64_t ldx,
                             double* dem, int64_t ldd, wa_rhs_stats* stats, bool device) {
   try {
