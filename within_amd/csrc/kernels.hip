@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "engine.hpp"
 #include "kernels.hpp"
 
 namespace wa {
@@ -22,121 +23,14 @@ static inline unsigned blocks_for(int64_t groups, int kp) {
   int64_t threads = groups * kp;
   return (unsigned)((threads + kBlock - 1) / kBlock);
 }
-
-// ===========================================================================
-// implicit operator: one level-sorted pass per factor
-//   Y[off_q + j] = diag_q[j] X[off_q + j] + sum_{e in level j} w_e sum_{r != q} X[other_r(e)]
-// (GramianOperator::apply, operator/gramian.rs:84-89, as D'W D x)
-// ===========================================================================
-template <typename T, bool WEIGHTED>
-__global__ void __launch_bounds__(kBlock) k_pull(const Item* __restrict__ items, int64_t n_items,
-                                                 const uint32_t* __restrict__ row_ptr,
-                                                 const uint32_t* __restrict__ other, int qm,
-                                                 const T* __restrict__ ew, const T* __restrict__ diag,
-                                                 uint32_t off, const T* __restrict__ X,
-                                                 T* __restrict__ Y, T* __restrict__ part, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_items) return;
-  const Item it = items[grp];
-  if (it.slot < 0) {
-    for (uint32_t r = it.row0; r < it.row1; ++r) {
-      T acc = diag[r] * X[(int64_t)(off + r) * kp + lane];
-      const uint32_t e1 = row_ptr[r + 1];
-      for (uint32_t e = row_ptr[r]; e < e1; ++e) {
-        T s = 0;
-        for (int t = 0; t < qm; ++t) s += X[(int64_t)other[(int64_t)e * qm + t] * kp + lane];
-        acc += WEIGHTED ? ew[e] * s : s;
-      }
-      Y[(int64_t)(off + r) * kp + lane] = acc;
-    }
-  } else {
-    T acc = 0;
-    for (uint32_t e = it.e0; e < it.e1; ++e) {
-      T s = 0;
-      for (int t = 0; t < qm; ++t) s += X[(int64_t)other[(int64_t)e * qm + t] * kp + lane];
-      acc += WEIGHTED ? ew[e] * s : s;
-    }
-    part[(int64_t)it.slot * kp + lane] = acc;
-  }
+static inline unsigned grid_stride_blocks(int64_t n, int kp) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(n, kp), 65536));
 }
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pull_fix(const Fixup* __restrict__ fx, int64_t n_fix,
-                                                     const T* __restrict__ diag, uint32_t off,
-                                                     const T* __restrict__ X, T* __restrict__ Y,
-                                                     const T* __restrict__ part, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_fix) return;
-  const Fixup f = fx[grp];
-  T acc = diag ? diag[f.row] * X[(int64_t)(off + f.row) * kp + lane] : T(0);
-  for (uint32_t s = f.slot0; s < f.slot1; ++s) acc += part[(int64_t)s * kp + lane];
-  Y[(int64_t)(off + f.row) * kp + lane] = acc;
-}
-
-template <typename T>
-void launch_pull(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp) {
-  const DevSched& sc = kp >= 8 ? p.wide : p.narrow;
-  if (sc.n_items > 0) {
-    if (p.ew)
-      k_pull<T, true><<<blocks_for(sc.n_items, kp), kBlock, 0, st>>>(
-          sc.items, sc.n_items, p.row_ptr, p.other, p.qm, p.ew, p.diag, p.off, X, Y, part, kp);
-    else
-      k_pull<T, false><<<blocks_for(sc.n_items, kp), kBlock, 0, st>>>(
-          sc.items, sc.n_items, p.row_ptr, p.other, p.qm, nullptr, p.diag, p.off, X, Y, part, kp);
-  }
-  if (sc.n_fix > 0)
-    k_pull_fix<T><<<blocks_for(sc.n_fix, kp), kBlock, 0, st>>>(sc.fix, sc.n_fix, p.diag, p.off, X, Y,
-                                                               part, kp);
-}
-template void launch_pull<float>(hipStream_t, const DevPass<float>&, const float*, float*, float*, int);
-template void launch_pull<double>(hipStream_t, const DevPass<double>&, const double*, double*, double*, int);
 
 // ===========================================================================
 // observation space (FP64): D'W V and V = Y - D X, row-major n x kp
 // (rmatvec_wdt / matvec_d, domain.rs:390-418)
 // ===========================================================================
-__global__ void __launch_bounds__(kBlock) k_scatter_obs(const Item* __restrict__ items, int64_t n_items,
-                                                        const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ obs,
-                                                        const double* __restrict__ w,
-                                                        const double* __restrict__ V, uint32_t off,
-                                                        double* __restrict__ out,
-                                                        double* __restrict__ part, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_items) return;
-  const Item it = items[grp];
-  if (it.slot < 0) {
-    for (uint32_t r = it.row0; r < it.row1; ++r) {
-      double acc = 0.0;
-      for (uint32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
-        const uint32_t i = obs[e];
-        const double v = V[(int64_t)i * kp + lane];
-        acc += w ? w[i] * v : v;
-      }
-      out[(int64_t)(off + r) * kp + lane] = acc;
-    }
-  } else {
-    double acc = 0.0;
-    for (uint32_t e = it.e0; e < it.e1; ++e) {
-      const uint32_t i = obs[e];
-      const double v = V[(int64_t)i * kp + lane];
-      acc += w ? w[i] * v : v;
-    }
-    part[(int64_t)it.slot * kp + lane] = acc;
-  }
-}
-
-void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const uint32_t* obs, const double* w,
-                        const double* V, double* out, double* part, int kp) {
-  const DevSched& sc = kp >= 8 ? p.wide : p.narrow;
-  if (sc.n_items > 0)
-    k_scatter_obs<<<blocks_for(sc.n_items, kp), kBlock, 0, st>>>(sc.items, sc.n_items, p.row_ptr, obs, w,
-                                                                 V, p.off, out, part, kp);
-  if (sc.n_fix > 0)
-    k_pull_fix<double><<<blocks_for(sc.n_fix, kp), kBlock, 0, st>>>(sc.fix, sc.n_fix, nullptr, p.off,
-                                                                    nullptr, out, part, kp);
-}
-
 // E[i] = Y[i] - sum_q X[off_q + code_q(i)]   (Y == nullptr: E = D X)
 __global__ void __launch_bounds__(kBlock) k_demean(int64_t n, int Q, const uint32_t* __restrict__ codes,
                                                    const uint32_t* __restrict__ offs,
@@ -471,55 +365,6 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const uint32_t* __restrict_
     B[l * kp + lane] = scale[l] * R[(int64_t)gidx[l] * kp + lane];
 }
 
-// per-segment sums of V over rows [ptr[s], ptr[s+1])  -> out[s]
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_segsum(const Item* __restrict__ items, int64_t n_items,
-                                                   const uint32_t* __restrict__ ptr, const T* __restrict__ V,
-                                                   T* __restrict__ out, T* __restrict__ part, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_items) return;
-  const Item it = items[grp];
-  if (it.slot < 0) {
-    for (uint32_t s = it.row0; s < it.row1; ++s) {
-      T acc = 0;
-      for (uint32_t e = ptr[s]; e < ptr[s + 1]; ++e) acc += V[(int64_t)e * kp + lane];
-      out[(int64_t)s * kp + lane] = acc;
-    }
-  } else {
-    T acc = 0;
-    for (uint32_t e = it.e0; e < it.e1; ++e) acc += V[(int64_t)e * kp + lane];
-    part[(int64_t)it.slot * kp + lane] = acc;
-  }
-}
-
-template <typename T>
-void launch_segsum(hipStream_t st, const DevSched& sc, const uint32_t* ptr, const T* V, T* out, T* part, int kp) {
-  if (sc.n_items > 0)
-    k_segsum<T><<<blocks_for(sc.n_items, kp), kBlock, 0, st>>>(sc.items, sc.n_items, ptr, V, out, part, kp);
-  if (sc.n_fix > 0)
-    k_pull_fix<T><<<blocks_for(sc.n_fix, kp), kBlock, 0, st>>>(sc.fix, sc.n_fix, nullptr, 0, nullptr, out, part,
-                                                               kp);
-}
-
-// keep rows: Rr[j] = B[keep_j] - mu1 + sum_e val_e B[e] - mu1 * rowsum_j
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_keep_rows(int64_t n_red, const uint32_t* __restrict__ red_loc,
-                                                      const uint32_t* __restrict__ red_sub,
-                                                      const uint32_t* __restrict__ loc_ptr,
-                                                      const uint32_t* __restrict__ kptr,
-                                                      const uint32_t* __restrict__ kidx,
-                                                      const T* __restrict__ kval, const T* __restrict__ krs,
-                                                      const T* __restrict__ B, const T* __restrict__ sum1,
-                                                      T* __restrict__ Rr, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_red) return;
-  const uint32_t s = red_sub[grp];
-  const T mu = sum1[(int64_t)s * kp + lane] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
-  T acc = 0;
-  for (uint32_t e = kptr[grp]; e < kptr[grp + 1]; ++e) acc += kval[e] * B[(int64_t)kidx[e] * kp + lane];
-  Rr[grp * kp + lane] = B[(int64_t)red_loc[grp] * kp + lane] - mu + acc - mu * krs[grp];
-}
-
 // Rr[j] -= sum2[s] / n_red(s)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, const uint32_t* __restrict__ sub,
@@ -533,53 +378,6 @@ __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, const uint32_t* 
   }
 }
 
-// forward level: z_v = z_v + sum_{a earlier} c_av z_a  (in place)
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fwd_level(int64_t nrows, const uint32_t* __restrict__ rows,
-                                                      const uint32_t* __restrict__ iptr,
-                                                      const uint32_t* __restrict__ iidx,
-                                                      const T* __restrict__ ival, T* __restrict__ Z, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= nrows) return;
-  const uint32_t v = rows[grp];
-  T acc = Z[(int64_t)v * kp + lane];
-  for (uint32_t e = iptr[grp]; e < iptr[grp + 1]; ++e) acc += ival[e] * Z[(int64_t)iidx[e] * kp + lane];
-  Z[(int64_t)v * kp + lane] = acc;
-}
-
-// backward level: x_v = z_v / d_v + sum_{u later} (w/d_v) x_u  (in place)
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_bwd_level(int64_t nrows, const uint32_t* __restrict__ rows,
-                                                      const uint32_t* __restrict__ optr,
-                                                      const uint32_t* __restrict__ oidx,
-                                                      const T* __restrict__ oval, const T* __restrict__ inv_piv,
-                                                      T* __restrict__ Z, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= nrows) return;
-  const uint32_t v = rows[grp];
-  T acc = inv_piv[v] * Z[(int64_t)v * kp + lane];
-  for (uint32_t e = optr[grp]; e < optr[grp + 1]; ++e) acc += oval[e] * Z[(int64_t)oidx[e] * kp + lane];
-  Z[(int64_t)v * kp + lane] = acc;
-}
-
-// dense cores: out[cr] = sum_j Minv_c[row, j] z[core_j]
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_core(int64_t n_rows, const uint32_t* __restrict__ row_core,
-                                                 const uint32_t* __restrict__ cptr,
-                                                 const uint32_t* __restrict__ cidx,
-                                                 const uint64_t* __restrict__ moff, const T* __restrict__ mat,
-                                                 const T* __restrict__ Z, T* __restrict__ out, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_rows) return;
-  const uint32_t c = row_core[grp];
-  const uint32_t b = cptr[c], n = cptr[c + 1] - b;
-  const uint32_t row = (uint32_t)grp - b;
-  const T* M = mat + moff[c] + (uint64_t)row * n;
-  T acc = 0;
-  for (uint32_t j = 0; j < n; ++j) acc += M[j] * Z[(int64_t)cidx[b + j] * kp + lane];
-  out[grp * kp + lane] = acc;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_scatter_rows(int64_t n, const uint32_t* __restrict__ idx,
                                                          const T* __restrict__ src, T* __restrict__ dst, int kp) {
@@ -587,28 +385,6 @@ __global__ void __launch_bounds__(kBlock) k_scatter_rows(int64_t n, const uint32
   const int gpb = kBlock / kp;
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * gpb)
     dst[(int64_t)idx[i] * kp + lane] = src[i * kp + lane];
-}
-
-// elimination rows: S[e] = (B[e] - mu1) inv_e + inv_e * sum_j C[e,j] x_keep[j]
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_elim_rows(int64_t n_elim, const uint32_t* __restrict__ eloc,
-                                                      const T* __restrict__ einv,
-                                                      const uint32_t* __restrict__ eptr,
-                                                      const uint32_t* __restrict__ eidx,
-                                                      const T* __restrict__ eval,
-                                                      const uint32_t* __restrict__ loc_sub,
-                                                      const uint32_t* __restrict__ loc_ptr,
-                                                      const T* __restrict__ B, const T* __restrict__ sum1,
-                                                      const T* __restrict__ Rr, T* __restrict__ S, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_elim) return;
-  const uint32_t l = eloc[grp];
-  const uint32_t s = loc_sub[l];
-  const T mu = sum1[(int64_t)s * kp + lane] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
-  T acc = 0;
-  for (uint32_t e = eptr[grp]; e < eptr[grp + 1]; ++e) acc += eval[e] * Rr[(int64_t)eidx[e] * kp + lane];
-  const T iv = einv[grp];
-  S[(int64_t)l * kp + lane] = (B[(int64_t)l * kp + lane] - mu) * iv + iv * acc;
 }
 
 // Z[d] = sum_{l in dof d} scale_l (S[l] - mu3(sub(l)))   (0 for uncovered DOFs)
@@ -634,53 +410,296 @@ __global__ void __launch_bounds__(kBlock) k_combine(int64_t m, const uint32_t* _
   }
 }
 
-static inline unsigned grid_stride_blocks(int64_t n, int kp) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(n, kp), 65536));
+// ===========================================================================
+// row kinds for the engine (engine.hpp)
+// ===========================================================================
+// implicit operator pass of factor q (GramianOperator::apply,
+// operator/gramian.rs:84-89, as D'W D x):
+//   Y[off + j] = diag_j X[off + j] + sum_{e in level j} w_e sum_{t<qm} X[other_t(e)]
+template <typename T, int QM_, bool W>
+struct OpKind {
+  static constexpr int QM = QM_;
+  static constexpr bool COEF = W;
+  const T* X;
+  T* Y;
+  const uint32_t* row_ptr;
+  const uint32_t* other;
+  const T* ew;
+  const T* diag;
+  uint32_t off;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return row_ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return row_ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+#pragma unroll
+    for (int t = 0; t < QM; ++t) id[t] = other[(int64_t)e * QM + t];
+    if (W) c = ew[e];
+  }
+  __device__ void finish(uint32_t r, int col, T acc) const {
+    const int64_t o = (int64_t)(off + r) * kp + col;
+    Y[o] = diag[r] * X[o] + acc;
+  }
+};
+
+template <typename T, int QM, bool W>
+static void run_op(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp) {
+  OpKind<T, QM, W> k{X, Y, p.row_ptr, p.other, p.ew, p.diag, p.off, kp};
+  run_rows<OpKind<T, QM, W>, T>(st, k, p.sched, part, kp);
 }
+
+template <typename T>
+void launch_pull(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp) {
+  const bool w = p.ew != nullptr;
+  switch (p.qm) {
+    case 0: w ? run_op<T, 0, true>(st, p, X, Y, part, kp) : run_op<T, 0, false>(st, p, X, Y, part, kp); break;
+    case 1: w ? run_op<T, 1, true>(st, p, X, Y, part, kp) : run_op<T, 1, false>(st, p, X, Y, part, kp); break;
+    case 2: w ? run_op<T, 2, true>(st, p, X, Y, part, kp) : run_op<T, 2, false>(st, p, X, Y, part, kp); break;
+    case 3: w ? run_op<T, 3, true>(st, p, X, Y, part, kp) : run_op<T, 3, false>(st, p, X, Y, part, kp); break;
+    case 4: w ? run_op<T, 4, true>(st, p, X, Y, part, kp) : run_op<T, 4, false>(st, p, X, Y, part, kp); break;
+    case 5: w ? run_op<T, 5, true>(st, p, X, Y, part, kp) : run_op<T, 5, false>(st, p, X, Y, part, kp); break;
+    case 6: w ? run_op<T, 6, true>(st, p, X, Y, part, kp) : run_op<T, 6, false>(st, p, X, Y, part, kp); break;
+    case 7: w ? run_op<T, 7, true>(st, p, X, Y, part, kp) : run_op<T, 7, false>(st, p, X, Y, part, kp); break;
+    default: break;  // more than 8 factors is rejected when the solver is built
+  }
+}
+template void launch_pull<float>(hipStream_t, const DevPass<float>&, const float*, float*, float*, int);
+template void launch_pull<double>(hipStream_t, const DevPass<double>&, const double*, double*, double*, int);
+
+// observation-space D'W V for factor q (rmatvec_wdt, domain.rs:409-418)
+template <bool W>
+struct ObsKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = W;
+  const double* X;  // V, row-major n x kp
+  double* out;
+  const uint32_t* row_ptr;
+  const uint32_t* obs;
+  const double* w;
+  uint32_t off;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return row_ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return row_ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, double& c) const {
+    const uint32_t i = obs[e];
+    id[0] = i;
+    if (W) c = w[i];
+  }
+  __device__ void finish(uint32_t r, int col, double acc) const { out[(int64_t)(off + r) * kp + col] = acc; }
+};
+
+void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* w, const double* V, double* out,
+                        double* part, int kp) {
+  if (w) {
+    ObsKind<true> k{V, out, p.row_ptr, p.obs, w, p.off, kp};
+    run_rows<ObsKind<true>, double>(st, k, p.sched, part, kp);
+  } else {
+    ObsKind<false> k{V, out, p.row_ptr, p.obs, nullptr, p.off, kp};
+    run_rows<ObsKind<false>, double>(st, k, p.sched, part, kp);
+  }
+}
+
+// explicit Gramian rows (gramian/explicit.rs:261-277)
+template <typename T>
+struct SpmmKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  const T* X;
+  T* Y;
+  const uint32_t* ptr;
+  const uint32_t* idx;
+  const T* val;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = idx[e];
+    c = val[e];
+  }
+  __device__ void finish(uint32_t r, int col, T acc) const { Y[(int64_t)r * kp + col] = acc; }
+};
+
+template <typename T>
+void launch_spmm(hipStream_t st, const DevCsr<T>& g, const T* X, T* Y, T* part, int kp) {
+  SpmmKind<T> k{X, Y, g.ptr, g.idx, g.val, kp};
+  run_rows<SpmmKind<T>, T>(st, k, g.sched, part, kp);
+}
+template void launch_spmm<float>(hipStream_t, const DevCsr<float>&, const float*, float*, float*, int);
+template void launch_spmm<double>(hipStream_t, const DevCsr<double>&, const double*, double*, double*, int);
+
+// per-subdomain sums over contiguous ranges
+template <typename T>
+struct SegKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = false;
+  const T* X;
+  T* out;
+  const uint32_t* ptr;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T&) const { id[0] = e; }
+  __device__ void finish(uint32_t r, int col, T acc) const { out[(int64_t)r * kp + col] = acc; }
+};
+
+// Schur right-hand side rows (local_solver.rs:375-389):
+//   Rr[j] = B[keep_j] - mu1 + sum_e (C[e,j]/D_e) B[e] - mu1 * rowsum_j
+template <typename T>
+struct KeepKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  const T* X;  // B
+  T* Rr;
+  const uint32_t *ptr, *idx;
+  const T *val, *rowsum;
+  const uint32_t *red_loc, *red_sub, *loc_ptr;
+  const T* sum1;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = idx[e];
+    c = val[e];
+  }
+  __device__ void finish(uint32_t j, int col, T acc) const {
+    const uint32_t s = red_sub[j];
+    const T mu = sum1[(int64_t)s * kp + col] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
+    Rr[(int64_t)j * kp + col] = X[(int64_t)red_loc[j] * kp + col] - mu + acc - mu * rowsum[j];
+  }
+};
+
+// back-substitution rows (local_solver.rs:111-151, :400-447):
+//   S[e] = (B[e] - mu1) / D_e + (1/D_e) sum_j C[e,j] x_keep[j]
+template <typename T>
+struct ElimKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  const T* X;  // Rr (solved reduced system)
+  T* S;
+  const uint32_t *ptr, *idx;
+  const T *val, *inv;
+  const uint32_t *eloc, *loc_sub, *loc_ptr;
+  const T *B, *sum1;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = idx[e];
+    c = val[e];
+  }
+  __device__ void finish(uint32_t i, int col, T acc) const {
+    const uint32_t l = eloc[i];
+    const uint32_t s = loc_sub[l];
+    const T mu = sum1[(int64_t)s * kp + col] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
+    const T iv = inv[i];
+    S[(int64_t)l * kp + col] = (B[(int64_t)l * kp + col] - mu) * iv + iv * acc;
+  }
+};
+
+// forward elimination level: z_v += sum_{a earlier} (w_av / d_a) z_a  (in place)
+template <typename T>
+struct FwdKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  T* X;  // Z
+  const uint32_t *rows, *ptr, *idx;
+  const T* val;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = idx[e];
+    c = val[e];
+  }
+  __device__ void finish(uint32_t r, int col, T acc) const { X[(int64_t)rows[r] * kp + col] += acc; }
+};
+
+// backward level: x_v = z_v / d_v + sum_{u later} (w_vu / d_v) x_u  (in place)
+template <typename T>
+struct BwdKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  T* X;
+  const uint32_t *rows, *ptr, *idx;
+  const T *val, *inv_piv;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = idx[e];
+    c = val[e];
+  }
+  __device__ void finish(uint32_t r, int col, T acc) const {
+    const int64_t o = (int64_t)rows[r] * kp + col;
+    X[o] = inv_piv[rows[r]] * X[o] + acc;
+  }
+};
+
+// dense cores: C[row] = sum_j Minv_c[row_local, j] z[core_j]  (rows of all cores)
+template <typename T>
+struct CoreKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  const T* X;  // Z
+  T* out;
+  const uint32_t *row_ptr, *row_core, *cptr, *cidx;
+  const uint64_t* moff;
+  const T* mat;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return row_ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return row_ptr[r + 1]; }
+  __device__ void load(uint32_t r, uint32_t e, uint32_t* id, T& c) const {
+    const uint32_t cc = row_core[r];
+    const uint32_t b = cptr[cc], n = cptr[cc + 1] - b;
+    const uint32_t j = e - row_ptr[r];
+    id[0] = cidx[b + j];
+    c = mat[moff[cc] + (uint64_t)(r - b) * n + j];
+  }
+  __device__ void finish(uint32_t r, int col, T acc) const { out[(int64_t)r * kp + col] = acc; }
+};
 
 template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m) {
-  // 1. restrict + per-subdomain sums (mu1)
+  // 1. restrict (signed, partition-weighted) and per-subdomain sums -> mu1
   k_restrict<T><<<grid_stride_blocks(p.n_loc, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, R, w.B, p.n_loc, kp);
-  launch_segsum<T>(st, p.sched_loc, p.loc_ptr, w.B, w.sum1, w.part, kp);
-  // 2. keep rows (Schur right-hand side) + sums (mu2), then centre
+  run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, p.s_loc, w.part, kp);
   if (p.n_red > 0) {
-    k_keep_rows<T><<<blocks_for(p.n_red, kp), kBlock, 0, st>>>(p.n_red, p.red_loc, p.red_sub, p.loc_ptr, p.keep_ptr,
-                                                               p.keep_idx, p.keep_val, p.keep_rowsum, w.B, w.sum1,
-                                                               w.Rr, kp);
-    launch_segsum<T>(st, p.sched_red, p.red_ptr, w.Rr, w.sum2, w.part, kp);
+    // 2. Schur right-hand side, centred (mu2)
+    run_rows<KeepKind<T>, T>(st,
+                             KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
+                                         p.red_sub, p.loc_ptr, w.sum1, kp},
+                             p.s_keep, w.part, kp);
+    run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, p.s_red, w.part, kp);
     k_sub_mean<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(p.n_red, p.red_sub, p.red_ptr, w.sum2, w.Rr,
                                                                       kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
-    for (int64_t t = 0; t < p.n_levels; ++t) {
-      int64_t b = p.h_lvl_ptr[t], e = p.h_lvl_ptr[t + 1];
-      if (e > b)
-        k_fwd_level<T><<<blocks_for(e - b, kp), kBlock, 0, st>>>(e - b, p.lvl_rows + b, p.fin_ptr + b, p.fin_idx,
-                                                                 p.fin_val, w.Rr, kp);
-    }
+    FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
+    for (int64_t t = 0; t < p.n_levels; ++t)
+      run_rows<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.h_item_ptr[t], p.s_fwd.h_item_ptr[t + 1],
+                              p.s_fwd.h_fix_ptr[t], p.s_fwd.h_fix_ptr[t + 1], w.part, kp);
     if (p.n_core_rows > 0) {
-      k_core<T><<<blocks_for(p.n_core_rows, kp), kBlock, 0, st>>>(p.n_core_rows, p.core_row_core, p.core_ptr,
-                                                                  p.core_idx, p.core_mat_off, p.core_mat, w.Rr,
-                                                                  w.C, kp);
+      run_rows<CoreKind<T>, T>(st,
+                               CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
+                                           p.core_mat_off, p.core_mat, kp},
+                               p.s_core, w.part, kp);
       k_scatter_rows<T><<<grid_stride_blocks(p.n_core_rows, kp), kBlock, 0, st>>>(p.n_core_rows, p.core_idx, w.C,
                                                                                   w.Rr, kp);
     }
-    for (int64_t t = p.n_levels - 2; t >= 0; --t) {
-      int64_t b = p.h_lvl_ptr[t], e = p.h_lvl_ptr[t + 1];
-      if (e > b)
-        k_bwd_level<T><<<blocks_for(e - b, kp), kBlock, 0, st>>>(e - b, p.lvl_rows + b, p.bout_ptr + b, p.bout_idx,
-                                                                 p.bout_val, p.inv_pivot, w.Rr, kp);
-    }
-    // 4. back-substitution for the eliminated block; kept block copied over
+    BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
+    for (int64_t t = p.n_levels - 2; t >= 0; --t)
+      run_rows<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.h_item_ptr[t], p.s_bwd.h_item_ptr[t + 1],
+                              p.s_bwd.h_fix_ptr[t], p.s_bwd.h_fix_ptr[t + 1], w.part, kp);
+    // 4. kept block copied into the local solution
     k_scatter_rows<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(p.n_red, p.red_loc, w.Rr, w.S, kp);
   }
+  // 5. back-substitution for the eliminated block
   if (p.n_elim > 0)
-    k_elim_rows<T><<<blocks_for(p.n_elim, kp), kBlock, 0, st>>>(p.n_elim, p.elim_loc, p.elim_inv, p.elim_ptr,
-                                                                p.elim_idx, p.elim_val, p.loc_sub, p.loc_ptr, w.B,
-                                                                w.sum1, w.Rr, w.S, kp);
-  // 5. centre the local solutions (mu3), sign, partition weights, combine
-  launch_segsum<T>(st, p.sched_loc, p.loc_ptr, w.S, w.sum3, w.part, kp);
+    run_rows<ElimKind<T>, T>(st,
+                             ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
+                                         p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
+                             p.s_elim, w.part, kp);
+  // 6. centre (mu3), sign and partition weights, combine per DOF
+  run_rows<SegKind<T>, T>(st, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, p.s_loc, w.part, kp);
   k_combine<T><<<grid_stride_blocks(m, kp), kBlock, 0, st>>>(m, p.dof_ptr, p.dof_loc, p.loc_scale, p.loc_sub,
                                                              p.loc_ptr, w.S, w.sum3, Z, kp);
 }
@@ -688,41 +707,5 @@ template void launch_precond<float>(hipStream_t, const DevPrecond<float>&, const
                                     PrecondWork<float>&, int, int64_t);
 template void launch_precond<double>(hipStream_t, const DevPrecond<double>&, const double*, double*,
                                      PrecondWork<double>&, int, int64_t);
-
-// ===========================================================================
-// explicit Gramian SpMM: Y[row] = sum_e val_e X[col_e]   (explicit.rs:261-277)
-// ===========================================================================
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmm(const Item* __restrict__ items, int64_t n_items,
-                                                 const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ idx,
-                                                 const T* __restrict__ val, const T* __restrict__ X,
-                                                 T* __restrict__ Y, T* __restrict__ part, int kp) {
-  GROUP_SETUP(kp)
-  if (grp >= n_items) return;
-  const Item it = items[grp];
-  if (it.slot < 0) {
-    for (uint32_t r = it.row0; r < it.row1; ++r) {
-      T acc = 0;
-      for (uint32_t e = ptr[r]; e < ptr[r + 1]; ++e) acc += val[e] * X[(int64_t)idx[e] * kp + lane];
-      Y[(int64_t)r * kp + lane] = acc;
-    }
-  } else {
-    T acc = 0;
-    for (uint32_t e = it.e0; e < it.e1; ++e) acc += val[e] * X[(int64_t)idx[e] * kp + lane];
-    part[(int64_t)it.slot * kp + lane] = acc;
-  }
-}
-
-template <typename T>
-void launch_spmm(hipStream_t st, const DevCsr<T>& g, const T* X, T* Y, T* part, int kp) {
-  const DevSched& sc = kp >= 8 ? g.wide : g.narrow;
-  if (sc.n_items > 0)
-    k_spmm<T><<<blocks_for(sc.n_items, kp), kBlock, 0, st>>>(sc.items, sc.n_items, g.ptr, g.idx, g.val, X, Y, part,
-                                                             kp);
-  if (sc.n_fix > 0)
-    k_pull_fix<T><<<blocks_for(sc.n_fix, kp), kBlock, 0, st>>>(sc.fix, sc.n_fix, nullptr, 0, nullptr, Y, part, kp);
-}
-template void launch_spmm<float>(hipStream_t, const DevCsr<float>&, const float*, float*, float*, int);
-template void launch_spmm<double>(hipStream_t, const DevCsr<double>&, const double*, double*, double*, int);
 
 }  // namespace wa

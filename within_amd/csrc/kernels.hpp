@@ -10,8 +10,10 @@
 
 namespace wa {
 
-constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
-constexpr int kRedBlocks = 1024; // fixed reduction grid (deterministic sums)
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
+constexpr uint32_t kChunk = 128;  // elements per work item (long rows are split)
+constexpr uint32_t kMaxRows = 32; // rows packed into one work item
 
 struct DevSched {
   const Item* items = nullptr;
@@ -19,6 +21,7 @@ struct DevSched {
   const Fixup* fix = nullptr;
   int64_t n_fix = 0;
   int64_t n_slots = 0;
+  std::vector<int64_t> h_item_ptr, h_fix_ptr;  // per-level ranges (level schedules only)
 };
 
 // one level-sorted factor pass (plan.hpp FactorPass) in precision T
@@ -32,7 +35,7 @@ struct DevPass {
   int qm = 0;
   uint32_t off = 0;
   int64_t levels = 0, n_elem = 0;
-  DevSched wide, narrow;
+  DevSched sched;
 };
 
 template <typename T>
@@ -41,7 +44,7 @@ struct DevCsr {
   const uint32_t* idx = nullptr;
   const T* val = nullptr;
   int64_t nrows = 0, nnz = 0;
-  DevSched wide, narrow;
+  DevSched sched;
 };
 
 template <typename T>
@@ -55,12 +58,11 @@ struct DevPrecond {
   const T *keep_val, *keep_rowsum;
   const uint32_t *lvl_rows, *fin_ptr, *fin_idx, *bout_ptr, *bout_idx;
   const T *fin_val, *bout_val, *inv_pivot;
-  const uint32_t *core_ptr, *core_idx, *core_row_core;
+  const uint32_t *core_ptr, *core_idx, *core_row_core, *core_row_ptr;
   const uint64_t* core_mat_off;
   const T* core_mat;
   const uint32_t *dof_ptr, *dof_loc;
-  DevSched sched_loc, sched_red;
-  std::vector<uint32_t> h_lvl_ptr;  // host copy of level boundaries
+  DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
 };
 
 // scratch for one preconditioner apply (kp columns)
@@ -78,8 +80,8 @@ struct ColState {
 
 template <typename T>
 void launch_pull(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp);
-void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const uint32_t* obs, const double* w,
-                        const double* V, double* out, double* part, int kp);
+void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* w, const double* V, double* out,
+                        double* part, int kp);
 void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
                    const double* X, const double* Y, double* E, int kp);
 void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp);

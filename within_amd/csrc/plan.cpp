@@ -63,11 +63,9 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
 // ---------------------------------------------------------------------------
 // segmented-reduction work items
 // ---------------------------------------------------------------------------
-Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows) {
-  Schedule s;
-  u32 nrows = (u32)row_ptr.size() - 1;
-  u32 r = 0;
-  while (r < nrows) {
+void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk, u32 max_rows) {
+  u32 r = r0;
+  while (r < r1) {
     u32 len = row_ptr[r + 1] - row_ptr[r];
     if (len > chunk) {
       u32 slot0 = (u32)s.n_slots;
@@ -79,16 +77,21 @@ Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows)
       ++r;
       continue;
     }
-    u32 r1 = r, tot = 0;
-    while (r1 < nrows && r1 - r < max_rows) {
-      u32 l = row_ptr[r1 + 1] - row_ptr[r1];
-      if (l > chunk || (r1 > r && tot + l > chunk)) break;
+    u32 q = r, tot = 0;
+    while (q < r1 && q - r < max_rows) {
+      u32 l = row_ptr[q + 1] - row_ptr[q];
+      if (l > chunk || (q > r && tot + l > chunk)) break;
       tot += l;
-      ++r1;
+      ++q;
     }
-    s.items.push_back({r, r1, row_ptr[r], row_ptr[r1], -1, 0});
-    r = r1;
+    s.items.push_back({r, q, row_ptr[r], row_ptr[q], -1, 0});
+    r = q;
   }
+}
+
+Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows) {
+  Schedule s;
+  append_schedule(s, row_ptr, 0, (u32)row_ptr.size() - 1, chunk, max_rows);
   return s;
 }
 
@@ -133,8 +136,6 @@ static FactorPass build_pass(const Design& d, int q) {
   fp.diag.assign(d.levels[q], 0.0);
   for (i64 j = 0; j < d.levels[q]; ++j)
     for (u32 t = fp.row_ptr[j]; t < fp.row_ptr[j + 1]; ++t) fp.diag[j] += d.w(idx[t]);
-  fp.sched_wide = make_schedule(fp.row_ptr, 256, 64);
-  fp.sched_narrow = make_schedule(fp.row_ptr, 64, 64);
   return fp;
 }
 
@@ -835,8 +836,6 @@ static void flatten(Plan& p) {
     std::vector<u32> cur(f.dof_ptr.begin(), f.dof_ptr.end() - 1);
     for (i64 l = 0; l < f.n_loc; ++l) f.dof_loc[cur[f.loc_gidx[l]]++] = (u32)l;
   }
-  f.sched_loc = make_schedule(f.loc_ptr, 1024, 64);
-  f.sched_red = make_schedule(f.red_ptr, 1024, 64);
 }
 
 void build_plan(Plan& p) {

@@ -59,6 +59,8 @@ struct Schedule {
   i64 n_slots = 0;
 };
 Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows);
+// items for rows [r0, r1) only, appended to `s` (slots continue from s.n_slots)
+void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk, u32 max_rows);
 
 // Level-sorted CSR of one factor: rows = levels of q, elements = observations
 // sorted by (level, code of the largest other factor, observation index).
@@ -69,7 +71,6 @@ struct FactorPass {
   std::vector<u32> other;     // element-major, (Q-1) global DOF indices each
   std::vector<double> w;      // element weights (empty = unit)
   std::vector<double> diag;   // sum of weights per level
-  Schedule sched_wide, sched_narrow;
 };
 
 struct Csr {
@@ -145,8 +146,6 @@ struct PrecondFlat {
   std::vector<u32> core_row_core;     // per core row -> core id
   // DOF combine: dof -> local entries (subdomain order)
   std::vector<u32> dof_ptr, dof_loc;
-  // segment schedules for per-subdomain sums
-  Schedule sched_loc, sched_red;
   i64 max_core = 0, ac_nnz = 0, c_nnz = 0, dense_subs = 0;
 };
 

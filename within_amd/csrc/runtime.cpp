@@ -249,25 +249,21 @@ void wa_solver::setup_device() {
     a.off = b.off = (uint32_t)d.offset[fp.q];
     a.levels = b.levels = d.levels[fp.q];
     a.n_elem = b.n_elem = n;
-    a.wide = b.wide = upload_sched(fp.sched_wide);
-    a.narrow = b.narrow = upload_sched(fp.sched_narrow);
-    max_slots = std::max({max_slots, a.wide.n_slots, a.narrow.n_slots});
+    a.sched = b.sched = upload_sched(make_schedule(fp.row_ptr, kChunk, kMaxRows));
+    max_slots = std::max(max_slots, a.sched.n_slots);
     p64.push_back(a);
     p32.push_back(b);
   }
   if (plan.has_gram) {
     const Csr& g = plan.gram;
-    std::vector<uint32_t> ptr(g.indptr.begin(), g.indptr.end());
-    Schedule wide = make_schedule(ptr, 256, 64), narrow = make_schedule(ptr, 64, 64);
     g64.ptr = g32.ptr = upload(g.indptr);
     g64.idx = g32.idx = upload(g.indices);
     g64.val = upload(g.data);
     g32.val = upload_as<float>(g.data);
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
-    g64.wide = g32.wide = upload_sched(wide);
-    g64.narrow = g32.narrow = upload_sched(narrow);
-    max_slots = std::max({max_slots, g64.wide.n_slots, g64.narrow.n_slots});
+    g64.sched = g32.sched = upload_sched(make_schedule(g.indptr, kChunk, kMaxRows));
+    max_slots = std::max(max_slots, g64.sched.n_slots);
     has_gram = true;
   }
   if (!plan.subs.empty()) {
@@ -280,7 +276,6 @@ void wa_solver::setup_device() {
       P.n_elim = (int64_t)f.elim_loc.size();
       P.n_levels = (int64_t)f.lvl_ptr.size() - 1;
       P.n_core_rows = (int64_t)f.core_idx.size();
-      P.h_lvl_ptr = f.lvl_ptr;
       P.loc_scale = upload_as<T>(f.loc_scale);
       P.elim_inv = upload_as<T>(f.elim_inv);
       P.elim_val = upload_as<T>(f.elim_val);
@@ -293,10 +288,6 @@ void wa_solver::setup_device() {
     };
     fill(pre64, double());
     fill(pre32, float());
-    auto share = [&](auto& P) {
-      P.sched_loc = upload_sched(f.sched_loc);
-      P.sched_red = upload_sched(f.sched_red);
-    };
     // integer arrays shared by both precisions
     const uint32_t* loc_gidx = upload(f.loc_gidx);
     const uint32_t* loc_ptr = upload(f.loc_ptr);
@@ -328,12 +319,47 @@ void wa_solver::setup_device() {
       P.core_ptr = core_ptr; P.core_idx = core_idx; P.core_row_core = core_row_core;
       P.core_mat_off = core_mat_off; P.dof_ptr = dof_ptr; P.dof_loc = dof_loc;
     };
+    // dense-core rows: row r of core c gathers |c| elements
+    std::vector<uint32_t> core_row_ptr(1, 0);
+    for (size_t c = 0; c + 1 < f.core_ptr.size(); ++c) {
+      uint64_t nc = f.core_ptr[c + 1] - f.core_ptr[c];
+      for (uint64_t r = 0; r < nc; ++r) {
+        if ((uint64_t)core_row_ptr.back() + nc >= UINT32_MAX) throw Error(WA_ERR_OVERFLOW, "dense cores too large");
+        core_row_ptr.push_back(core_row_ptr.back() + (uint32_t)nc);
+      }
+    }
+    const uint32_t* d_core_row_ptr = upload(core_row_ptr);
+    auto level_sched = [&](const std::vector<uint32_t>& ptr) {
+      Schedule sc;
+      std::vector<int64_t> ip(1, 0), fp(1, 0);
+      for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) {
+        append_schedule(sc, ptr, f.lvl_ptr[t], f.lvl_ptr[t + 1], kChunk, kMaxRows);
+        ip.push_back((int64_t)sc.items.size());
+        fp.push_back((int64_t)sc.fixups.size());
+      }
+      DevSched d = upload_sched(sc);
+      d.h_item_ptr = ip;
+      d.h_fix_ptr = fp;
+      return d;
+    };
+    DevSched s_loc = upload_sched(make_schedule(f.loc_ptr, kChunk, kMaxRows));
+    DevSched s_red = upload_sched(make_schedule(f.red_ptr, kChunk, kMaxRows));
+    DevSched s_keep = upload_sched(make_schedule(f.keep_ptr, kChunk, kMaxRows));
+    DevSched s_elim = upload_sched(make_schedule(f.elim_ptr, kChunk, kMaxRows));
+    DevSched s_fwd = level_sched(f.fin_ptr);
+    DevSched s_bwd = level_sched(f.bout_ptr);
+    DevSched s_core = upload_sched(make_schedule(core_row_ptr, kChunk, kMaxRows));
+    for (const DevSched* d : {&s_loc, &s_red, &s_keep, &s_elim, &s_fwd, &s_bwd, &s_core})
+      max_slots = std::max(max_slots, d->n_slots);
+    auto scheds = [&](auto& P) {
+      P.core_row_ptr = d_core_row_ptr;
+      P.s_loc = s_loc; P.s_red = s_red; P.s_keep = s_keep; P.s_elim = s_elim;
+      P.s_fwd = s_fwd; P.s_bwd = s_bwd; P.s_core = s_core;
+    };
     ints(pre64);
     ints(pre32);
-    share(pre64);
-    pre32.sched_loc = pre64.sched_loc;
-    pre32.sched_red = pre64.sched_red;
-    max_slots = std::max({max_slots, pre64.sched_loc.n_slots, pre64.sched_red.n_slots});
+    scheds(pre64);
+    scheds(pre32);
     int64_t nsub = std::max<int64_t>(f.n_sub, 1);
     w64.B = dalloc<double>(f.n_loc * KMAX);
     w64.S = dalloc<double>(f.n_loc * KMAX);
@@ -406,7 +432,7 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp) {
 void wa_solver::rhs64(const double* V, double* out, int kp) {
   hipEvent_t a = nullptr;
   tic(3, a);
-  for (int q = 0; q < Q; ++q) launch_scatter_obs(st, p64[q], p64[q].obs, wobs, V, out, part64, kp);
+  for (int q = 0; q < Q; ++q) launch_scatter_obs(st, p64[q], wobs, V, out, part64, kp);
   toc(3, a);
 }
 
@@ -709,6 +735,7 @@ wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** o
     if (p->preconditioner == 2 && p->krylov == 0)
       throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
     if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
+    if (d->d.Q > 8) throw Error(WA_ERR_UNSUPPORTED, "at most 8 factors are supported");
     if (p->local.ac_split < 1 || p->local.schur_split < 1)
       throw Error(WA_ERR_INVALID_ARGUMENT, "split must be >= 1");
     auto s = std::make_unique<wa_solver>();
