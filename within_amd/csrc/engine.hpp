@@ -3,18 +3,18 @@
 // A "row" is a sum over elements: out(row) = finish(row, sum_e coef(e) *
 // sum_{t<QM} X[idx_t(e)]).  Rows are processed through a host-built schedule
 // of work items: an item covers whole rows (finished in place) or one chunk of
-// a long row (written to a partial slot and summed in slot order by the
-// fix-up kernel), so long rows never serialise a group and the result is
+// a long row (written to a partial slot; a fix-up sums the slots of the row in
+// slot order), so long rows never serialise a wave and every result is
 // bitwise reproducible.
 //
-// Two lane mappings:
-//   COLS  (kp >= 8): a group of kp lanes, lane = column.  Each lane loads the
-//         indices of up to kp consecutive elements in one coalesced load and
-//         the group walks them with __shfl broadcasts, so the row gathers of
-//         one batch are independent and in flight together.
-//   ELEMS (kp in {1,2,4}): a group of 8 lanes, lane = element slot; each lane
-//         accumulates all kp columns of its elements, then an xor-butterfly
-//         over the 8 lanes (fixed order) forms the row sum.
+// Lane mapping (one wave64 per work item): a block-vector row holds kp values
+// of type T; each lane owns NV consecutive values (a 16-byte vector when the
+// row is at least 16 bytes), V = kp / NV lanes cover one row, and the wave
+// works on S = 64 / V elements at once ("slots").  Every lane loads its
+// element's indices itself (the V lanes of a slot read the same address, so
+// the wave issues S distinct index loads), then its vector slices of the
+// gathered rows: no cross-lane traffic until the row ends, where the S slot
+// partials are combined by a fixed xor-butterfly.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,122 +23,116 @@
 
 namespace wa {
 
-constexpr int kElemGroup = 8;
+constexpr int kWave = 64;
 
-// ---- COLS mode -------------------------------------------------------------
-template <class K, typename T>
-__device__ __forceinline__ T gather_cols(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int lane, int kp) {
-  T acc = 0;
+template <typename T, int NV>
+struct alignas(sizeof(T) * NV) VecT {
+  T v[NV];
+};
+
+template <typename T, int NV>
+__device__ __forceinline__ VecT<T, NV> vload(const T* p) {
+  return *reinterpret_cast<const VecT<T, NV>*>(p);
+}
+
+// V lanes per row, NV values per lane (V * NV == kp)
+template <class K, typename T, int V, int NV>
+__device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int slot, int part,
+                                           T (&acc)[NV]) {
+  constexpr int S = kWave / V;
   constexpr int QM = K::QM > 0 ? K::QM : 1;
-  for (uint32_t b = e0; b < e1; b += (uint32_t)kp) {
-    const uint32_t cnt = min((uint32_t)kp, e1 - b);
-    uint32_t id[QM];
-    T c = 0;
+  constexpr int KP = V * NV;
+  constexpr int U = 2;  // elements per lane in flight together
 #pragma unroll
-    for (int t = 0; t < QM; ++t) id[t] = 0;
-    if ((uint32_t)lane < cnt) k.load(row, b + lane, id, c);
-#pragma unroll 4
-    for (uint32_t j = 0; j < cnt; ++j) {
-      T s = 0;
+  for (int c = 0; c < NV; ++c) acc[c] = 0;
+  if (K::QM == 0) return;
+  for (uint32_t e = e0 + slot; e < e1; e += U * S) {
+    uint32_t id[U][QM];
+    T cf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cf[u] = 1;
+#pragma unroll
+      for (int t = 0; t < QM; ++t) id[u][t] = 0;
+      if (e + u * S < e1) k.load(row, e + u * S, id[u], cf[u]);
+    }
+    VecT<T, NV> g[U][QM];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int t = 0; t < QM; ++t) {
-        const uint32_t ix = (uint32_t)__shfl((int)id[t], (int)j, kp);
-        s += k.X[(int64_t)ix * kp + lane];
-      }
-      if (K::COEF) {
-        const T cj = __shfl(c, (int)j, kp);
-        acc += cj * s;
-      } else {
-        acc += s;
-      }
-    }
-  }
-  return acc;
-}
-
-template <class K, typename T>
-__global__ void __launch_bounds__(kBlock) k_rows_cols(K k, const Item* __restrict__ items, int64_t n_items,
-                                                      T* __restrict__ part, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kp;
-  if (g >= n_items) return;
-  const Item it = items[g];
-  if (it.slot < 0) {
-    for (uint32_t r = it.row0; r < it.row1; ++r) {
-      T acc = K::QM == 0 ? T(0) : gather_cols<K, T>(k, r, k.rbegin(r), k.rend(r), lane, kp);
-      k.finish(r, lane, acc);
-    }
-  } else {
-    T acc = K::QM == 0 ? T(0) : gather_cols<K, T>(k, it.row0, it.e0, it.e1, lane, kp);
-    part[(int64_t)it.slot * kp + lane] = acc;
-  }
-}
-
-// ---- ELEMS mode ------------------------------------------------------------
-template <int KP, class K, typename T>
-__device__ __forceinline__ void gather_elems(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int sub,
-                                             T (&acc)[KP]) {
-  constexpr int QM = K::QM > 0 ? K::QM : 1;
+        if (e + u * S < e1) {
+          g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
+        } else {
 #pragma unroll
-  for (int c = 0; c < KP; ++c) acc[c] = 0;
-  if (K::QM > 0) {
-    for (uint32_t e = e0 + sub; e < e1; e += kElemGroup) {
-      uint32_t id[QM];
-      T cf = 1;
-      k.load(row, e, id, cf);
+          for (int c = 0; c < NV; ++c) g[u][t].v[c] = 0;
+        }
+      }
 #pragma unroll
-      for (int c = 0; c < KP; ++c) {
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
         T s = 0;
 #pragma unroll
-        for (int t = 0; t < QM; ++t) s += k.X[(int64_t)id[t] * KP + c];
-        acc[c] += K::COEF ? cf * s : s;
+        for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
+        if (e + u * S < e1) acc[c] += K::COEF ? cf[u] * s : s;
       }
-    }
   }
+  // combine the S slots (lanes V apart) in a fixed order
 #pragma unroll
-  for (int off = kElemGroup / 2; off >= 1; off >>= 1)
+  for (int off = V; off < kWave; off <<= 1)
 #pragma unroll
-    for (int c = 0; c < KP; ++c) acc[c] += __shfl_xor(acc[c], off, kElemGroup);
+    for (int c = 0; c < NV; ++c) acc[c] += __shfl_xor(acc[c], off, kWave);
 }
 
-template <int KP, class K, typename T>
-__global__ void __launch_bounds__(kBlock) k_rows_elems(K k, const Item* __restrict__ items, int64_t n_items,
-                                                       T* __restrict__ part) {
-  const int sub = threadIdx.x & (kElemGroup - 1);
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kElemGroup;
+template <class K, typename T, int V, int NV>
+__global__ void __launch_bounds__(kBlock) k_rows_vec(K k, const Item* __restrict__ items, int64_t n_items,
+                                                     T* __restrict__ part) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int slot = lane / V, pt = lane % V;
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (g >= n_items) return;
   const Item it = items[g];
-  T acc[KP];
+  T acc[NV];
   if (it.slot < 0) {
     for (uint32_t r = it.row0; r < it.row1; ++r) {
-      gather_elems<KP, K, T>(k, r, k.rbegin(r), k.rend(r), sub, acc);
+      gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
+      if (slot == 0)
 #pragma unroll
-      for (int c = 0; c < KP; ++c)
-        if (sub == c) k.finish(r, c, acc[c]);
+        for (int c = 0; c < NV; ++c) k.finish(r, pt * NV + c, acc[c]);
     }
   } else {
-    gather_elems<KP, K, T>(k, it.row0, it.e0, it.e1, sub, acc);
+    gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
+    if (slot == 0)
 #pragma unroll
-    for (int c = 0; c < KP; ++c)
-      if (sub == c) part[(int64_t)it.slot * KP + c] = acc[c];
+      for (int c = 0; c < NV; ++c) part[(int64_t)it.slot * (V * NV) + pt * NV + c] = acc[c];
   }
 }
 
-// ---- fix-up: sum the partial slots of each split row in slot order ----------
+// ---- fix-up: one workgroup per split row; threads stride over the slots,
+// then a fixed-order tree over the workgroup ----------------------------------
 template <class K, typename T>
-__global__ void __launch_bounds__(kBlock) k_rows_fix(K k, const Fixup* __restrict__ fx, int64_t n_fix,
-                                                     const T* __restrict__ part, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kp;
-  if (g >= n_fix) return;
-  const Fixup f = fx[g];
+__global__ void __launch_bounds__(kBlock) k_rows_fix(K k, const Fixup* __restrict__ fx, const T* __restrict__ part,
+                                                     int kp) {
+  __shared__ T sh[kBlock];
+  const Fixup f = fx[blockIdx.x];
+  const int col = threadIdx.x % kp, str = kBlock / kp, s0 = threadIdx.x / kp;
   T acc = 0;
-  for (uint32_t s = f.slot0; s < f.slot1; ++s) acc += part[(int64_t)s * kp + lane];
-  k.finish(f.row, lane, acc);
+  for (uint32_t s = f.slot0 + s0; s < f.slot1; s += str) acc += part[(int64_t)s * kp + col];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = str / 2; w >= 1; w >>= 1) {
+    if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
+    __syncthreads();
+  }
+  if (s0 == 0) k.finish(f.row, col, sh[threadIdx.x]);
 }
 
-inline unsigned engine_blocks(int64_t groups, int lanes) {
-  return (unsigned)((groups * lanes + kBlock - 1) / kBlock);
+inline unsigned engine_blocks(int64_t waves) { return (unsigned)((waves * kWave + kBlock - 1) / kBlock); }
+
+template <class K, typename T, int V, int NV>
+void launch_vec(hipStream_t st, const K& k, const Item* items, int64_t ni, T* part) {
+  k_rows_vec<K, T, V, NV><<<engine_blocks(ni), kBlock, 0, st>>>(k, items, ni, part);
 }
 
 // Run items [i0, i1) and fix-ups [f0, f1) of a schedule.
@@ -147,16 +141,23 @@ void run_rows(hipStream_t st, const K& k, const DevSched& sc, int64_t i0, int64_
               T* part, int kp) {
   const int64_t ni = i1 - i0, nf = f1 - f0;
   if (ni > 0) {
-    if (kp >= 8)
-      k_rows_cols<K, T><<<engine_blocks(ni, kp), kBlock, 0, st>>>(k, sc.items + i0, ni, part, kp);
-    else if (kp == 4)
-      k_rows_elems<4, K, T><<<engine_blocks(ni, kElemGroup), kBlock, 0, st>>>(k, sc.items + i0, ni, part);
-    else if (kp == 2)
-      k_rows_elems<2, K, T><<<engine_blocks(ni, kElemGroup), kBlock, 0, st>>>(k, sc.items + i0, ni, part);
-    else
-      k_rows_elems<1, K, T><<<engine_blocks(ni, kElemGroup), kBlock, 0, st>>>(k, sc.items + i0, ni, part);
+    const Item* it = sc.items + i0;
+    constexpr int NV16 = 16 / sizeof(T);  // values per 16-byte vector
+    if (kp * (int)sizeof(T) >= 16) {
+      switch (kp / NV16) {  // lanes per row
+        case 1: launch_vec<K, T, 1, NV16>(st, k, it, ni, part); break;
+        case 2: launch_vec<K, T, 2, NV16>(st, k, it, ni, part); break;
+        case 4: launch_vec<K, T, 4, NV16>(st, k, it, ni, part); break;
+        case 8: launch_vec<K, T, 8, NV16>(st, k, it, ni, part); break;
+        default: launch_vec<K, T, 16, NV16>(st, k, it, ni, part); break;
+      }
+    } else if (kp == 2) {
+      launch_vec<K, T, 1, 2>(st, k, it, ni, part);
+    } else {
+      launch_vec<K, T, 1, 1>(st, k, it, ni, part);
+    }
   }
-  if (nf > 0) k_rows_fix<K, T><<<engine_blocks(nf, kp), kBlock, 0, st>>>(k, sc.fix + f0, nf, part, kp);
+  if (nf > 0) k_rows_fix<K, T><<<(unsigned)nf, kBlock, 0, st>>>(k, sc.fix + f0, part, kp);
 }
 
 template <class K, typename T>

@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_dot_part(const T* __restrict__ A, co
   const int lane = threadIdx.x & (kp - 1);
   const int gpb = kBlock / kp;
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb)
     acc += (double)A[i * kp + lane] * (double)B[i * kp + lane];
   sh[threadIdx.x] = acc;
@@ -134,19 +135,28 @@ __global__ void __launch_bounds__(kBlock) k_dot_part(const T* __restrict__ A, co
   }
 }
 
-__global__ void k_dot_fin(const double* __restrict__ part, int nb, int kp, double* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= kp) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * kp + c];
-  out[c] = s;
+// out[c] = sum_b part[b * kp + c]: threads stride over the blocks, then a
+// fixed-order tree over the workgroup
+__global__ void __launch_bounds__(kBlock) k_dot_fin(const double* __restrict__ part, int nb, int kp,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  const int col = threadIdx.x % kp, str = kBlock / kp, s0 = threadIdx.x / kp;
+  double acc = 0.0;
+  for (int b = s0; b < nb; b += str) acc += part[(int64_t)b * kp + col];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = str / 2; w >= 1; w >>= 1) {
+    if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
+    __syncthreads();
+  }
+  if (s0 == 0) out[col] = sh[threadIdx.x];
 }
 
 template <typename T>
 void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out) {
   unsigned nb = red_blocks(m, kp);
   k_dot_part<T><<<nb, kBlock, 0, st>>>(A, B, m, kp, part);
-  k_dot_fin<<<1, 64, 0, st>>>(part, (int)nb, kp, out);
+  k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
 }
 template void launch_dot<float>(hipStream_t, const float*, const float*, int64_t, int, double*, double*);
 template void launch_dot<double>(hipStream_t, const double*, const double*, int64_t, int, double*, double*);
@@ -162,6 +172,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T
   const int gpb = kBlock / kp;
   const T a = (T)alpha[lane];
   double acc = 0.0;
+#pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
     const int64_t id = i * kp + lane;
     X[id] += a * P[id];
@@ -183,7 +194,7 @@ void launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const
                       int kp, double* part, double* out) {
   unsigned nb = red_blocks(m, kp);
   k_cg_update<T><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part);
-  k_dot_fin<<<1, 64, 0, st>>>(part, (int)nb, kp, out);
+  k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
 }
 template void launch_cg_update<float>(hipStream_t, float*, const float*, float*, const float*, const double*,
                                       int64_t, int, double*, double*);

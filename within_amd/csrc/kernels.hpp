@@ -11,8 +11,8 @@
 namespace wa {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
-constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
-constexpr uint32_t kChunk = 128;  // elements per work item (long rows are split)
+constexpr int kRedBlocks = 512;   // fixed reduction grid (deterministic sums)
+constexpr uint32_t kChunk = 256;  // elements per work item (long rows are split)
 constexpr uint32_t kMaxRows = 32; // rows packed into one work item
 
 struct DevSched {
