@@ -176,3 +176,65 @@ def test_batch_equals_single():
     for j in range(3):
         r1 = s.solve(Y[:, j])
         assert rel(r1.demeaned, rb.demeaned[:, j]) <= 1e-9
+
+
+# ---- GMRES, multiplicative Schwarz, explicit operator ---------------------
+@pytest.mark.parametrize("idx", [0, 2, 3], ids=["2fe-uniform", "akm-3fe", "akm-4fe-weighted"])
+def test_explicit_operator_matches_oracle(idx, oracle_mod):
+    codes, w, _ = PROBS[idx]
+    o = oracle_mod.OracleSolver(codes, w, precond=0)
+    ip, ix, dt = o.explicit_gramian()
+    m = o.n_dofs
+    G = np.zeros((m, m))
+    for r in range(m):
+        G[r, ix[ip[r]:ip[r + 1]]] = dt[ip[r]:ip[r + 1]]
+    s = W.Solver(codes, weights=w, config=W.CG(operator=W.OperatorRepr.Explicit), preconditioner=W.Preconditioner.Off)
+    X = np.asfortranarray(np.random.default_rng(idx).standard_normal((m, 5)))
+    Y = np.zeros((m, 5), order="F")
+    N.check(N.load().wa_gram_apply_explicit(s._h, N.ptr(X), N.ptr(Y), 5, m))
+    assert rel(Y, G @ X) <= 1e-12
+
+
+@pytest.mark.parametrize("idx", [0, 2, 3], ids=["2fe-uniform", "akm-3fe", "akm-4fe-weighted"])
+def test_multiplicative_apply_matches_oracle(idx, oracle_mod):
+    codes, w, _ = PROBS[idx]
+    o = oracle_mod.OracleSolver(codes, w, krylov=1, precond=2, ac_split=2)
+    s = W.Solver(codes, weights=w, config=W.GMRES(), preconditioner=W.MultiplicativeSchwarz())
+    m = s.n_dofs
+    R = np.asfortranarray(np.random.default_rng(50 + idx).standard_normal((m, 3)))
+    Z = np.zeros((m, 3), order="F")
+    N.check(N.load().wa_precond_apply(s._h, N.ptr(R), N.ptr(Z), 3, m))
+    ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(3)])
+    assert rel(Z, ref) <= 1e-10, rel(Z, ref)
+
+
+@pytest.mark.parametrize("idx", [1, 2, 3], ids=["3fe-weighted", "akm-3fe", "akm-4fe-weighted"])
+@pytest.mark.parametrize("kind", ["gmres-additive", "gmres-multiplicative", "gmres-mixed", "cg-explicit"])
+def test_other_solver_paths_match_oracle(idx, kind, oracle_mod):
+    codes, w, _ = PROBS[idx]
+    n = codes.shape[0]
+    Y = np.asfortranarray(np.random.default_rng(idx).standard_normal((n, 3)))
+    o = oracle_mod.OracleSolver(codes, w, ac_split=2, tol=1e-10, max_refinements=3)
+    ro = o.solve_batch(Y)
+    if kind == "gmres-additive":
+        cfg, prec = W.GMRES(tol=1e-10, max_refinements=3), None
+    elif kind == "gmres-multiplicative":
+        cfg, prec = W.GMRES(tol=1e-10, max_refinements=3), W.MultiplicativeSchwarz()
+    elif kind == "gmres-mixed":
+        cfg, prec = W.GMRES(tol=1e-10, max_refinements=3, precision="mixed"), None
+    else:
+        cfg, prec = W.CG(tol=1e-10, max_refinements=3, operator=W.OperatorRepr.Explicit), None
+    r = W.solve_batch(codes, Y, config=cfg, weights=w, preconditioner=prec)
+    assert all(r.converged)
+    for j in range(3):
+        assert rel(r.demeaned[:, j], ro["demeaned"][:, j]) <= 1e-8
+
+
+def test_gmres_iterations_track_oracle(oracle_mod):
+    # same operator and preconditioner, same GMRES recurrence: iteration counts agree
+    codes, w, _ = PROBS[2]
+    y = np.random.default_rng(9).standard_normal(codes.shape[0])
+    ro = oracle_mod.OracleSolver(codes, w, krylov=1, precond=2, ac_split=2).solve(y)
+    r = W.solve(codes, y, config=W.GMRES(), weights=w, preconditioner=W.MultiplicativeSchwarz())
+    assert r.converged and ro["converged"]
+    assert abs(r.iterations - ro["iterations"]) <= 1

@@ -369,32 +369,32 @@ void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* be
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_restrict(const uint32_t* __restrict__ gidx,
                                                      const T* __restrict__ scale, const T* __restrict__ R,
-                                                     T* __restrict__ B, int64_t n, int kp) {
+                                                     T* __restrict__ B, int64_t b0, int64_t n, int kp) {
   const int lane = threadIdx.x & (kp - 1);
   const int gpb = kBlock / kp;
-  for (int64_t l = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; l < n; l += (int64_t)gridDim.x * gpb)
+  for (int64_t l = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; l < n; l += (int64_t)gridDim.x * gpb)
     B[l * kp + lane] = scale[l] * R[(int64_t)gidx[l] * kp + lane];
 }
 
 // Rr[j] -= sum2[s] / n_red(s)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t n, const uint32_t* __restrict__ sub,
+__global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t b0, int64_t n, const uint32_t* __restrict__ sub,
                                                      const uint32_t* __restrict__ ptr, const T* __restrict__ sums,
                                                      T* __restrict__ V, int kp) {
   const int lane = threadIdx.x & (kp - 1);
   const int gpb = kBlock / kp;
-  for (int64_t j = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; j < n; j += (int64_t)gridDim.x * gpb) {
+  for (int64_t j = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; j < n; j += (int64_t)gridDim.x * gpb) {
     const uint32_t s = sub[j];
     V[j * kp + lane] -= sums[(int64_t)s * kp + lane] / (T)(ptr[s + 1] - ptr[s]);
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_scatter_rows(int64_t n, const uint32_t* __restrict__ idx,
+__global__ void __launch_bounds__(kBlock) k_scatter_rows(int64_t b0, int64_t n, const uint32_t* __restrict__ idx,
                                                          const T* __restrict__ src, T* __restrict__ dst, int kp) {
   const int lane = threadIdx.x & (kp - 1);
   const int gpb = kBlock / kp;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * gpb)
+  for (int64_t i = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * gpb)
     dst[(int64_t)idx[i] * kp + lane] = src[i * kp + lane];
 }
 
@@ -672,7 +672,7 @@ template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m) {
   // 1. restrict (signed, partition-weighted) and per-subdomain sums -> mu1
-  k_restrict<T><<<grid_stride_blocks(p.n_loc, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, R, w.B, p.n_loc, kp);
+  k_restrict<T><<<grid_stride_blocks(p.n_loc, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, R, w.B, 0, p.n_loc, kp);
   run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, p.s_loc, w.part, kp);
   if (p.n_red > 0) {
     // 2. Schur right-hand side, centred (mu2)
@@ -681,8 +681,8 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                          p.red_sub, p.loc_ptr, w.sum1, kp},
                              p.s_keep, w.part, kp);
     run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, p.s_red, w.part, kp);
-    k_sub_mean<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(p.n_red, p.red_sub, p.red_ptr, w.sum2, w.Rr,
-                                                                      kp);
+    k_sub_mean<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(0, p.n_red, p.red_sub, p.red_ptr, w.sum2,
+                                                                      w.Rr, kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
     FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
     for (int64_t t = 0; t < p.n_levels; ++t)
@@ -693,15 +693,15 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
                                            p.core_mat_off, p.core_mat, kp},
                                p.s_core, w.part, kp);
-      k_scatter_rows<T><<<grid_stride_blocks(p.n_core_rows, kp), kBlock, 0, st>>>(p.n_core_rows, p.core_idx, w.C,
-                                                                                  w.Rr, kp);
+      k_scatter_rows<T><<<grid_stride_blocks(p.n_core_rows, kp), kBlock, 0, st>>>(0, p.n_core_rows, p.core_idx,
+                                                                                  w.C, w.Rr, kp);
     }
     BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
     for (int64_t t = p.n_levels - 2; t >= 0; --t)
       run_rows<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.h_item_ptr[t], p.s_bwd.h_item_ptr[t + 1],
                               p.s_bwd.h_fix_ptr[t], p.s_bwd.h_fix_ptr[t + 1], w.part, kp);
     // 4. kept block copied into the local solution
-    k_scatter_rows<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(p.n_red, p.red_loc, w.Rr, w.S, kp);
+    k_scatter_rows<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(0, p.n_red, p.red_loc, w.Rr, w.S, kp);
   }
   // 5. back-substitution for the eliminated block
   if (p.n_elim > 0)
@@ -718,5 +718,151 @@ template void launch_precond<float>(hipStream_t, const DevPrecond<float>&, const
                                     PrecondWork<float>&, int, int64_t);
 template void launch_precond<double>(hipStream_t, const DevPrecond<double>&, const double*, double*,
                                      PrecondWork<double>&, int, int64_t);
+
+}  // namespace wa
+
+namespace wa {
+
+// Y[:, c] += coef[c] * X[:, c]     (coef per column, FP64; sign applied)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axpy_col(T* __restrict__ Y, const T* __restrict__ X,
+                                                     const double* __restrict__ coef, double sign, int64_t m,
+                                                     int kp) {
+  const int lane = threadIdx.x & (kp - 1);
+  const T a = (T)(sign * coef[lane]);
+  const int gpb = kBlock / kp;
+#pragma unroll 4
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb)
+    Y[i * kp + lane] += a * X[i * kp + lane];
+}
+template <typename T>
+void launch_axpy_col(hipStream_t st, T* Y, const T* X, const double* coef, double sign, int64_t m, int kp) {
+  k_axpy_col<T><<<std::max<unsigned>(1, std::min<unsigned>((unsigned)((m * kp + kBlock - 1) / kBlock), kRedBlocks)),
+                  kBlock, 0, st>>>(Y, X, coef, sign, m, kp);
+}
+template void launch_axpy_col<float>(hipStream_t, float*, const float*, const double*, double, int64_t, int);
+template void launch_axpy_col<double>(hipStream_t, double*, const double*, const double*, double, int64_t, int);
+
+// Y[:, c] = coef[c] * X[:, c]
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scale_col(T* __restrict__ Y, const T* __restrict__ X,
+                                                      const double* __restrict__ coef, int64_t m, int kp) {
+  const int lane = threadIdx.x & (kp - 1);
+  const T a = (T)coef[lane];
+  const int gpb = kBlock / kp;
+#pragma unroll 4
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb)
+    Y[i * kp + lane] = a * X[i * kp + lane];
+}
+template <typename T>
+void launch_scale_col(hipStream_t st, T* Y, const T* X, const double* coef, int64_t m, int kp) {
+  k_scale_col<T><<<std::max<unsigned>(1, std::min<unsigned>((unsigned)((m * kp + kBlock - 1) / kBlock), kRedBlocks)),
+                   kBlock, 0, st>>>(Y, X, coef, m, kp);
+}
+template void launch_scale_col<float>(hipStream_t, float*, const float*, const double*, int64_t, int);
+template void launch_scale_col<double>(hipStream_t, double*, const double*, const double*, int64_t, int);
+
+// R = B - R  (true residual from R = A x)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rsub(T* __restrict__ R, const T* __restrict__ B, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    R[i] = B[i] - R[i];
+}
+template <typename T>
+void launch_rsub(hipStream_t st, T* R, const T* B, int64_t n) {
+  k_rsub<T><<<std::max<unsigned>(1, std::min<unsigned>((unsigned)((n + kBlock - 1) / kBlock), 65536)), kBlock, 0,
+              st>>>(R, B, n);
+}
+template void launch_rsub<float>(hipStream_t, float*, const float*, int64_t);
+template void launch_rsub<double>(hipStream_t, double*, const double*, int64_t);
+
+}  // namespace wa
+
+namespace wa {
+
+// prolongation of one pair (its subdomains cover distinct DOFs):
+//   d = scale_l (S[l] - mu3); Z[g_l] += d; delta[g_l] = d
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prolong_pair(int64_t b0, int64_t n, const uint32_t* __restrict__ gidx,
+                                                         const T* __restrict__ scale,
+                                                         const uint32_t* __restrict__ loc_sub,
+                                                         const uint32_t* __restrict__ loc_ptr,
+                                                         const T* __restrict__ S, const T* __restrict__ sum3,
+                                                         T* __restrict__ Z, T* __restrict__ delta, int kp) {
+  const int lane = threadIdx.x & (kp - 1);
+  const int gpb = kBlock / kp;
+  for (int64_t l = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; l < n; l += (int64_t)gridDim.x * gpb) {
+    const uint32_t s = loc_sub[l];
+    const T mu = sum3[(int64_t)s * kp + lane] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
+    const T d = scale[l] * (S[l * kp + lane] - mu);
+    const int64_t o = (int64_t)gidx[l] * kp + lane;
+    Z[o] += d;
+    delta[o] = d;
+  }
+}
+
+// Multiplicative Schwarz, forward sweep (schwarz/multiplicative.rs:128-245):
+// per factor pair (its components are mutually independent, so processing
+// them together equals the sequential sweep), restrict the working residual,
+// solve locally, add the correction to z and update r_work -= G delta with
+// the explicit Gramian (operator/residual_update.rs:36-53).
+template <typename T>
+void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vector<PairSched>& pairs,
+                         const DevCsr<T>& g, const T* R, T* Z, PrecondWork<T>& w, T* rwork, T* delta, T* gd,
+                         const double* ones, int kp, int64_t m) {
+  const size_t vbytes = sizeof(T) * (size_t)m * kp;
+  (void)hipMemcpyAsync(rwork, R, vbytes, hipMemcpyDeviceToDevice, st);
+  (void)hipMemsetAsync(Z, 0, vbytes, st);
+  for (const PairSched& ps : pairs) {
+    const int64_t nl = ps.loc1 - ps.loc0;
+    k_restrict<T><<<grid_stride_blocks(nl, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, rwork, w.B, ps.loc0,
+                                                                 ps.loc1, kp);
+    run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, ps.s_loc, w.part, kp);
+    const int64_t nr = ps.red1 - ps.red0;
+    if (nr > 0) {
+      run_rows<KeepKind<T>, T>(st,
+                               KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
+                                           p.red_sub, p.loc_ptr, w.sum1, kp},
+                               ps.s_keep, w.part, kp);
+      run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, w.part, kp);
+      k_sub_mean<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2,
+                                                                   w.Rr, kp);
+      FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
+      for (int64_t t = 0; t < p.n_levels; ++t)
+        run_rows<FwdKind<T>, T>(st, fk, ps.s_fwd, ps.s_fwd.h_item_ptr[t], ps.s_fwd.h_item_ptr[t + 1],
+                                ps.s_fwd.h_fix_ptr[t], ps.s_fwd.h_fix_ptr[t + 1], w.part, kp);
+      if (ps.core1 > ps.core0) {
+        run_rows<CoreKind<T>, T>(st,
+                                 CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
+                                             p.core_mat_off, p.core_mat, kp},
+                                 ps.s_core, w.part, kp);
+        k_scatter_rows<T><<<grid_stride_blocks(ps.core1 - ps.core0, kp), kBlock, 0, st>>>(
+            ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
+      }
+      BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
+      for (int64_t t = p.n_levels - 2; t >= 0; --t)
+        run_rows<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.s_bwd.h_item_ptr[t], ps.s_bwd.h_item_ptr[t + 1],
+                                ps.s_bwd.h_fix_ptr[t], ps.s_bwd.h_fix_ptr[t + 1], w.part, kp);
+      k_scatter_rows<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
+    }
+    run_rows<ElimKind<T>, T>(st,
+                             ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
+                                         p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
+                             ps.s_elim, w.part, kp);
+    run_rows<SegKind<T>, T>(st, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, ps.s_loc, w.part, kp);
+    (void)hipMemsetAsync(delta, 0, vbytes, st);
+    k_prolong_pair<T><<<grid_stride_blocks(nl, kp), kBlock, 0, st>>>(ps.loc0, ps.loc1, p.loc_gidx, p.loc_scale,
+                                                                     p.loc_sub, p.loc_ptr, w.S, w.sum3, Z, delta,
+                                                                     kp);
+    launch_spmm<T>(st, g, delta, gd, w.part, kp);
+    launch_axpy_col<T>(st, rwork, gd, ones, -1.0, m, kp);
+  }
+}
+template void launch_precond_mult<float>(hipStream_t, const DevPrecond<float>&, const std::vector<PairSched>&,
+                                         const DevCsr<float>&, const float*, float*, PrecondWork<float>&, float*,
+                                         float*, float*, const double*, int, int64_t);
+template void launch_precond_mult<double>(hipStream_t, const DevPrecond<double>&, const std::vector<PairSched>&,
+                                          const DevCsr<double>&, const double*, double*, PrecondWork<double>&,
+                                          double*, double*, double*, const double*, int, int64_t);
 
 }  // namespace wa

@@ -65,6 +65,12 @@ struct DevPrecond {
   DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
 };
 
+// subranges of one factor pair, for the multiplicative sweep
+struct PairSched {
+  int64_t loc0 = 0, loc1 = 0, red0 = 0, red1 = 0, core0 = 0, core1 = 0;
+  DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
+};
+
 // scratch for one preconditioner apply (kp columns)
 template <typename T>
 struct PrecondWork {
@@ -109,5 +115,15 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                     int64_t m);
 template <typename T>
 void launch_spmm(hipStream_t st, const DevCsr<T>& g, const T* X, T* Y, T* part, int kp);
+template <typename T>
+void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vector<PairSched>& pairs,
+                         const DevCsr<T>& g, const T* R, T* Z, PrecondWork<T>& w, T* rwork, T* delta, T* gd,
+                         const double* ones, int kp, int64_t m);
+template <typename T>
+void launch_axpy_col(hipStream_t st, T* Y, const T* X, const double* coef, double sign, int64_t m, int kp);
+template <typename T>
+void launch_scale_col(hipStream_t st, T* Y, const T* X, const double* coef, int64_t m, int kp);
+template <typename T>
+void launch_rsub(hipStream_t st, T* R, const T* B, int64_t n);
 
 }  // namespace wa

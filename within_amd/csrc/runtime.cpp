@@ -123,6 +123,20 @@ struct wa_solver {
   ColState* cs = nullptr;
   double* hbuf = nullptr;    // pinned host scratch
   Timer timer;
+  // GMRES bases, allocated on first use
+  template <typename T>
+  struct GmresBufs {
+    int restart = 0;
+    T *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
+    double* hd = nullptr;
+  };
+  std::vector<PairSched> pairs;            // multiplicative sweep subranges
+  double* mw64 = nullptr;                  // r_work, delta, G delta (multiplicative)
+  float* mw32 = nullptr;
+  double* ones = nullptr;
+  GmresBufs<double> gm64;
+  GmresBufs<float> gm32;
+  template <typename T> GmresBufs<T>& gm(int restart);
   // device user buffers for host-pointer calls
   double* ubuf = nullptr;
   int64_t ubuf_len = 0;
@@ -198,6 +212,26 @@ struct wa_solver {
   std::vector<double> col_norms(const double* V, int64_t rows, int kp);
   template <typename T> T* vec(int which);
 };
+
+template <typename T>
+static void alloc_gmres(wa_solver* s, wa_solver::GmresBufs<T>& b, int restart) {
+  if (b.restart >= restart) return;
+  // bases sized for the largest block (KMAX columns); freed with the solver
+  b.V = s->dalloc<T>((int64_t)(restart + 1) * s->m * KMAX);
+  b.Z = s->dalloc<T>((int64_t)restart * s->m * KMAX);
+  b.w = s->dalloc<T>(s->m * KMAX);
+  b.r = s->dalloc<T>(s->m * KMAX);
+  b.hd = s->dalloc<double>((int64_t)(restart + 2) * KMAX);
+  b.restart = restart;
+}
+template <> wa_solver::GmresBufs<double>& wa_solver::gm<double>(int restart) {
+  alloc_gmres(this, gm64, restart);
+  return gm64;
+}
+template <> wa_solver::GmresBufs<float>& wa_solver::gm<float>(int restart) {
+  alloc_gmres(this, gm32, restart);
+  return gm32;
+}
 
 template <> double* wa_solver::vec<double>(int w) {
   switch (w) { case 0: return cx64; case 1: return cr64; case 2: return cz64; case 3: return cp64; default: return cap64; }
@@ -360,6 +394,65 @@ void wa_solver::setup_device() {
     ints(pre32);
     scheds(pre64);
     scheds(pre32);
+    if (plan.precond == 2) {
+      // per-pair subranges (subdomains are ordered by factor pair)
+      std::vector<int64_t> elim_ptr_sub(1, 0), core_row_sub(1, 0);
+      for (const Subdomain& sd : plan.subs) {
+        elim_ptr_sub.push_back(elim_ptr_sub.back() + (sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r()));
+        core_row_sub.push_back(core_row_sub.back() + (int64_t)sd.f.core.size());
+      }
+      size_t s0 = 0;
+      while (s0 < plan.subs.size()) {
+        size_t s1 = s0;
+        while (s1 < plan.subs.size() && plan.subs[s1].q == plan.subs[s0].q && plan.subs[s1].r == plan.subs[s0].r) ++s1;
+        PairSched ps;
+        ps.loc0 = f.loc_ptr[s0]; ps.loc1 = f.loc_ptr[s1];
+        ps.red0 = f.red_ptr[s0]; ps.red1 = f.red_ptr[s1];
+        ps.core0 = core_row_sub[s0]; ps.core1 = core_row_sub[s1];
+        Schedule a;
+        append_schedule(a, f.loc_ptr, (u32)s0, (u32)s1, kChunk, kMaxRows);
+        ps.s_loc = upload_sched(a);
+        Schedule b;
+        append_schedule(b, f.red_ptr, (u32)s0, (u32)s1, kChunk, kMaxRows);
+        ps.s_red = upload_sched(b);
+        Schedule c;
+        append_schedule(c, f.keep_ptr, (u32)ps.red0, (u32)ps.red1, kChunk, kMaxRows);
+        ps.s_keep = upload_sched(c);
+        Schedule e;
+        append_schedule(e, f.elim_ptr, (u32)elim_ptr_sub[s0], (u32)elim_ptr_sub[s1], kChunk, kMaxRows);
+        ps.s_elim = upload_sched(e);
+        Schedule cr;
+        append_schedule(cr, core_row_ptr, (u32)ps.core0, (u32)ps.core1, kChunk, kMaxRows);
+        ps.s_core = upload_sched(cr);
+        auto pair_levels = [&](const std::vector<uint32_t>& ptr) {
+          Schedule sc;
+          std::vector<int64_t> ip(1, 0), fp(1, 0);
+          for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) {
+            u32 a0 = f.lvl_ptr[t], a1 = f.lvl_ptr[t + 1];
+            while (a0 < a1 && f.red_sub[f.lvl_rows[a0]] < s0) ++a0;
+            u32 b1 = a0;
+            while (b1 < a1 && f.red_sub[f.lvl_rows[b1]] < s1) ++b1;
+            append_schedule(sc, ptr, a0, b1, kChunk, kMaxRows);
+            ip.push_back((int64_t)sc.items.size());
+            fp.push_back((int64_t)sc.fixups.size());
+          }
+          DevSched d = upload_sched(sc);
+          d.h_item_ptr = ip;
+          d.h_fix_ptr = fp;
+          return d;
+        };
+        ps.s_fwd = pair_levels(f.fin_ptr);
+        ps.s_bwd = pair_levels(f.bout_ptr);
+        for (const DevSched* d : {&ps.s_loc, &ps.s_red, &ps.s_keep, &ps.s_elim, &ps.s_fwd, &ps.s_bwd, &ps.s_core})
+          max_slots = std::max(max_slots, d->n_slots);
+        pairs.push_back(std::move(ps));
+        s0 = s1;
+      }
+      mw64 = dalloc<double>(3 * m * KMAX);
+      mw32 = dalloc<float>(3 * m * KMAX);
+      std::vector<double> one(KMAX, 1.0);
+      ones = upload(one);
+    }
     int64_t nsub = std::max<int64_t>(f.n_sub, 1);
     w64.B = dalloc<double>(f.n_loc * KMAX);
     w64.S = dalloc<double>(f.n_loc * KMAX);
@@ -424,8 +517,18 @@ template <typename T>
 void wa_solver::pre_apply(const T* R, T* Z, int kp) {
   hipEvent_t a = nullptr;
   tic(1, a);
-  if constexpr (std::is_same<T, double>::value) launch_precond<double>(st, pre64, R, Z, w64, kp, m);
-  else launch_precond<float>(st, pre32, R, Z, w32, kp, m);
+  if (params.preconditioner == 2) {
+    if constexpr (std::is_same<T, double>::value)
+      launch_precond_mult<double>(st, pre64, pairs, g64, R, Z, w64, mw64, mw64 + m * KMAX, mw64 + 2 * m * KMAX,
+                                  ones, kp, m);
+    else
+      launch_precond_mult<float>(st, pre32, pairs, g32, R, Z, w32, mw32, mw32 + m * KMAX, mw32 + 2 * m * KMAX,
+                                 ones, kp, m);
+  } else if constexpr (std::is_same<T, double>::value) {
+    launch_precond<double>(st, pre64, R, Z, w64, kp, m);
+  } else {
+    launch_precond<float>(st, pre32, R, Z, w32, kp, m);
+  }
   toc(1, a);
 }
 
@@ -515,13 +618,169 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
 }
 
 // Right-preconditioned GMRES(restart) with modified Gram-Schmidt and Givens
-// rotations, one Arnoldi process per column (solve/gmres.rs:201-530).  The
-// Hessenberg / Givens recurrences stay in FP64 on the host side of the loop.
+// rotations, one Arnoldi process per column (solve/gmres.rs:201-530).  All
+// active columns advance in lockstep (they share the iteration count); the
+// Hessenberg / Givens recurrences and the triangular solve stay in FP64 on
+// the host, the basis and every vector operation on the device.
 template <typename T>
-void wa_solver::gmres(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+void wa_solver::gmres(int k, int kp, const T* b, T* x, const std::vector<double>& tol, std::vector<int>& conv,
                       std::vector<int64_t>& its) {
-  (void)k; (void)kp; (void)rhs; (void)x; (void)tol; (void)conv; (void)its;
-  throw Error(WA_ERR_UNSUPPORTED, "GMRES on the device is not available in this build");
+  const int mr = params.restart;
+  const int64_t vb = m * kp;  // elements per block vector
+  GmresBufs<T>& G = gm<T>(mr);
+  T *w = G.w, *r = G.r;
+  auto V = [&](int i) { return G.V + (int64_t)i * m * KMAX; };
+  auto Z = [&](int i) { return G.Z + (int64_t)i * m * KMAX; };
+  double* hd = G.hd;  // (mr + 2) x KMAX dots of the current column
+  HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * vb, st));
+  auto norms = [&](const T* v) {
+    launch_dot<T>(st, v, v, m, kp, redp, s_(10));
+    HIPCHK(hipMemcpyAsync(hbuf, s_(10), sizeof(double) * kp, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> o(kp);
+    for (int c = 0; c < kp; ++c) o[c] = std::sqrt(hbuf[c]);
+    return o;
+  };
+  auto upload_coef = [&](const std::vector<double>& v, double* dst) {
+    HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+  };
+  std::vector<double> bn = norms(b), abs_tol(kp, 0.0);
+  std::vector<char> live(kp, 0);
+  for (int c = 0; c < kp; ++c) {
+    if (c >= k) continue;
+    abs_tol[c] = tol[c] * bn[c];
+    if (bn[c] < F64_EPS) { conv[c] = 1; its[c] = 0; } else live[c] = 1;
+  }
+  auto any_live = [&]() { for (int c = 0; c < kp; ++c) if (live[c]) return true; return false; };
+  int64_t total = 0;
+  std::vector<double> H((size_t)(mr + 1) * mr * KMAX), cs(mr * KMAX), sn(mr * KMAX), g((mr + 1) * KMAX);
+  auto h = [&](int c, int i, int j) -> double& { return H[((size_t)c * mr + j) * (mr + 1) + i]; };
+  while (any_live()) {
+    // true residual r = b - A x (restart), gmres.rs:324-335
+    op_apply<T>(x, r, kp);
+    launch_rsub<T>(st, r, b, vb);
+    std::vector<double> beta = norms(r);
+    for (int c = 0; c < kp; ++c) {
+      if (!live[c]) continue;
+      if (beta[c] <= abs_tol[c]) { live[c] = 0; conv[c] = 1; its[c] = total; }
+      else if (total >= params.maxiter) { live[c] = 0; conv[c] = 0; its[c] = total; }
+    }
+    if (!any_live()) break;
+    std::vector<double> coef(KMAX, 0.0);
+    for (int c = 0; c < kp; ++c) coef[c] = live[c] ? 1.0 / beta[c] : 0.0;
+    upload_coef(coef, s_(11));
+    launch_scale_col<T>(st, V(0), r, s_(11), m, kp);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(cs.begin(), cs.end(), 0.0);
+    std::fill(sn.begin(), sn.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    std::vector<double> hmax(kp, 0.0);
+    for (int c = 0; c < kp; ++c) g[(size_t)c * (mr + 1)] = live[c] ? beta[c] : 0.0;
+    const int64_t cycle = std::min<int64_t>(params.maxiter - total, mr);
+    std::vector<char> inc = live;           // still extending its basis this cycle
+    std::vector<int> jc(kp, 0), outcome(kp, 3);  // 0 converged, 1 maxiter, 2 breakdown, 3 restart
+    std::vector<double> resid(kp, 0.0);
+    int j = 0;
+    auto any_inc = [&]() { for (int c = 0; c < kp; ++c) if (inc[c]) return true; return false; };
+    while (j < cycle && any_inc()) {
+      if (has_pre && params.preconditioner != 0) pre_apply<T>(V(j), Z(j), kp);
+      else HIPCHK(hipMemcpyAsync(Z(j), V(j), sizeof(T) * vb, hipMemcpyDeviceToDevice, st));
+      op_apply<T>(Z(j), w, kp);
+      hipEvent_t ev = nullptr;
+      tic(2, ev);
+      for (int i = 0; i <= j; ++i) {  // modified Gram-Schmidt, gmres.rs:201-220
+        launch_dot<T>(st, w, V(i), m, kp, redp, hd + (int64_t)i * KMAX);
+        launch_axpy_col<T>(st, w, V(i), hd + (int64_t)i * KMAX, -1.0, m, kp);
+      }
+      launch_dot<T>(st, w, w, m, kp, redp, hd + (int64_t)(j + 1) * KMAX);
+      toc(2, ev);
+      std::vector<double> hh((size_t)(j + 2) * KMAX);
+      HIPCHK(hipMemcpyAsync(hh.data(), hd, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      ++total;
+      std::fill(coef.begin(), coef.end(), 0.0);
+      for (int c = 0; c < kp; ++c) {
+        if (!inc[c]) continue;
+        double lmax = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          h(c, i, j) = hh[(size_t)i * KMAX + c];
+          lmax = std::max(lmax, std::fabs(h(c, i, j)));
+        }
+        const double hj1 = std::sqrt(hh[(size_t)(j + 1) * KMAX + c]);
+        h(c, j + 1, j) = hj1;
+        hmax[c] = std::max(std::max(hmax[c], lmax), hj1);
+        double* csc = &cs[(size_t)c * mr];
+        double* snc = &sn[(size_t)c * mr];
+        double* gc = &g[(size_t)c * (mr + 1)];
+        for (int i = 0; i < j; ++i) {  // previous rotations, gmres.rs:223-231
+          const double a = h(c, i, j), bb = h(c, i + 1, j);
+          const double t = csc[i] * a + snc[i] * bb;
+          h(c, i + 1, j) = -snc[i] * a + csc[i] * bb;
+          h(c, i, j) = t;
+        }
+        const double a = h(c, j, j), bb = h(c, j + 1, j), rr = std::hypot(a, bb);
+        const double cc = rr == 0.0 ? 1.0 : a / rr, ss = rr == 0.0 ? 0.0 : bb / rr;
+        csc[j] = cc;
+        snc[j] = ss;
+        h(c, j, j) = cc * a + ss * bb;
+        h(c, j + 1, j) = 0.0;
+        const double t = cc * gc[j] + ss * gc[j + 1];
+        gc[j + 1] = -ss * gc[j] + cc * gc[j + 1];
+        gc[j] = t;
+        jc[c] = j + 1;
+        resid[c] = std::fabs(gc[j + 1]);
+        if (resid[c] <= abs_tol[c]) { outcome[c] = 0; inc[c] = 0; }
+        else if (total >= params.maxiter) { outcome[c] = 1; inc[c] = 0; }
+        else if (hj1 > 1e-14 * hmax[c]) coef[c] = 1.0 / hj1;
+        else { outcome[c] = 2; inc[c] = 0; }
+      }
+      ++j;
+      if (any_inc() && j < cycle) {
+        upload_coef(coef, s_(11));
+        launch_scale_col<T>(st, V(j), w, s_(11), m, kp);
+      }
+    }
+    // y = R^-1 g per column (gmres.rs:499-519), x += Z y (:523-530)
+    int jmax = 0;
+    std::vector<std::vector<double>> y(kp);
+    for (int c = 0; c < kp; ++c) {
+      if (!live[c]) continue;
+      jmax = std::max(jmax, jc[c]);
+      y[c].assign(jc[c], 0.0);
+      const double* gc = &g[(size_t)c * (mr + 1)];
+      for (int i = jc[c] - 1; i >= 0; --i) {
+        double sum = gc[i];
+        for (int q = i + 1; q < jc[c]; ++q) sum -= h(c, i, q) * y[c][q];
+        const double dg = h(c, i, i);
+        y[c][i] = std::fabs(dg) < 1e-14 * (1.0 + std::fabs(sum)) ? 0.0 : sum / dg;
+      }
+    }
+    for (int i = 0; i < jmax; ++i) {
+      std::fill(coef.begin(), coef.end(), 0.0);
+      for (int c = 0; c < kp; ++c)
+        if (live[c] && i < jc[c]) coef[c] = y[c][i];
+      upload_coef(coef, s_(11));
+      launch_axpy_col<T>(st, x, Z(i), s_(11), 1.0, m, kp);
+    }
+    bool need_true = false;
+    for (int c = 0; c < kp; ++c) {
+      if (!live[c]) continue;
+      if (outcome[c] == 0) { live[c] = 0; conv[c] = 1; its[c] = total - (j - jc[c]); }
+      else if (outcome[c] == 1) { live[c] = 0; conv[c] = 0; its[c] = total - (j - jc[c]); }
+      else if (outcome[c] == 2) need_true = true;
+    }
+    if (need_true) {  // lucky breakdown: decide on the true residual
+      op_apply<T>(x, r, kp);
+      launch_rsub<T>(st, r, b, vb);
+      std::vector<double> tr = norms(r);
+      for (int c = 0; c < kp; ++c)
+        if (live[c] && outcome[c] == 2) {
+          live[c] = 0;
+          conv[c] = tr[c] <= abs_tol[c];
+          its[c] = total - (j - jc[c]);
+        }
+    }
+  }
 }
 
 // One block of up to 32 right-hand sides (Solver::solve per column,
