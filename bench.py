@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=0.05)
     ap.add_argument("--cpu-k", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-9,
+                    help="Krylov tolerance for both the measured path and the plain baseline")
+    ap.add_argument("--operator", choices=["implicit", "explicit"], default="implicit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1"}))
@@ -141,7 +144,8 @@ def main():
 
     # --- headline: FP32 block CG + FP64 refinement, k RHS per block -------------
     log("bench: setup mixed-precision block solver ...")
-    fast = Run(p, W.CG(precision="mixed"))
+    opr = W.OperatorRepr.Explicit if args.operator == "explicit" else W.OperatorRepr.Implicit
+    fast = Run(p, W.CG(precision="mixed", tol=args.tol, operator=opr))
     m = fast.m
     log(f"bench: setup {fast.setup_s:.1f}s  n_dofs={m}  info={fast.s.info().tolist()}")
     X = torch.zeros((k, m), device="cuda", dtype=torch.float64)
@@ -173,7 +177,7 @@ def main():
     # --- FP64 reference solution (tight) for the 1e-8 check ---------------------
     log("bench: FP64 reference solve (tol 1e-11) ...")
     ref = fast
-    ref.s.set_config(W.CG(tol=1e-11, max_refinements=4))
+    ref.s.set_config(W.CG(tol=1e-11, max_refinements=4, operator=opr))
     Dref = torch.zeros((k, n), device="cuda", dtype=torch.float64)
     Xr = torch.zeros((k, m), device="cuda", dtype=torch.float64)
     _, st_ref = ref.solve(Y, k, torch, Xr, Dref)
@@ -185,7 +189,7 @@ def main():
     # --- plain FP64 Schwarz-PCG, one RHS per solve -------------------------------
     log("bench: plain FP64 k=1 baseline ...")
     base = fast
-    base.s.set_config(W.CG(precision="fp64", block=1))
+    base.s.set_config(W.CG(precision="fp64", block=1, tol=args.tol, operator=opr))
     nb = min(args.baseline_rhs, k)
     Xb = torch.zeros((nb, m), device="cuda", dtype=torch.float64)
     Db = torch.zeros((nb, n), device="cuda", dtype=torch.float64)
@@ -197,7 +201,7 @@ def main():
     log(f"bench: baseline {nb} RHS in {tb:.2f}s ({base_rate:.2f} RHS/s), iters={[s[1] for s in stb]}")
 
     value = sum(solved) / t_step
-    n_op, ms_op = cls["operator"]
+    n_op, ms_op = cls["explicit" if args.operator == "explicit" else "operator"]
     Q = 3
     b_op = 4 * Q * n + 2 * 4 * k * m  # codes once + X read + GX written (FP32, k columns)
     op_ms = ms_op / max(n_op, 1)

@@ -165,4 +165,84 @@ void run_rows(hipStream_t st, const K& k, const DevSched& sc, T* part, int kp) {
   run_rows<K, T>(st, k, sc, 0, sc.n_items, 0, sc.n_fix, part, kp);
 }
 
+// ---- many small levels in one workgroup -----------------------------------
+// Levels t0..t1-1 (ascending when dir > 0, else descending) of a level
+// schedule, processed by the 16 waves of one workgroup with a barrier between
+// levels (rows of a level depend only on earlier levels).  The device-scope
+// fences make each level's global writes visible to every wave before the
+// next level reads them.
+constexpr int kTailThreads = 1024;
+
+template <class K, typename T, int V, int NV>
+__global__ void __launch_bounds__(kTailThreads) k_levels_wg(K k, const Item* __restrict__ items,
+                                                            const int64_t* __restrict__ item_ptr,
+                                                            const Fixup* __restrict__ fx,
+                                                            const int64_t* __restrict__ fix_ptr,
+                                                            T* __restrict__ part, int t0, int t1, int dir) {
+  constexpr int KP = V * NV;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  constexpr int nw = kTailThreads / kWave;
+  const int slot = lane / V, pt = lane % V;
+  T acc[NV];
+  for (int step = 0; step < t1 - t0; ++step) {
+    const int t = dir > 0 ? t0 + step : t1 - 1 - step;
+    for (int64_t g = item_ptr[t] + wave; g < item_ptr[t + 1]; g += nw) {
+      const Item it = items[g];
+      if (it.slot < 0) {
+        for (uint32_t r = it.row0; r < it.row1; ++r) {
+          gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
+          if (slot == 0)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) k.finish(r, pt * NV + c, acc[c]);
+        }
+      } else {
+        gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
+        if (slot == 0)
+#pragma unroll
+          for (int c = 0; c < NV; ++c) part[(int64_t)it.slot * KP + pt * NV + c] = acc[c];
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (fix_ptr[t + 1] > fix_ptr[t]) {
+      // one wave per split row: lanes stride over its slots, then an xor tree
+      const int col = lane % KP;
+      for (int64_t f = fix_ptr[t] + wave; f < fix_ptr[t + 1]; f += nw) {
+        const Fixup fxr = fx[f];
+        T a = 0;
+        for (uint32_t s = fxr.slot0 + lane / KP; s < fxr.slot1; s += kWave / KP) a += part[(int64_t)s * KP + col];
+#pragma unroll
+        for (int off = KP; off < kWave; off <<= 1) a += __shfl_xor(a, off, kWave);
+        if (lane < KP) k.finish(fxr.row, col, a);
+      }
+      __threadfence();
+      __syncthreads();
+    }
+  }
+}
+
+template <class K, typename T>
+void run_levels_wg(hipStream_t st, const K& k, const DevSched& sc, const int64_t* item_ptr, const int64_t* fix_ptr,
+                   int t0, int t1, int dir, T* part, int kp) {
+  if (t1 <= t0) return;
+  constexpr int NV16 = 16 / sizeof(T);
+#define WA_LEVELS_WG(V, NV) \
+  k_levels_wg<K, T, V, NV><<<1, kTailThreads, 0, st>>>(k, sc.items, item_ptr, sc.fix, fix_ptr, part, t0, t1, dir)
+  if (kp * (int)sizeof(T) >= 16) {
+    switch (kp / NV16) {
+      case 1: WA_LEVELS_WG(1, NV16); break;
+      case 2: WA_LEVELS_WG(2, NV16); break;
+      case 4: WA_LEVELS_WG(4, NV16); break;
+      case 8: WA_LEVELS_WG(8, NV16); break;
+      default: WA_LEVELS_WG(16, NV16); break;
+    }
+  } else if (kp == 2) {
+    WA_LEVELS_WG(1, 2);
+  } else {
+    WA_LEVELS_WG(1, 1);
+  }
+#undef WA_LEVELS_WG
+}
+
 }  // namespace wa

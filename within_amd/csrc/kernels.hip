@@ -685,9 +685,15 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                                                       w.Rr, kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
     FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-    for (int64_t t = 0; t < p.n_levels; ++t)
+    const int64_t last = p.n_levels - 1;  // the core gather level
+    for (int64_t t = 0; t < p.tail0; ++t)
       run_rows<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.h_item_ptr[t], p.s_fwd.h_item_ptr[t + 1],
                               p.s_fwd.h_fix_ptr[t], p.s_fwd.h_fix_ptr[t + 1], w.part, kp);
+    run_levels_wg<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.d_item_ptr, p.s_fwd.d_fix_ptr, (int)p.tail0, (int)last, 1,
+                                 w.part, kp);
+    if (last >= 0)
+      run_rows<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.h_item_ptr[last], p.s_fwd.h_item_ptr[last + 1],
+                              p.s_fwd.h_fix_ptr[last], p.s_fwd.h_fix_ptr[last + 1], w.part, kp);
     if (p.n_core_rows > 0) {
       run_rows<CoreKind<T>, T>(st,
                                CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
@@ -697,7 +703,9 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                                                                   w.C, w.Rr, kp);
     }
     BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
-    for (int64_t t = p.n_levels - 2; t >= 0; --t)
+    run_levels_wg<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.d_item_ptr, p.s_bwd.d_fix_ptr, (int)p.tail0, (int)last, -1,
+                                 w.part, kp);
+    for (int64_t t = p.tail0 - 1; t >= 0; --t)
       run_rows<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.h_item_ptr[t], p.s_bwd.h_item_ptr[t + 1],
                               p.s_bwd.h_fix_ptr[t], p.s_bwd.h_fix_ptr[t + 1], w.part, kp);
     // 4. kept block copied into the local solution

@@ -14,6 +14,7 @@ constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 constexpr int kRedBlocks = 512;   // fixed reduction grid (deterministic sums)
 constexpr uint32_t kChunk = 256;  // elements per work item (long rows are split)
 constexpr uint32_t kMaxRows = 32; // rows packed into one work item
+constexpr int64_t kTailRows = 1024; // levels this small run inside one workgroup
 
 struct DevSched {
   const Item* items = nullptr;
@@ -22,6 +23,8 @@ struct DevSched {
   int64_t n_fix = 0;
   int64_t n_slots = 0;
   std::vector<int64_t> h_item_ptr, h_fix_ptr;  // per-level ranges (level schedules only)
+  const int64_t* d_item_ptr = nullptr;         // device copies of the per-level ranges
+  const int64_t* d_fix_ptr = nullptr;
 };
 
 // one level-sorted factor pass (plan.hpp FactorPass) in precision T
@@ -50,6 +53,7 @@ struct DevCsr {
 template <typename T>
 struct DevPrecond {
   int64_t n_loc = 0, n_red = 0, n_sub = 0, n_elim = 0, n_levels = 0, n_core_rows = 0;
+  int64_t tail0 = 0;  // elimination levels [tail0, n_levels - 1) run in one workgroup
   const uint32_t *loc_gidx, *loc_ptr, *red_ptr, *loc_sub, *red_sub, *red_loc;
   const T* loc_scale;
   const uint32_t *elim_loc, *elim_ptr, *elim_idx;

@@ -374,8 +374,14 @@ void wa_solver::setup_device() {
       DevSched d = upload_sched(sc);
       d.h_item_ptr = ip;
       d.h_fix_ptr = fp;
+      d.d_item_ptr = upload(ip);
+      d.d_fix_ptr = upload(fp);
       return d;
     };
+    // trailing elimination levels small enough for one workgroup
+    int64_t n_lv = (int64_t)f.lvl_ptr.size() - 1, tail0 = std::max<int64_t>(n_lv - 1, 0);
+    while (tail0 > 0 && f.lvl_ptr[tail0] - f.lvl_ptr[tail0 - 1] <= kTailRows) --tail0;
+    pre64.tail0 = pre32.tail0 = tail0;
     DevSched s_loc = upload_sched(make_schedule(f.loc_ptr, kChunk, kMaxRows));
     DevSched s_red = upload_sched(make_schedule(f.red_ptr, kChunk, kMaxRows));
     DevSched s_keep = upload_sched(make_schedule(f.keep_ptr, kChunk, kMaxRows));
