@@ -238,3 +238,52 @@ def test_gmres_iterations_track_oracle(oracle_mod):
     r = W.solve(codes, y, config=W.GMRES(), weights=w, preconditioner=W.MultiplicativeSchwarz())
     assert r.converged and ro["converged"]
     assert abs(r.iterations - ro["iterations"]) <= 1
+
+
+# A factor pair whose reduced system ends in a dense core of at least 512
+# vertices (the tiled and matrix-vector dense-core kernels): 971 vertices with
+# the default core size, 538 after one round of eliminations at dense_core=600.
+BIG = synth.akm_panel(60000, 6000, 1000, 10, seed=8)["codes"]
+RUST_DEFAULT = W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+
+
+@pytest.mark.parametrize("dense_core", [0, 600])
+def test_dense_core_apply_matches_oracle(dense_core, oracle_mod):
+    okw = dict(ac_split=1)
+    if dense_core:
+        okw["dense_core"] = dense_core
+    o = oracle_mod.OracleSolver(BIG, None, **okw)
+    s = W.Solver(BIG, preconditioner=RUST_DEFAULT, _dense_core=dense_core)
+    assert s.info()[13] >= 512
+    m = s.n_dofs
+    rng = np.random.default_rng(123 + dense_core)
+    for k in (1, 2, 3, 4, 8, 16, 32):  # every kernel width
+        R = np.asfortranarray(rng.standard_normal((m, k)))
+        Z = np.zeros((m, k), order="F")
+        N.check(N.load().wa_precond_apply(s._h, N.ptr(R), N.ptr(Z), k, m))
+        ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(k)])
+        assert rel(Z, ref) <= 1e-10, (k, rel(Z, ref))
+
+
+@pytest.mark.parametrize("k", [4, 32])
+def test_dense_core_mixed_solve_matches_oracle(k, oracle_mod):
+    n = BIG.shape[0]
+    Y = np.asfortranarray(np.random.default_rng(9 + k).standard_normal((n, k)))
+    o = oracle_mod.OracleSolver(BIG, None, ac_split=1, tol=1e-10, max_refinements=3)
+    ro = o.solve_batch(Y)
+    s = W.Solver(BIG, config=W.CG(tol=1e-10, max_refinements=3, precision="mixed"), preconditioner=RUST_DEFAULT)
+    r = s.solve_batch(Y)
+    assert all(r.converged) and all(ro["converged"])
+    for j in range(k):
+        assert rel(r.demeaned[:, j], ro["demeaned"][:, j]) <= 1e-8, (j, rel(r.demeaned[:, j], ro["demeaned"][:, j]))
+
+
+def test_dense_core_multiplicative_apply_matches_oracle(oracle_mod):
+    o = oracle_mod.OracleSolver(BIG, None, krylov=1, precond=2, ac_split=2)
+    s = W.Solver(BIG, config=W.GMRES(), preconditioner=W.MultiplicativeSchwarz())
+    m = s.n_dofs
+    R = np.asfortranarray(np.random.default_rng(77).standard_normal((m, 3)))
+    Z = np.zeros((m, 3), order="F")
+    N.check(N.load().wa_precond_apply(s._h, N.ptr(R), N.ptr(Z), 3, m))
+    ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(3)])
+    assert rel(Z, ref) <= 1e-10, rel(Z, ref)

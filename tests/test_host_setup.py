@@ -156,3 +156,33 @@ def test_level_schedule_invariants():
     s = host_solver(codes)
     info = s.info()
     assert info[3] >= 0 and info[5] > 0
+
+
+@pytest.mark.parametrize("dense_core", [0, 600])
+def test_large_dense_core_is_bitwise_the_oracles(dense_core, oracle_mod):
+    # the blocked, vectorised host factorisation of a ~1000-vertex core keeps
+    # every entry's operation sequence: bitwise equal to the oracle's loops
+    codes = synth.akm_panel(60000, 6000, 1000, 10, seed=8)["codes"]
+    okw = dict(ac_split=1)
+    if dense_core:
+        okw["dense_core"] = dense_core
+    o = oracle_mod.OracleSolver(codes, None, **okw)
+    prec = W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+    s = W.Solver(codes, preconditioner=prec, _host_only=True, _dense_core=dense_core)
+    L = N.load()
+    big = 0
+    for i in range(o.n_subdomains):
+        if o.subdomain(i)["dense"]:
+            continue
+        oa = o.subdomain_ac(i)
+        n = len(oa["pivot"])
+        nnz = C.c_int64()
+        arrs = dict(pivot=np.zeros(n), col_ptr=np.zeros(n + 1, np.uint32), col_idx=np.zeros(len(oa["col_idx"]), np.uint32),
+                    col_w=np.zeros(len(oa["col_w"])), round_ptr=np.zeros(len(oa["round_ptr"]), np.uint32),
+                    order=np.zeros(len(oa["order"]), np.uint32), core=np.zeros(len(oa["core"]), np.uint32),
+                    core_inv=np.zeros(oa["core_inv"].size))
+        N.check(L.wa_solver_subdomain_factor(s._h, i, C.byref(nnz), *[N.ptr(a) for a in arrs.values()]))
+        for k, a in arrs.items():
+            assert np.array_equal(a, np.ravel(oa[k])), (i, k)
+        big = max(big, len(oa["core"]))
+    assert big >= 512

@@ -328,7 +328,8 @@ class Solver:
     """Persistent solver: preconditioner built and uploaded once
     (crates/within/src/solver.rs:82-447)."""
 
-    def __init__(self, categories, config=None, weights=None, preconditioner=None, *, _host_only=False):
+    def __init__(self, categories, config=None, weights=None, preconditioner=None, *, _host_only=False,
+                 _dense_core=0):
         t0 = time.perf_counter()
         if isinstance(preconditioner, FePreconditioner):
             base = preconditioner._solver
@@ -337,6 +338,7 @@ class Solver:
         self._design = _Design(categories, weights)
         p, self._reduction = _params(config, preconditioner)
         p.host_only = 1 if _host_only else 0
+        p.local.dense_core = int(_dense_core)
         self._kind = p.preconditioner
         self._params = p
         h = C.c_void_p()
@@ -353,6 +355,8 @@ class Solver:
         """Switch the Krylov settings (CG/GMRES, tol, maxiter, refinements,
         precision, block) of this built solver; the preconditioner stays."""
         p, _ = _params(config, self._precond_arg)
+        p.host_only = self._params.host_only
+        p.local.dense_core = self._params.local.dense_core
         N.check(N.load().wa_solver_set_params(self._h, C.byref(p)))
         self._params = p
 

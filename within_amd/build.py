@@ -46,7 +46,8 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
     plan_o = os.path.join(BUILD, "plan.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
+    # no FP contraction: host factors must match the oracle bit for bit
+    _run(["g++", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
           os.path.join(CSRC, "plan.cpp"), "-o", plan_o], verbose)
     objs.append(plan_o)
     for src in ("kernels.hip", "runtime.cpp"):

@@ -1,20 +1,21 @@
 // engine.hpp -- the row-gather engine shared by every CSR-like kernel.
 //
 // A "row" is a sum over elements: out(row) = finish(row, sum_e coef(e) *
-// sum_{t<QM} X[idx_t(e)]).  Rows are processed through a host-built schedule
-// of work items: an item covers whole rows (finished in place) or one chunk of
-// a long row (written to a partial slot; a fix-up sums the slots of the row in
-// slot order), so long rows never serialise a wave and every result is
-// bitwise reproducible.
+// sum_{t<QM} X[idx_t(e)]).  Rows are processed over a host-built schedule
+// (plan.hpp Schedule): short rows (at most `short_max` elements) are summed by
+// one lane group each, in element order; a longer row is a work item for a
+// whole wave, and a row longer than one chunk is split into chunks written to
+// partial slots that a fix-up sums in slot order.  Every result is therefore
+// bitwise reproducible for a given schedule and column count.
 //
-// Lane mapping (one wave64 per work item): a block-vector row holds kp values
-// of type T; each lane owns NV consecutive values (a 16-byte vector when the
-// row is at least 16 bytes), V = kp / NV lanes cover one row, and the wave
-// works on S = 64 / V elements at once ("slots").  Every lane loads its
-// element's indices itself (the V lanes of a slot read the same address, so
-// the wave issues S distinct index loads), then its vector slices of the
-// gathered rows: no cross-lane traffic until the row ends, where the S slot
-// partials are combined by a fixed xor-butterfly.
+// Lane mapping (wave64): a block-vector row holds kp values of type T; each
+// lane owns NV consecutive values (a 16-byte vector when the row is at least
+// 16 bytes), and V = kp / NV lanes form a lane group covering one row, so a
+// wave holds S = 64 / V lane groups ("slots").
+//  * short rows: slot s of wave w owns row r0 + w * S + s and walks its
+//    elements with U gathers in flight;
+//  * work items: the S slots stride over the item's elements, and the slot
+//    partials are combined by a fixed xor-butterfly at the end of the row.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,6 +25,7 @@
 namespace wa {
 
 constexpr int kWave = 64;
+constexpr int kFixBlock = 512;
 
 template <typename T, int NV>
 struct alignas(sizeof(T) * NV) VecT {
@@ -35,7 +37,50 @@ __device__ __forceinline__ VecT<T, NV> vload(const T* p) {
   return *reinterpret_cast<const VecT<T, NV>*>(p);
 }
 
-// V lanes per row, NV values per lane (V * NV == kp)
+// one row by one lane group: elements in order, U gathers in flight
+template <class K, typename T, int KP, int NV>
+__device__ __forceinline__ void gather_row(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int part,
+                                           T (&acc)[NV]) {
+  constexpr int QM = K::QM > 0 ? K::QM : 1;
+  constexpr int U = QM == 1 ? 4 : 2;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0;
+  if (K::QM == 0) return;
+  for (uint32_t e = e0; e < e1; e += U) {
+    uint32_t id[U][QM];
+    T cf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cf[u] = 1;
+#pragma unroll
+      for (int t = 0; t < QM; ++t) id[u][t] = 0;
+      if (e + u < e1) k.load(row, e + u, id[u], cf[u]);
+    }
+    VecT<T, NV> g[U][QM];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int t = 0; t < QM; ++t) {
+        if (e + u < e1) {
+          g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
+        } else {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) g[u][t].v[c] = 0;
+        }
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        T s = 0;
+#pragma unroll
+        for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
+        if (e + u < e1) acc[c] += K::COEF ? cf[u] * s : s;
+      }
+  }
+}
+
+// one row (or chunk) by a whole wave: V lanes per row, S slots over elements
 template <class K, typename T, int V, int NV>
 __device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int slot, int part,
                                            T (&acc)[NV]) {
@@ -85,15 +130,31 @@ __device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0
     for (int c = 0; c < NV; ++c) acc[c] += __shfl_xor(acc[c], off, kWave);
 }
 
+// Waves [0, n_sw) sum the short rows of [r0, r1), S rows per wave; the
+// remaining waves run one work item each.
 template <class K, typename T, int V, int NV>
-__global__ void __launch_bounds__(kBlock) k_rows_vec(K k, const Item* __restrict__ items, int64_t n_items,
-                                                     T* __restrict__ part) {
+__global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, uint32_t short_max, int64_t n_sw,
+                                                 const Item* __restrict__ items, int64_t n_items,
+                                                 T* __restrict__ part) {
+  constexpr int S = kWave / V;
+  constexpr int KP = V * NV;
   const int lane = threadIdx.x & (kWave - 1);
   const int slot = lane / V, pt = lane % V;
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  T acc[NV];
+  if (g < n_sw) {
+    const int64_t r = (int64_t)r0 + g * S + slot;
+    if (r >= (int64_t)r1) return;
+    const uint32_t e0 = k.rbegin((uint32_t)r), e1 = k.rend((uint32_t)r);
+    if (e1 - e0 > short_max) return;
+    gather_row<K, T, KP, NV>(k, (uint32_t)r, e0, e1, pt, acc);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) k.finish((uint32_t)r, pt * NV + c, acc[c]);
+    return;
+  }
+  g -= n_sw;
   if (g >= n_items) return;
   const Item it = items[g];
-  T acc[NV];
   if (it.slot < 0) {
     for (uint32_t r = it.row0; r < it.row1; ++r) {
       gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
@@ -105,21 +166,28 @@ __global__ void __launch_bounds__(kBlock) k_rows_vec(K k, const Item* __restrict
     gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
     if (slot == 0)
 #pragma unroll
-      for (int c = 0; c < NV; ++c) part[(int64_t)it.slot * (V * NV) + pt * NV + c] = acc[c];
+      for (int c = 0; c < NV; ++c) part[(int64_t)it.slot * KP + pt * NV + c] = acc[c];
   }
 }
 
-// ---- fix-up: one workgroup per split row; threads stride over the slots,
-// then a fixed-order tree over the workgroup ----------------------------------
+// ---- fix-up: one workgroup per split row; threads stride over the slots
+// with four independent sums, then a fixed-order tree over the workgroup -----
 template <class K, typename T>
-__global__ void __launch_bounds__(kBlock) k_rows_fix(K k, const Fixup* __restrict__ fx, const T* __restrict__ part,
-                                                     int kp) {
-  __shared__ T sh[kBlock];
+__global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __restrict__ fx,
+                                                        const T* __restrict__ part, int kp) {
+  __shared__ T sh[kFixBlock];
   const Fixup f = fx[blockIdx.x];
-  const int col = threadIdx.x % kp, str = kBlock / kp, s0 = threadIdx.x / kp;
-  T acc = 0;
-  for (uint32_t s = f.slot0 + s0; s < f.slot1; s += str) acc += part[(int64_t)s * kp + col];
-  sh[threadIdx.x] = acc;
+  const int col = threadIdx.x % kp, str = kFixBlock / kp, s0 = threadIdx.x / kp;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  uint32_t s = f.slot0 + s0;
+  for (; s + 3 * str < f.slot1; s += 4 * str) {
+    a0 += part[(int64_t)s * kp + col];
+    a1 += part[(int64_t)(s + str) * kp + col];
+    a2 += part[(int64_t)(s + 2 * str) * kp + col];
+    a3 += part[(int64_t)(s + 3 * str) * kp + col];
+  }
+  for (; s < f.slot1; s += str) a0 += part[(int64_t)s * kp + col];
+  sh[threadIdx.x] = (a0 + a1) + (a2 + a3);
   __syncthreads();
   for (int w = str / 2; w >= 1; w >>= 1) {
     if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
@@ -131,118 +199,38 @@ __global__ void __launch_bounds__(kBlock) k_rows_fix(K k, const Fixup* __restric
 inline unsigned engine_blocks(int64_t waves) { return (unsigned)((waves * kWave + kBlock - 1) / kBlock); }
 
 template <class K, typename T, int V, int NV>
-void launch_vec(hipStream_t st, const K& k, const Item* items, int64_t ni, T* part) {
-  k_rows_vec<K, T, V, NV><<<engine_blocks(ni), kBlock, 0, st>>>(k, items, ni, part);
+void launch_rows(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T* part) {
+  constexpr int S = kWave / V;
+  const int64_t n_sw = ((int64_t)g.r1 - g.r0 + S - 1) / S, ni = g.i1 - g.i0;
+  if (n_sw + ni == 0) return;
+  k_rows<K, T, V, NV><<<engine_blocks(n_sw + ni), kBlock, 0, st>>>(k, g.r0, g.r1, sc.short_max, n_sw,
+                                                                   sc.items + g.i0, ni, part);
 }
 
-// Run items [i0, i1) and fix-ups [f0, f1) of a schedule.
+// Run one segment of a schedule: short rows and items, then the fix-ups.
 template <class K, typename T>
-void run_rows(hipStream_t st, const K& k, const DevSched& sc, int64_t i0, int64_t i1, int64_t f0, int64_t f1,
-              T* part, int kp) {
-  const int64_t ni = i1 - i0, nf = f1 - f0;
-  if (ni > 0) {
-    const Item* it = sc.items + i0;
-    constexpr int NV16 = 16 / sizeof(T);  // values per 16-byte vector
-    if (kp * (int)sizeof(T) >= 16) {
-      switch (kp / NV16) {  // lanes per row
-        case 1: launch_vec<K, T, 1, NV16>(st, k, it, ni, part); break;
-        case 2: launch_vec<K, T, 2, NV16>(st, k, it, ni, part); break;
-        case 4: launch_vec<K, T, 4, NV16>(st, k, it, ni, part); break;
-        case 8: launch_vec<K, T, 8, NV16>(st, k, it, ni, part); break;
-        default: launch_vec<K, T, 16, NV16>(st, k, it, ni, part); break;
-      }
-    } else if (kp == 2) {
-      launch_vec<K, T, 1, 2>(st, k, it, ni, part);
-    } else {
-      launch_vec<K, T, 1, 1>(st, k, it, ni, part);
-    }
-  }
-  if (nf > 0) k_rows_fix<K, T><<<(unsigned)nf, kBlock, 0, st>>>(k, sc.fix + f0, part, kp);
-}
-
-template <class K, typename T>
-void run_rows(hipStream_t st, const K& k, const DevSched& sc, T* part, int kp) {
-  run_rows<K, T>(st, k, sc, 0, sc.n_items, 0, sc.n_fix, part, kp);
-}
-
-// ---- many small levels in one workgroup -----------------------------------
-// Levels t0..t1-1 (ascending when dir > 0, else descending) of a level
-// schedule, processed by the 16 waves of one workgroup with a barrier between
-// levels (rows of a level depend only on earlier levels).  The device-scope
-// fences make each level's global writes visible to every wave before the
-// next level reads them.
-constexpr int kTailThreads = 1024;
-
-template <class K, typename T, int V, int NV>
-__global__ void __launch_bounds__(kTailThreads) k_levels_wg(K k, const Item* __restrict__ items,
-                                                            const int64_t* __restrict__ item_ptr,
-                                                            const Fixup* __restrict__ fx,
-                                                            const int64_t* __restrict__ fix_ptr,
-                                                            T* __restrict__ part, int t0, int t1, int dir) {
-  constexpr int KP = V * NV;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  constexpr int nw = kTailThreads / kWave;
-  const int slot = lane / V, pt = lane % V;
-  T acc[NV];
-  for (int step = 0; step < t1 - t0; ++step) {
-    const int t = dir > 0 ? t0 + step : t1 - 1 - step;
-    for (int64_t g = item_ptr[t] + wave; g < item_ptr[t + 1]; g += nw) {
-      const Item it = items[g];
-      if (it.slot < 0) {
-        for (uint32_t r = it.row0; r < it.row1; ++r) {
-          gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
-          if (slot == 0)
-#pragma unroll
-            for (int c = 0; c < NV; ++c) k.finish(r, pt * NV + c, acc[c]);
-        }
-      } else {
-        gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
-        if (slot == 0)
-#pragma unroll
-          for (int c = 0; c < NV; ++c) part[(int64_t)it.slot * KP + pt * NV + c] = acc[c];
-      }
-    }
-    __threadfence();
-    __syncthreads();
-    if (fix_ptr[t + 1] > fix_ptr[t]) {
-      // one wave per split row: lanes stride over its slots, then an xor tree
-      const int col = lane % KP;
-      for (int64_t f = fix_ptr[t] + wave; f < fix_ptr[t + 1]; f += nw) {
-        const Fixup fxr = fx[f];
-        T a = 0;
-        for (uint32_t s = fxr.slot0 + lane / KP; s < fxr.slot1; s += kWave / KP) a += part[(int64_t)s * KP + col];
-#pragma unroll
-        for (int off = KP; off < kWave; off <<= 1) a += __shfl_xor(a, off, kWave);
-        if (lane < KP) k.finish(fxr.row, col, a);
-      }
-      __threadfence();
-      __syncthreads();
-    }
-  }
-}
-
-template <class K, typename T>
-void run_levels_wg(hipStream_t st, const K& k, const DevSched& sc, const int64_t* item_ptr, const int64_t* fix_ptr,
-                   int t0, int t1, int dir, T* part, int kp) {
-  if (t1 <= t0) return;
-  constexpr int NV16 = 16 / sizeof(T);
-#define WA_LEVELS_WG(V, NV) \
-  k_levels_wg<K, T, V, NV><<<1, kTailThreads, 0, st>>>(k, sc.items, item_ptr, sc.fix, fix_ptr, part, t0, t1, dir)
+void run_seg(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T* part, int kp) {
+  constexpr int NV16 = 16 / sizeof(T);  // values per 16-byte vector
   if (kp * (int)sizeof(T) >= 16) {
-    switch (kp / NV16) {
-      case 1: WA_LEVELS_WG(1, NV16); break;
-      case 2: WA_LEVELS_WG(2, NV16); break;
-      case 4: WA_LEVELS_WG(4, NV16); break;
-      case 8: WA_LEVELS_WG(8, NV16); break;
-      default: WA_LEVELS_WG(16, NV16); break;
+    switch (kp / NV16) {  // lanes per row
+      case 1: launch_rows<K, T, 1, NV16>(st, k, sc, g, part); break;
+      case 2: launch_rows<K, T, 2, NV16>(st, k, sc, g, part); break;
+      case 4: launch_rows<K, T, 4, NV16>(st, k, sc, g, part); break;
+      case 8: launch_rows<K, T, 8, NV16>(st, k, sc, g, part); break;
+      default: launch_rows<K, T, 16, NV16>(st, k, sc, g, part); break;
     }
   } else if (kp == 2) {
-    WA_LEVELS_WG(1, 2);
+    launch_rows<K, T, 1, 2>(st, k, sc, g, part);
   } else {
-    WA_LEVELS_WG(1, 1);
+    launch_rows<K, T, 1, 1>(st, k, sc, g, part);
   }
-#undef WA_LEVELS_WG
+  if (g.f1 > g.f0) k_rows_fix<K, T><<<(unsigned)(g.f1 - g.f0), kFixBlock, 0, st>>>(k, sc.fix + g.f0, part, kp);
+}
+
+// Run every segment of a schedule (in order).
+template <class K, typename T>
+void run_rows(hipStream_t st, const K& k, const DevSched& sc, T* part, int kp) {
+  for (const Segment& g : sc.segs) run_seg<K, T>(st, k, sc, g, part, kp);
 }
 
 }  // namespace wa

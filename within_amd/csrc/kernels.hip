@@ -668,6 +668,174 @@ struct CoreKind {
   __device__ void finish(uint32_t r, int col, T acc) const { out[(int64_t)r * kp + col] = acc; }
 };
 
+// ---- dense cores ------------------------------------------------------------
+// C[b + i] = sum_j M[i, j] Z[core_j] for a core of n rows (matrix padded to
+// ld columns with zeros).  Z's rows are first gathered into Zc (ld x kp, rows
+// beyond n zero).  kp >= 8: 128-row tiles through LDS, each thread holding a
+// (kp/8) x 4 register block, the column range split in `split` parts whose
+// partials are summed in part order; kp < 8: one wave per 4 rows with lanes
+// along the row (a matrix-vector product).  Every sum has a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_core_gather(const uint32_t* __restrict__ cidx, const T* __restrict__ X,
+                                                        T* __restrict__ Zc, uint32_t n, uint32_t ld, int kp) {
+  const int lane = threadIdx.x & (kp - 1);
+  const int gpb = kBlock / kp;
+  for (int64_t j = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; j < ld; j += (int64_t)gridDim.x * gpb)
+    Zc[j * kp + lane] = j < n ? X[(int64_t)cidx[j] * kp + lane] : T(0);
+}
+
+constexpr int kCoreRows = 128;  // rows per tile
+
+template <typename T, int KP>
+__global__ void __launch_bounds__(kBlock) k_core_tiled(const T* __restrict__ M, const T* __restrict__ Zc,
+                                                       T* __restrict__ P, uint32_t n, uint32_t ld, uint32_t ks) {
+  constexpr int NV = 16 / sizeof(T);  // values per 16-byte vector
+  constexpr int TK = 128 / sizeof(T); // columns per step (128-byte row segments)
+  constexpr int TR = kCoreRows, CG = KP / 4, RT = KP / 8;
+  __shared__ T Ms[TK][TR + NV];
+  __shared__ __align__(16) T Zs[TK * KP];
+  const int tid = threadIdx.x;
+  const int g = tid % CG, rg = tid / CG;
+  const uint32_t r0 = blockIdx.x * TR;
+  const uint32_t ka = blockIdx.y * ks, kb = min(ld, ka + ks);
+  T acc[RT][4];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[i][c] = 0;
+  for (uint32_t k0 = ka; k0 < kb; k0 += TK) {
+    constexpr int VPR = TK / NV;  // vectors per row segment
+    for (int v = tid; v < TR * VPR; v += kBlock) {
+      const int row = v / VPR, kv = (v % VPR) * NV;
+      VecT<T, NV> x;
+      if (r0 + row < n) {
+        x = vload<T, NV>(M + (size_t)(r0 + row) * ld + k0 + kv);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) x.v[c] = 0;
+      }
+#pragma unroll
+      for (int c = 0; c < NV; ++c) Ms[kv + c][row] = x.v[c];
+    }
+    for (int v = tid; v < TK * KP / NV; v += kBlock)
+      *reinterpret_cast<VecT<T, NV>*>(&Zs[v * NV]) = vload<T, NV>(Zc + (size_t)k0 * KP + v * NV);
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < TK; ++kk) {
+      T mv[RT], zv[4];
+#pragma unroll
+      for (int i = 0; i < RT; ++i) mv[i] = Ms[kk][rg * RT + i];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) zv[c] = Zs[kk * KP + g * 4 + c];
+#pragma unroll
+      for (int i = 0; i < RT; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[i][c] += mv[i] * zv[c];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < RT; ++i) {
+    const uint32_t r = r0 + rg * RT + i;
+    if (r < n)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) P[((size_t)blockIdx.y * ld + r) * KP + g * 4 + c] = acc[i][c];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_core_sum(const T* __restrict__ P, T* __restrict__ C, uint32_t n,
+                                                     uint32_t ld, int split, int kp) {
+  const int64_t tot = (int64_t)n * kp;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kBlock) {
+    T a = 0;
+    for (int s = 0; s < split; ++s) a += P[(int64_t)s * ld * kp + i];
+    C[i] = a;
+  }
+}
+
+template <typename T, int KP>
+__global__ void __launch_bounds__(kBlock) k_core_gemv(const T* __restrict__ M, const T* __restrict__ Zc,
+                                                      T* __restrict__ C, uint32_t n, uint32_t ld) {
+  constexpr int NV = 16 / sizeof(T), R = 4;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t r0 = (uint32_t)(((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave) * R;
+  if (r0 >= n) return;
+  T acc[R][KP];
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int c = 0; c < KP; ++c) acc[i][c] = 0;
+  for (uint32_t k = lane * NV; k < ld; k += kWave * NV) {
+    VecT<T, NV> mv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (r0 + i < n) {
+        mv[i] = vload<T, NV>(M + (size_t)(r0 + i) * ld + k);
+      } else {
+#pragma unroll
+        for (int u = 0; u < NV; ++u) mv[i].v[u] = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      T zv[KP];
+#pragma unroll
+      for (int c = 0; c < KP; ++c) zv[c] = Zc[(size_t)(k + u) * KP + c];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int c = 0; c < KP; ++c) acc[i][c] += mv[i].v[u] * zv[c];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int c = 0; c < KP; ++c)
+#pragma unroll
+      for (int off = kWave / 2; off >= 1; off >>= 1) acc[i][c] += __shfl_xor(acc[i][c], off, kWave);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (r0 + i < n)
+#pragma unroll
+        for (int c = 0; c < KP; ++c) C[(size_t)(r0 + i) * KP + c] = acc[i][c];
+}
+
+// column split for a core: enough workgroups to cover the chip
+static inline int core_split(uint32_t n, uint32_t ld, int tk) {
+  const int tiles = (int)((n + kCoreRows - 1) / kCoreRows);
+  int s = std::max(1, std::min(kCoreSplitMax, 1024 / std::max(tiles, 1)));
+  return std::min<int>(s, (int)(ld / tk));
+}
+
+template <typename T>
+static void run_dense_core(hipStream_t st, const DenseCore<T>& d, const uint32_t* core_idx, const T* X, T* C,
+                           T* Zc, T* P, int kp) {
+  k_core_gather<T><<<grid_stride_blocks(d.ld, kp), kBlock, 0, st>>>(core_idx + d.b, X, Zc, d.n, d.ld, kp);
+  T* out = C + (size_t)d.b * kp;
+  if (kp >= 8) {
+    constexpr int TK = 128 / sizeof(T);
+    const int split = core_split(d.n, d.ld, TK);
+    const uint32_t ks = (uint32_t)(((d.ld / TK + split - 1) / split) * TK);
+    const int nsplit = (int)((d.ld + ks - 1) / ks);
+    dim3 grid((d.n + kCoreRows - 1) / kCoreRows, nsplit);
+    switch (kp) {
+      case 8: k_core_tiled<T, 8><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
+      case 16: k_core_tiled<T, 16><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
+      default: k_core_tiled<T, 32><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
+    }
+    k_core_sum<T><<<grid_stride_blocks((int64_t)d.n * kp, 1), kBlock, 0, st>>>(P, out, d.n, d.ld, nsplit, kp);
+  } else {
+    const unsigned blocks = (unsigned)(((d.n + 3) / 4 * (int64_t)kWave + kBlock - 1) / kBlock);
+    switch (kp) {
+      case 1: k_core_gemv<T, 1><<<blocks, kBlock, 0, st>>>(d.M, Zc, out, d.n, d.ld); break;
+      case 2: k_core_gemv<T, 2><<<blocks, kBlock, 0, st>>>(d.M, Zc, out, d.n, d.ld); break;
+      default: k_core_gemv<T, 4><<<blocks, kBlock, 0, st>>>(d.M, Zc, out, d.n, d.ld); break;
+    }
+  }
+}
+
 template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m) {
@@ -685,29 +853,20 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                                                       w.Rr, kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
     FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-    const int64_t last = p.n_levels - 1;  // the core gather level
-    for (int64_t t = 0; t < p.tail0; ++t)
-      run_rows<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.h_item_ptr[t], p.s_fwd.h_item_ptr[t + 1],
-                              p.s_fwd.h_fix_ptr[t], p.s_fwd.h_fix_ptr[t + 1], w.part, kp);
-    run_levels_wg<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.d_item_ptr, p.s_fwd.d_fix_ptr, (int)p.tail0, (int)last, 1,
-                                 w.part, kp);
-    if (last >= 0)
-      run_rows<FwdKind<T>, T>(st, fk, p.s_fwd, p.s_fwd.h_item_ptr[last], p.s_fwd.h_item_ptr[last + 1],
-                              p.s_fwd.h_fix_ptr[last], p.s_fwd.h_fix_ptr[last + 1], w.part, kp);
+    for (const Segment& g : p.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, p.s_fwd, g, w.part, kp);
     if (p.n_core_rows > 0) {
       run_rows<CoreKind<T>, T>(st,
                                CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
                                            p.core_mat_off, p.core_mat, kp},
                                p.s_core, w.part, kp);
+      for (const DenseCore<T>& d : p.big) run_dense_core<T>(st, d, p.core_idx, w.Rr, w.C, w.Zc, w.P, kp);
       k_scatter_rows<T><<<grid_stride_blocks(p.n_core_rows, kp), kBlock, 0, st>>>(0, p.n_core_rows, p.core_idx,
                                                                                   w.C, w.Rr, kp);
     }
     BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
-    run_levels_wg<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.d_item_ptr, p.s_bwd.d_fix_ptr, (int)p.tail0, (int)last, -1,
-                                 w.part, kp);
-    for (int64_t t = p.tail0 - 1; t >= 0; --t)
-      run_rows<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.h_item_ptr[t], p.s_bwd.h_item_ptr[t + 1],
-                              p.s_bwd.h_fix_ptr[t], p.s_bwd.h_fix_ptr[t + 1], w.part, kp);
+    // the last level gathers the dense cores: no backward step
+    for (int64_t t = (int64_t)p.s_bwd.segs.size() - 2; t >= 0; --t)
+      run_seg<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.segs[t], w.part, kp);
     // 4. kept block copied into the local solution
     k_scatter_rows<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(0, p.n_red, p.red_loc, w.Rr, w.S, kp);
   }
@@ -836,21 +995,19 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
       k_sub_mean<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2,
                                                                    w.Rr, kp);
       FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-      for (int64_t t = 0; t < p.n_levels; ++t)
-        run_rows<FwdKind<T>, T>(st, fk, ps.s_fwd, ps.s_fwd.h_item_ptr[t], ps.s_fwd.h_item_ptr[t + 1],
-                                ps.s_fwd.h_fix_ptr[t], ps.s_fwd.h_fix_ptr[t + 1], w.part, kp);
+      for (const Segment& sg : ps.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, ps.s_fwd, sg, w.part, kp);
       if (ps.core1 > ps.core0) {
         run_rows<CoreKind<T>, T>(st,
                                  CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
                                              p.core_mat_off, p.core_mat, kp},
                                  ps.s_core, w.part, kp);
+        for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, w.Zc, w.P, kp);
         k_scatter_rows<T><<<grid_stride_blocks(ps.core1 - ps.core0, kp), kBlock, 0, st>>>(
             ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
       }
       BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
-      for (int64_t t = p.n_levels - 2; t >= 0; --t)
-        run_rows<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.s_bwd.h_item_ptr[t], ps.s_bwd.h_item_ptr[t + 1],
-                                ps.s_bwd.h_fix_ptr[t], ps.s_bwd.h_fix_ptr[t + 1], w.part, kp);
+      for (int64_t t = (int64_t)ps.s_bwd.segs.size() - 2; t >= 0; --t)
+        run_seg<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.s_bwd.segs[t], w.part, kp);
       k_scatter_rows<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
     }
     run_rows<ElimKind<T>, T>(st,

@@ -13,8 +13,8 @@ namespace wa {
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 constexpr int kRedBlocks = 512;   // fixed reduction grid (deterministic sums)
 constexpr uint32_t kChunk = 256;  // elements per work item (long rows are split)
-constexpr uint32_t kMaxRows = 32; // rows packed into one work item
-constexpr int64_t kTailRows = 1024; // levels this small run inside one workgroup
+constexpr uint32_t kShortRow = 16;   // rows this short are summed by one lane group
+constexpr uint32_t kShortPass = 8;   // the same for factor passes (several gathers per element)
 
 struct DevSched {
   const Item* items = nullptr;
@@ -22,9 +22,8 @@ struct DevSched {
   const Fixup* fix = nullptr;
   int64_t n_fix = 0;
   int64_t n_slots = 0;
-  std::vector<int64_t> h_item_ptr, h_fix_ptr;  // per-level ranges (level schedules only)
-  const int64_t* d_item_ptr = nullptr;         // device copies of the per-level ranges
-  const int64_t* d_fix_ptr = nullptr;
+  uint32_t short_max = 0;
+  std::vector<Segment> segs;  // row ranges (one per level for level schedules)
 };
 
 // one level-sorted factor pass (plan.hpp FactorPass) in precision T
@@ -50,10 +49,19 @@ struct DevCsr {
   DevSched sched;
 };
 
+// A dense core applied by the dense kernels: core rows [b, b + n), matrix
+// n x n row-major with leading dimension ld (a multiple of 32, zero padded).
+constexpr uint32_t kDenseCore = 512;  // cores at least this large
+constexpr int kCoreSplitMax = 16;     // split of the column range (partials)
+template <typename T>
+struct DenseCore {
+  uint32_t b = 0, n = 0, ld = 0;
+  const T* M = nullptr;
+};
+
 template <typename T>
 struct DevPrecond {
   int64_t n_loc = 0, n_red = 0, n_sub = 0, n_elim = 0, n_levels = 0, n_core_rows = 0;
-  int64_t tail0 = 0;  // elimination levels [tail0, n_levels - 1) run in one workgroup
   const uint32_t *loc_gidx, *loc_ptr, *red_ptr, *loc_sub, *red_sub, *red_loc;
   const T* loc_scale;
   const uint32_t *elim_loc, *elim_ptr, *elim_idx;
@@ -67,12 +75,14 @@ struct DevPrecond {
   const T* core_mat;
   const uint32_t *dof_ptr, *dof_loc;
   DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
+  std::vector<DenseCore<T>> big;  // cores applied by the dense kernels (not in s_core)
 };
 
 // subranges of one factor pair, for the multiplicative sweep
 struct PairSched {
   int64_t loc0 = 0, loc1 = 0, red0 = 0, red1 = 0, core0 = 0, core1 = 0;
   DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
+  std::vector<size_t> big;  // indices into DevPrecond::big
 };
 
 // scratch for one preconditioner apply (kp columns)
@@ -80,6 +90,7 @@ template <typename T>
 struct PrecondWork {
   T *B = nullptr, *S = nullptr, *Rr = nullptr, *C = nullptr;
   T *sum1 = nullptr, *sum2 = nullptr, *sum3 = nullptr, *part = nullptr;
+  T *Zc = nullptr, *P = nullptr;  // dense cores: gathered right-hand side, column-split partials
 };
 
 // per-column Krylov state (FP64 scalars)

@@ -63,35 +63,32 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
 // ---------------------------------------------------------------------------
 // segmented-reduction work items
 // ---------------------------------------------------------------------------
-void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk, u32 max_rows) {
-  u32 r = r0;
-  while (r < r1) {
-    u32 len = row_ptr[r + 1] - row_ptr[r];
-    if (len > chunk) {
-      u32 slot0 = (u32)s.n_slots;
-      for (u32 e0 = row_ptr[r]; e0 < row_ptr[r + 1]; e0 += chunk) {
-        u32 e1 = std::min(e0 + chunk, row_ptr[r + 1]);
-        s.items.push_back({r, r + 1, e0, e1, (int32_t)s.n_slots++, 0});
-      }
-      s.fixups.push_back({r, slot0, (u32)s.n_slots, 0});
-      ++r;
+void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk,
+                     const std::vector<char>* skip) {
+  Segment g{r0, r1, (i64)s.items.size(), 0, (i64)s.fixups.size(), 0};
+  for (u32 r = r0; r < r1; ++r) {
+    const u32 len = row_ptr[r + 1] - row_ptr[r];
+    if (len <= s.short_max || (skip && (*skip)[r])) continue;
+    if (len <= chunk) {
+      s.items.push_back({r, r + 1, row_ptr[r], row_ptr[r + 1], -1, 0});
       continue;
     }
-    u32 q = r, tot = 0;
-    while (q < r1 && q - r < max_rows) {
-      u32 l = row_ptr[q + 1] - row_ptr[q];
-      if (l > chunk || (q > r && tot + l > chunk)) break;
-      tot += l;
-      ++q;
+    const u32 slot0 = (u32)s.n_slots;
+    for (u32 e0 = row_ptr[r]; e0 < row_ptr[r + 1]; e0 += chunk) {
+      const u32 e1 = std::min(e0 + chunk, row_ptr[r + 1]);
+      s.items.push_back({r, r + 1, e0, e1, (int32_t)s.n_slots++, 0});
     }
-    s.items.push_back({r, q, row_ptr[r], row_ptr[q], -1, 0});
-    r = q;
+    s.fixups.push_back({r, slot0, (u32)s.n_slots, 0});
   }
+  g.i1 = (i64)s.items.size();
+  g.f1 = (i64)s.fixups.size();
+  s.segs.push_back(g);
 }
 
-Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows) {
+Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 short_max, const std::vector<char>* skip) {
   Schedule s;
-  append_schedule(s, row_ptr, 0, (u32)row_ptr.size() - 1, chunk, max_rows);
+  s.short_max = short_max;
+  append_schedule(s, row_ptr, 0, (u32)row_ptr.size() - 1, chunk, skip);
   return s;
 }
 
@@ -371,43 +368,121 @@ static void sample_clique_tree(const std::vector<std::pair<u32, double>>& nbrs, 
 }
 
 // Cholesky of the anchored minor (drop the last index) of a dense n x n
-// Laplacian-like matrix, then its inverse T'T with T = L^-1 (sums over k
-// ascending).  Empty result = minor not positive definite.
+// Laplacian-like matrix, then its inverse.  Entry by entry the arithmetic is
+// fixed:
+//   l_ij = (a_ij - sum_{k<j} l_ik l_jk) / l_jj,  l_jj = sqrt(a_jj - sum_{k<j} l_jk^2),
+//   U = L^-1:  U_ij = (delta_ij - sum_{k=j}^{i-1} l_ik U_kj) / l_ii,
+//   inv_ij = sum_{k >= max(i,j)} U_ki U_kj,
+// every sum taken term by term for k ascending from 0 (from j, from
+// max(i,j)).  The loops below are blocked and vectorised across entries, which
+// leaves each entry's sequence of operations unchanged (a term whose factor is
+// an exact zero leaves a nonzero or +0 partial sum unchanged).  Empty result =
+// minor not positive definite.
 std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n) {
   std::vector<double> inv(n * n, 0.0);
-  i64 m = n - 1;
+  const i64 m = n - 1;
   if (m <= 0) return inv;
-  std::vector<double> l(m * m);
+  std::vector<double> l(m * m, 0.0);
   for (i64 i = 0; i < m; ++i)
-    for (i64 j = 0; j < m; ++j) l[i * m + j] = a[i * n + j];
-  for (i64 j = 0; j < m; ++j) {
-    double s = l[j * m + j];
-    for (i64 k = 0; k < j; ++k) s -= l[j * m + k] * l[j * m + k];
-    if (!(s > 0.0)) return {};
-    double ljj = std::sqrt(s);
-    l[j * m + j] = ljj;
-#pragma omp parallel for schedule(static) if (m - j > 512)
-    for (i64 i = j + 1; i < m; ++i) {
-      double t = l[i * m + j];
-      for (i64 k = 0; k < j; ++k) t -= l[i * m + k] * l[j * m + k];
-      l[i * m + j] = t / ljj;
+    for (i64 j = 0; j <= i; ++j) l[i * m + j] = a[i * n + j];
+  // 1. Cholesky, right-looking over panels of nb columns: the trailing update
+  //    subtracts a panel's products from each entry in k order.
+  constexpr i64 nb = 32, jt = 512;
+  std::vector<double> pt;
+  for (i64 k0 = 0; k0 < m; k0 += nb) {
+    const i64 k1 = std::min(k0 + nb, m);
+    for (i64 j = k0; j < k1; ++j) {
+      double s = l[j * m + j];
+      for (i64 k = k0; k < j; ++k) s -= l[j * m + k] * l[j * m + k];
+      if (!(s > 0.0)) return {};
+      const double ljj = std::sqrt(s);
+      l[j * m + j] = ljj;
+      for (i64 i = j + 1; i < k1; ++i) {
+        double t = l[i * m + j];
+        for (i64 k = k0; k < j; ++k) t -= l[i * m + k] * l[j * m + k];
+        l[i * m + j] = t / ljj;
+      }
+    }
+    if (k1 == m) break;
+#pragma omp parallel for schedule(static) if (m - k1 > 256)
+    for (i64 i = k1; i < m; ++i)
+      for (i64 j = k0; j < k1; ++j) {
+        double t = l[i * m + j];
+        for (i64 k = k0; k < j; ++k) t -= l[i * m + k] * l[j * m + k];
+        l[i * m + j] = t / l[j * m + j];
+      }
+    pt.assign((size_t)(k1 - k0) * m, 0.0);  // pt[kk][j] = l_j,(k0+kk)
+    for (i64 j = k1; j < m; ++j)
+      for (i64 kk = 0; kk < k1 - k0; ++kk) pt[kk * m + j] = l[j * m + k0 + kk];
+#pragma omp parallel for schedule(dynamic, 16) if (m - k1 > 256)
+    for (i64 i = k1; i < m; ++i) {
+      double* __restrict li = &l[i * m];
+      for (i64 j0 = k1; j0 <= i; j0 += jt) {
+        const i64 j1 = std::min(j0 + jt, i + 1);
+        for (i64 kk = 0; kk < k1 - k0; ++kk) {
+          const double c = li[k0 + kk];
+          const double* __restrict p = &pt[kk * m];
+          for (i64 j = j0; j < j1; ++j) li[j] -= c * p[j];
+        }
+      }
     }
   }
-  std::vector<double> T(m * m, 0.0);
-#pragma omp parallel for schedule(dynamic, 8) if (m > 256)
-  for (i64 j = 0; j < m; ++j)
-    for (i64 i = j; i < m; ++i) {
-      double s = i == j ? 1.0 : 0.0;
-      for (i64 k = j; k < i; ++k) s -= l[i * m + k] * T[j * m + k];
-      T[j * m + i] = s / l[i * m + i];
+  // 2. U = L^-1 by forward substitution, columns in blocks of jb (one block
+  //    per task, rows vectorised across the block's columns).
+  constexpr i64 jb = 32;
+  std::vector<double> u(m * m, 0.0);
+  const i64 nblk = (m + jb - 1) / jb;
+#pragma omp parallel if (m > 256)
+  {
+    std::vector<double> ub;  // ub[i][jj] = U_i,(j0+jj) for i >= j0
+    std::vector<double> s(jb);
+#pragma omp for schedule(dynamic, 1)
+    for (i64 bi = 0; bi < nblk; ++bi) {
+      const i64 j0 = bi * jb, w = std::min(jb, m - j0);
+      ub.assign((size_t)(m - j0) * jb, 0.0);
+      for (i64 i = j0; i < m; ++i) {
+        for (i64 jj = 0; jj < jb; ++jj) s[jj] = (jj < w && i == j0 + jj) ? 1.0 : 0.0;
+        const double* __restrict li = &l[i * m];
+        double* __restrict sp = s.data();
+        for (i64 k = j0; k < i; ++k) {
+          const double c = li[k];
+          const double* __restrict uk = &ub[(k - j0) * jb];
+          for (i64 jj = 0; jj < jb; ++jj) sp[jj] -= c * uk[jj];
+        }
+        const double lii = li[i];
+        double* ui = &ub[(i - j0) * jb];
+        for (i64 jj = 0; jj < w; ++jj) ui[jj] = (i >= j0 + jj) ? s[jj] / lii : 0.0;
+      }
+      for (i64 i = j0; i < m; ++i)
+        for (i64 jj = 0; jj < w && j0 + jj <= i; ++jj) u[i * m + j0 + jj] = ub[(i - j0) * jb + jj];
     }
-#pragma omp parallel for schedule(dynamic, 8) if (m > 256)
-  for (i64 i = 0; i < m; ++i)
-    for (i64 j = 0; j < m; ++j) {
-      double s = 0.0;
-      for (i64 k = std::max(i, j); k < m; ++k) s += T[i * m + k] * T[j * m + k];
-      inv[i * n + j] = s;
+  }
+  // 3. inv_ij (j <= i) = sum_{k >= i} U_ki U_kj, rows in blocks of ib; mirrored.
+  constexpr i64 ib = 32;
+  const i64 nrb = (m + ib - 1) / ib;
+#pragma omp parallel if (m > 256)
+  {
+    std::vector<double> acc;
+#pragma omp for schedule(dynamic, 1)
+    for (i64 bi = 0; bi < nrb; ++bi) {
+      const i64 i0 = bi * ib, i1 = std::min(i0 + ib, m);
+      acc.assign((size_t)(i1 - i0) * i1, 0.0);  // acc[i - i0][j], j <= i
+      for (i64 k = i0; k < m; ++k) {
+        const double* __restrict uk = &u[k * m];
+        for (i64 i = i0; i < std::min(i1, k + 1); ++i) {
+          const double c = uk[i];
+          double* __restrict ai = &acc[(i - i0) * i1];
+          for (i64 j = 0; j <= i; ++j) ai[j] += c * uk[j];
+        }
+      }
+      for (i64 i = i0; i < i1; ++i)
+        for (i64 j = 0; j <= i; ++j) {
+          const double v = acc[(i - i0) * i1 + j];
+          inv[i * n + j] = v;
+          inv[j * n + i] = v;
+        }
     }
+  }
   return inv;
 }
 
@@ -859,8 +934,11 @@ void build_plan(Plan& p) {
     PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (size_t i = 0; i < p.subs.size(); ++i) {
+    // subdomains with a large reduced system first, one at a time (their dense
+    // cores use every thread), then the rest in parallel
+    std::vector<size_t> serial, rest;
+    for (size_t i = 0; i < p.subs.size(); ++i) (p.subs[i].n_keep() >= 1024 ? serial : rest).push_back(i);
+    auto build_one = [&](size_t i) {
       try {
         auto t0 = std::chrono::steady_clock::now();
         build_local_solver(p.subs[i], p.cfg);
@@ -873,7 +951,10 @@ void build_plan(Plan& p) {
         err[i] = e.what();
         code[i] = e.code;
       }
-    }
+    };
+    for (size_t i : serial) build_one(i);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t t = 0; t < rest.size(); ++t) build_one(rest[t]);
     for (size_t i = 0; i < p.subs.size(); ++i)
       if (code[i]) throw Error(code[i], err[i]);
     PhaseTimer tf("flatten");

@@ -34,7 +34,7 @@ struct LocalConfig {
   uint64_t schur_seed = 0;
   int schur_split = 1;
   int dense_threshold = 24;
-  i64 dense_core = 2048;
+  i64 dense_core = 8192;
 };
 
 struct Design {
@@ -48,19 +48,29 @@ struct Design {
 };
 Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* weights);
 
-// Work decomposition of a segmented reduction: a work item either covers whole
-// rows [row0, row1) (written directly) or one chunk [e0, e1) of a long row
-// (written to partial slot `slot`, summed in slot order by a fix-up).
+// Work decomposition of a segmented reduction over rows [r0, r1) of a CSR:
+// rows of at most `short_max` elements need no item (each is summed by one lane
+// group); a longer row is one work item [row0, row1 = row0 + 1) (written
+// directly) or, beyond `chunk` elements, several chunks [e0, e1) written to
+// partial slots `slot` that a fix-up sums in slot order.
 struct Item { u32 row0, row1, e0, e1; int32_t slot; int32_t pad; };
 struct Fixup { u32 row, slot0, slot1, pad; };
+// one row range with its items [i0, i1) and fix-ups [f0, f1)
+struct Segment { u32 r0, r1; i64 i0, i1, f0, f1; };
 struct Schedule {
   std::vector<Item> items;
   std::vector<Fixup> fixups;
+  std::vector<Segment> segs;
   i64 n_slots = 0;
+  u32 short_max = 0;
 };
-Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 max_rows);
-// items for rows [r0, r1) only, appended to `s` (slots continue from s.n_slots)
-void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk, u32 max_rows);
+// rows r with skip[r] set get no item (another kernel computes them; they
+// must be longer than short_max)
+Schedule make_schedule(const std::vector<u32>& row_ptr, u32 chunk, u32 short_max,
+                       const std::vector<char>* skip = nullptr);
+// one more segment, rows [r0, r1) (slots continue from s.n_slots)
+void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk,
+                     const std::vector<char>* skip = nullptr);
 
 // Level-sorted CSR of one factor: rows = levels of q, elements = observations
 // sorted by (level, code of the largest other factor, observation index).

@@ -167,6 +167,8 @@ struct wa_solver {
     d.fix = upload(s.fixups);
     d.n_fix = (int64_t)s.fixups.size();
     d.n_slots = s.n_slots;
+    d.short_max = s.short_max;
+    d.segs = s.segs;
     return d;
   }
   ~wa_solver() {
@@ -283,7 +285,7 @@ void wa_solver::setup_device() {
     a.off = b.off = (uint32_t)d.offset[fp.q];
     a.levels = b.levels = d.levels[fp.q];
     a.n_elem = b.n_elem = n;
-    a.sched = b.sched = upload_sched(make_schedule(fp.row_ptr, kChunk, kMaxRows));
+    a.sched = b.sched = upload_sched(make_schedule(fp.row_ptr, kChunk, kShortPass));
     max_slots = std::max(max_slots, a.sched.n_slots);
     p64.push_back(a);
     p32.push_back(b);
@@ -296,12 +298,47 @@ void wa_solver::setup_device() {
     g32.val = upload_as<float>(g.data);
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
-    g64.sched = g32.sched = upload_sched(make_schedule(g.indptr, kChunk, kMaxRows));
+    g64.sched = g32.sched = upload_sched(make_schedule(g.indptr, kChunk, kShortRow));
     max_slots = std::max(max_slots, g64.sched.n_slots);
     has_gram = true;
   }
   if (!plan.subs.empty()) {
     const PrecondFlat& f = plan.pre;
+    // dense cores: the large ones are applied by the dense kernels from
+    // zero-padded copies; the rest keep the gather path (their matrices packed)
+    const size_t n_cores = f.core_ptr.empty() ? 0 : f.core_ptr.size() - 1;
+    std::vector<char> big_row(f.core_idx.size(), 0);
+    std::vector<double> small_mat;
+    std::vector<uint64_t> small_off(n_cores, 0);
+    std::vector<size_t> big_cores;
+    for (size_t c = 0; c < n_cores; ++c) {
+      const uint32_t b = f.core_ptr[c], nc = f.core_ptr[c + 1] - b;
+      const double* src = f.core_mat.data() + f.core_mat_off[c];
+      if (nc >= kDenseCore) {
+        big_cores.push_back(c);
+        std::fill(big_row.begin() + b, big_row.begin() + b + nc, 1);
+      } else {
+        small_off[c] = small_mat.size();
+        small_mat.insert(small_mat.end(), src, src + (size_t)nc * nc);
+      }
+    }
+    auto upload_big = [&](auto& P, auto tag) {
+      using T = decltype(tag);
+      for (size_t c : big_cores) {
+        DenseCore<T> d;
+        d.b = f.core_ptr[c];
+        d.n = f.core_ptr[c + 1] - d.b;
+        d.ld = (d.n + 31) / 32 * 32;
+        std::vector<T> m((size_t)d.n * d.ld, T(0));
+        const double* src = f.core_mat.data() + f.core_mat_off[c];
+        for (size_t i = 0; i < d.n; ++i)
+          for (size_t j = 0; j < d.n; ++j) m[i * d.ld + j] = (T)src[i * d.n + j];
+        d.M = upload(m);
+        P.big.push_back(d);
+      }
+    };
+    upload_big(pre64, double());
+    upload_big(pre32, float());
     auto fill = [&](auto& P, auto tag) {
       using T = decltype(tag);
       P.n_loc = f.n_loc;
@@ -318,7 +355,7 @@ void wa_solver::setup_device() {
       P.fin_val = upload_as<T>(f.fin_val);
       P.bout_val = upload_as<T>(f.bout_val);
       P.inv_pivot = upload_as<T>(f.inv_pivot);
-      P.core_mat = upload_as<T>(f.core_mat);
+      P.core_mat = upload_as<T>(small_mat);
     };
     fill(pre64, double());
     fill(pre32, float());
@@ -342,7 +379,7 @@ void wa_solver::setup_device() {
     const uint32_t* core_ptr = upload(f.core_ptr);
     const uint32_t* core_idx = upload(f.core_idx);
     const uint32_t* core_row_core = upload(f.core_row_core);
-    const uint64_t* core_mat_off = upload(f.core_mat_off);
+    const uint64_t* core_mat_off = upload(small_off);
     const uint32_t* dof_ptr = upload(f.dof_ptr);
     const uint32_t* dof_loc = upload(f.dof_loc);
     auto ints = [&](auto& P) {
@@ -365,30 +402,18 @@ void wa_solver::setup_device() {
     const uint32_t* d_core_row_ptr = upload(core_row_ptr);
     auto level_sched = [&](const std::vector<uint32_t>& ptr) {
       Schedule sc;
-      std::vector<int64_t> ip(1, 0), fp(1, 0);
-      for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) {
-        append_schedule(sc, ptr, f.lvl_ptr[t], f.lvl_ptr[t + 1], kChunk, kMaxRows);
-        ip.push_back((int64_t)sc.items.size());
-        fp.push_back((int64_t)sc.fixups.size());
-      }
-      DevSched d = upload_sched(sc);
-      d.h_item_ptr = ip;
-      d.h_fix_ptr = fp;
-      d.d_item_ptr = upload(ip);
-      d.d_fix_ptr = upload(fp);
-      return d;
+      sc.short_max = kShortRow;
+      for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) append_schedule(sc, ptr, f.lvl_ptr[t], f.lvl_ptr[t + 1], kChunk);
+      return upload_sched(sc);
     };
-    // trailing elimination levels small enough for one workgroup
-    int64_t n_lv = (int64_t)f.lvl_ptr.size() - 1, tail0 = std::max<int64_t>(n_lv - 1, 0);
-    while (tail0 > 0 && f.lvl_ptr[tail0] - f.lvl_ptr[tail0 - 1] <= kTailRows) --tail0;
-    pre64.tail0 = pre32.tail0 = tail0;
-    DevSched s_loc = upload_sched(make_schedule(f.loc_ptr, kChunk, kMaxRows));
-    DevSched s_red = upload_sched(make_schedule(f.red_ptr, kChunk, kMaxRows));
-    DevSched s_keep = upload_sched(make_schedule(f.keep_ptr, kChunk, kMaxRows));
-    DevSched s_elim = upload_sched(make_schedule(f.elim_ptr, kChunk, kMaxRows));
+    DevSched s_loc = upload_sched(make_schedule(f.loc_ptr, kChunk, kShortRow));
+    DevSched s_red = upload_sched(make_schedule(f.red_ptr, kChunk, kShortRow));
+    DevSched s_keep = upload_sched(make_schedule(f.keep_ptr, kChunk, kShortRow));
+    DevSched s_elim = upload_sched(make_schedule(f.elim_ptr, kChunk, kShortRow));
     DevSched s_fwd = level_sched(f.fin_ptr);
     DevSched s_bwd = level_sched(f.bout_ptr);
-    DevSched s_core = upload_sched(make_schedule(core_row_ptr, kChunk, kMaxRows));
+    static_assert(kDenseCore > kShortRow, "dense-core rows must not be short rows");
+    DevSched s_core = upload_sched(make_schedule(core_row_ptr, kChunk, kShortRow, &big_row));
     for (const DevSched* d : {&s_loc, &s_red, &s_keep, &s_elim, &s_fwd, &s_bwd, &s_core})
       max_slots = std::max(max_slots, d->n_slots);
     auto scheds = [&](auto& P) {
@@ -415,37 +440,31 @@ void wa_solver::setup_device() {
         ps.loc0 = f.loc_ptr[s0]; ps.loc1 = f.loc_ptr[s1];
         ps.red0 = f.red_ptr[s0]; ps.red1 = f.red_ptr[s1];
         ps.core0 = core_row_sub[s0]; ps.core1 = core_row_sub[s1];
-        Schedule a;
-        append_schedule(a, f.loc_ptr, (u32)s0, (u32)s1, kChunk, kMaxRows);
-        ps.s_loc = upload_sched(a);
-        Schedule b;
-        append_schedule(b, f.red_ptr, (u32)s0, (u32)s1, kChunk, kMaxRows);
-        ps.s_red = upload_sched(b);
-        Schedule c;
-        append_schedule(c, f.keep_ptr, (u32)ps.red0, (u32)ps.red1, kChunk, kMaxRows);
-        ps.s_keep = upload_sched(c);
-        Schedule e;
-        append_schedule(e, f.elim_ptr, (u32)elim_ptr_sub[s0], (u32)elim_ptr_sub[s1], kChunk, kMaxRows);
-        ps.s_elim = upload_sched(e);
-        Schedule cr;
-        append_schedule(cr, core_row_ptr, (u32)ps.core0, (u32)ps.core1, kChunk, kMaxRows);
-        ps.s_core = upload_sched(cr);
+        auto range = [&](const std::vector<uint32_t>& ptr, int64_t r0, int64_t r1,
+                         const std::vector<char>* skip = nullptr) {
+          Schedule sc;
+          sc.short_max = kShortRow;
+          append_schedule(sc, ptr, (u32)r0, (u32)r1, kChunk, skip);
+          return upload_sched(sc);
+        };
+        ps.s_loc = range(f.loc_ptr, (int64_t)s0, (int64_t)s1);
+        ps.s_red = range(f.red_ptr, (int64_t)s0, (int64_t)s1);
+        ps.s_keep = range(f.keep_ptr, ps.red0, ps.red1);
+        ps.s_elim = range(f.elim_ptr, elim_ptr_sub[s0], elim_ptr_sub[s1]);
+        ps.s_core = range(core_row_ptr, ps.core0, ps.core1, &big_row);
+        for (size_t bi = 0; bi < pre64.big.size(); ++bi)
+          if (pre64.big[bi].b >= ps.core0 && pre64.big[bi].b < ps.core1) ps.big.push_back(bi);
         auto pair_levels = [&](const std::vector<uint32_t>& ptr) {
           Schedule sc;
-          std::vector<int64_t> ip(1, 0), fp(1, 0);
+          sc.short_max = kShortRow;
           for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) {
             u32 a0 = f.lvl_ptr[t], a1 = f.lvl_ptr[t + 1];
             while (a0 < a1 && f.red_sub[f.lvl_rows[a0]] < s0) ++a0;
             u32 b1 = a0;
             while (b1 < a1 && f.red_sub[f.lvl_rows[b1]] < s1) ++b1;
-            append_schedule(sc, ptr, a0, b1, kChunk, kMaxRows);
-            ip.push_back((int64_t)sc.items.size());
-            fp.push_back((int64_t)sc.fixups.size());
+            append_schedule(sc, ptr, a0, b1, kChunk);
           }
-          DevSched d = upload_sched(sc);
-          d.h_item_ptr = ip;
-          d.h_fix_ptr = fp;
-          return d;
+          return upload_sched(sc);
         };
         ps.s_fwd = pair_levels(f.fin_ptr);
         ps.s_bwd = pair_levels(f.bout_ptr);
@@ -464,6 +483,14 @@ void wa_solver::setup_device() {
     w64.S = dalloc<double>(f.n_loc * KMAX);
     w64.Rr = dalloc<double>(f.n_red * KMAX);
     w64.C = dalloc<double>((int64_t)f.core_idx.size() * KMAX);
+    int64_t max_ld = 0;
+    for (const DenseCore<double>& d : pre64.big) max_ld = std::max<int64_t>(max_ld, d.ld);
+    if (max_ld > 0) {
+      w64.Zc = dalloc<double>(max_ld * KMAX);
+      w64.P = dalloc<double>(kCoreSplitMax * max_ld * KMAX);
+      w32.Zc = dalloc<float>(max_ld * KMAX);
+      w32.P = dalloc<float>(kCoreSplitMax * max_ld * KMAX);
+    }
     w64.sum1 = dalloc<double>(nsub * KMAX);
     w64.sum2 = dalloc<double>(nsub * KMAX);
     w64.sum3 = dalloc<double>(nsub * KMAX);
@@ -1014,7 +1041,7 @@ wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** o
     pl.cfg.schur_seed = p->local.schur_seed;
     pl.cfg.schur_split = p->local.schur_split;
     pl.cfg.dense_threshold = p->local.dense_threshold;
-    pl.cfg.dense_core = p->local.dense_core == 0 ? 2048 : p->local.dense_core;
+    pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
     pl.precond = p->preconditioner;
     pl.explicit_op = p->operator_repr == 1;
     build_plan(pl);
