@@ -91,22 +91,25 @@ class Run:
         return dt, [(st.converged, st.iterations, st.residual) for st in stats]
 
 
-def cpu_baseline(args):
-    """Oracle (FP64 CPU restatement) on a bounded sample of the same generator."""
+def cpu_baseline(args, p_full):
+    """Oracle (FP64 CPU restatement) timed on a bounded sample: the same
+    workload (or a fraction of it), one RHS per host thread."""
     import oracle
     from within_amd import synth
 
     scale = args.cpu_scale
-    p = make_problem(scale, seed=20)
-    Y = synth.akm_response(p, k=args.cpu_k, beta_cols=args.cpu_k - 1)
-    threads = os.cpu_count() or 1
-    threads = min(threads, 16)
+    p = p_full if scale == args.scale else make_problem(scale, seed=20)
+    threads = min(os.cpu_count() or 1, 16)
+    k = args.cpu_k or threads
+    Y = synth.akm_response(p, k=k, beta_cols=k - 1)
+    t0 = time.perf_counter()
     o = oracle.OracleSolver(p["codes"], ac_split=1, dense_core=1)  # pure approximate Cholesky
+    setup = time.perf_counter() - t0
     t, conv = o.time_solve_batch(Y, threads=threads)
-    return {"value": round(args.cpu_k / t, 4), "unit": "RHS/s", "cores": threads, "kind": "port",
-            "sample": f"same AKM generator at n={p['codes'].shape[0]} ({scale:g} of 20M), k={args.cpu_k}, "
-                      f"additive Schwarz (Rust default local solver, pure approximate Cholesky) + CG FP64 (oracle/), setup excluded, "
-                      f"{t:.2f} s",
+    return {"value": round(k / t, 4), "unit": "RHS/s", "cores": threads, "kind": "port",
+            "sample": f"AKM workload n={p['codes'].shape[0]} ({scale:g} of 20M obs), k={k} RHS (one per thread), "
+                      f"additive Schwarz (Rust default local solver, pure approximate Cholesky) + CG FP64 "
+                      f"(oracle/), tol 1e-8; solve {t:.2f} s, setup {setup:.1f} s excluded",
             "converged": int(conv.sum())}
 
 
@@ -118,8 +121,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 20M-observation workload")
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--baseline-rhs", type=int, default=32, help="RHS timed for the plain FP64 k=1 baseline")
-    ap.add_argument("--cpu-scale", type=float, default=0.05)
-    ap.add_argument("--cpu-k", type=int, default=4)
+    ap.add_argument("--cpu-scale", type=float, default=1.0, help="workload fraction of the CPU sample")
+    ap.add_argument("--cpu-k", type=int, default=0, help="RHS in the CPU sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-9,
                     help="Krylov tolerance for both the measured path and the plain baseline")
@@ -243,7 +246,7 @@ def main():
     }
     if not args.no_cpu:
         log("bench: CPU oracle sample ...")
-        out["cpu_baseline"] = cpu_baseline(args)
+        out["cpu_baseline"] = cpu_baseline(args, p)
     print(json.dumps(out), flush=True)
 
 

@@ -37,6 +37,19 @@ __device__ __forceinline__ VecT<T, NV> vload(const T* p) {
   return *reinterpret_cast<const VecT<T, NV>*>(p);
 }
 
+template <typename T, int NV>
+__device__ __forceinline__ void vstore(T* p, const VecT<T, NV>& v) {
+  *reinterpret_cast<VecT<T, NV>*>(p) = v;
+}
+
+template <typename T, int NV>
+__device__ __forceinline__ void vstore(T* p, const T (&a)[NV]) {
+  VecT<T, NV> v;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) v.v[c] = a[c];
+  vstore<T, NV>(p, v);
+}
+
 // one row by one lane group: elements in order, U gathers in flight
 template <class K, typename T, int KP, int NV>
 __device__ __forceinline__ void gather_row(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int part,
@@ -148,8 +161,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
     const uint32_t e0 = k.rbegin((uint32_t)r), e1 = k.rend((uint32_t)r);
     if (e1 - e0 > short_max) return;
     gather_row<K, T, KP, NV>(k, (uint32_t)r, e0, e1, pt, acc);
-#pragma unroll
-    for (int c = 0; c < NV; ++c) k.finish((uint32_t)r, pt * NV + c, acc[c]);
+    k.template finish<NV>((uint32_t)r, pt * NV, acc);
     return;
   }
   g -= n_sw;
@@ -158,9 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
   if (it.slot < 0) {
     for (uint32_t r = it.row0; r < it.row1; ++r) {
       gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
-      if (slot == 0)
-#pragma unroll
-        for (int c = 0; c < NV; ++c) k.finish(r, pt * NV + c, acc[c]);
+      if (slot == 0) k.template finish<NV>(r, pt * NV, acc);
     }
   } else {
     gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
@@ -193,7 +203,10 @@ __global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __rest
     if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
     __syncthreads();
   }
-  if (s0 == 0) k.finish(f.row, col, sh[threadIdx.x]);
+  if (s0 == 0) {
+    const T a[1] = {sh[threadIdx.x]};
+    k.template finish<1>(f.row, col, a);
+  }
 }
 
 inline unsigned engine_blocks(int64_t waves) { return (unsigned)((waves * kWave + kBlock - 1) / kBlock); }

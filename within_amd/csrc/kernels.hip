@@ -360,46 +360,92 @@ void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* be
   k_cg_beta<<<1, 64, 0, st>>>(s, rznew, beta, mask, n_active, itn, kp);
 }
 
-// ===========================================================================
-// additive Schwarz: z = sum_i R_i' D_i A_i^+ D_i R_i r over all subdomains at
-// once (schwarz-precond/src/local_solve.rs:182-248 per subdomain;
-// operator/local_solver.rs:360-449 block elimination)
-// ===========================================================================
-// B[l] = scale_l * R[gidx_l]
+// Elementwise kernels over kp-wide rows: each lane owns NV consecutive
+// values (a 16-byte vector when the row is at least 16 bytes), V = kp / NV
+// lanes per row.
 template <typename T>
+static inline int nv_for(int kp) {
+  const int full = 16 / (int)sizeof(T);
+  return kp >= full ? full : kp;
+}
+#define WA_NV_SWITCH(T, kp, CALL)                   \
+  do {                                              \
+    const int nv_ = nv_for<T>(kp);                  \
+    if (nv_ == 16 / (int)sizeof(T)) {               \
+      constexpr int NV = 16 / sizeof(T);            \
+      CALL;                                         \
+    } else if (nv_ == 2) {                          \
+      constexpr int NV = 2;                         \
+      CALL;                                         \
+    } else {                                        \
+      constexpr int NV = 1;                         \
+      CALL;                                         \
+    }                                               \
+  } while (0)
+
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_restrict(const uint32_t* __restrict__ gidx,
                                                      const T* __restrict__ scale, const T* __restrict__ R,
                                                      T* __restrict__ B, int64_t b0, int64_t n, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  for (int64_t l = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; l < n; l += (int64_t)gridDim.x * gpb)
-    B[l * kp + lane] = scale[l] * R[(int64_t)gidx[l] * kp + lane];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  for (int64_t l = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / V; l < n; l += (int64_t)gridDim.x * gpb) {
+    const T sc = scale[l];
+    VecT<T, NV> x = vload<T, NV>(R + (int64_t)gidx[l] * kp + pt * NV);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) x.v[c] = sc * x.v[c];
+    vstore<T, NV>(B + l * kp + pt * NV, x);
+  }
+}
+template <typename T>
+static void launch_restrict(hipStream_t st, const uint32_t* gidx, const T* scale, const T* R, T* B, int64_t b0,
+                            int64_t n, int kp) {
+  if (n <= b0) return;
+  WA_NV_SWITCH(T, kp, (k_restrict<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(gidx, scale, R, B,
+                                                                                                   b0, n, kp)));
 }
 
 // Rr[j] -= sum2[s] / n_red(s)
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_sub_mean(int64_t b0, int64_t n, const uint32_t* __restrict__ sub,
                                                      const uint32_t* __restrict__ ptr, const T* __restrict__ sums,
-                                                     T* __restrict__ V, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  for (int64_t j = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; j < n; j += (int64_t)gridDim.x * gpb) {
+                                                     T* __restrict__ V_, int kp) {
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  for (int64_t j = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / V; j < n; j += (int64_t)gridDim.x * gpb) {
     const uint32_t s = sub[j];
-    V[j * kp + lane] -= sums[(int64_t)s * kp + lane] / (T)(ptr[s + 1] - ptr[s]);
+    const T cnt = (T)(ptr[s + 1] - ptr[s]);
+    const VecT<T, NV> m = vload<T, NV>(sums + (int64_t)s * kp + pt * NV);
+    VecT<T, NV> x = vload<T, NV>(V_ + j * kp + pt * NV);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) x.v[c] -= m.v[c] / cnt;
+    vstore<T, NV>(V_ + j * kp + pt * NV, x);
   }
 }
-
 template <typename T>
+static void launch_sub_mean(hipStream_t st, int64_t b0, int64_t n, const uint32_t* sub, const uint32_t* ptr,
+                            const T* sums, T* V, int kp) {
+  if (n <= b0) return;
+  WA_NV_SWITCH(T, kp, (k_sub_mean<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, sub, ptr,
+                                                                                                   sums, V, kp)));
+}
+
+// dst[idx[i]] = src[i]
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_scatter_rows(int64_t b0, int64_t n, const uint32_t* __restrict__ idx,
                                                          const T* __restrict__ src, T* __restrict__ dst, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  for (int64_t i = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * gpb)
-    dst[(int64_t)idx[i] * kp + lane] = src[i * kp + lane];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  for (int64_t i = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < n; i += (int64_t)gridDim.x * gpb)
+    vstore<T, NV>(dst + (int64_t)idx[i] * kp + pt * NV, vload<T, NV>(src + i * kp + pt * NV));
+}
+template <typename T>
+static void launch_scatter_rows(hipStream_t st, int64_t b0, int64_t n, const uint32_t* idx, const T* src, T* dst,
+                                int kp) {
+  if (n <= b0) return;
+  WA_NV_SWITCH(T, kp, (k_scatter_rows<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, idx, src,
+                                                                                                       dst, kp)));
 }
 
 // Z[d] = sum_{l in dof d} scale_l (S[l] - mu3(sub(l)))   (0 for uncovered DOFs)
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_combine(int64_t m, const uint32_t* __restrict__ dptr,
                                                     const uint32_t* __restrict__ dloc,
                                                     const T* __restrict__ scale,
@@ -407,18 +453,30 @@ __global__ void __launch_bounds__(kBlock) k_combine(int64_t m, const uint32_t* _
                                                     const uint32_t* __restrict__ loc_ptr,
                                                     const T* __restrict__ S, const T* __restrict__ sum3,
                                                     T* __restrict__ Z, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  for (int64_t d = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; d < m; d += (int64_t)gridDim.x * gpb) {
-    T acc = 0;
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  for (int64_t d = (int64_t)blockIdx.x * gpb + threadIdx.x / V; d < m; d += (int64_t)gridDim.x * gpb) {
+    T acc[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] = 0;
     for (uint32_t p = dptr[d]; p < dptr[d + 1]; ++p) {
       const uint32_t l = dloc[p];
       const uint32_t s = loc_sub[l];
-      const T mu = sum3[(int64_t)s * kp + lane] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
-      acc += scale[l] * (S[(int64_t)l * kp + lane] - mu);
+      const T cnt = (T)(loc_ptr[s + 1] - loc_ptr[s]), sc = scale[l];
+      const VecT<T, NV> mu = vload<T, NV>(sum3 + (int64_t)s * kp + pt * NV);
+      const VecT<T, NV> x = vload<T, NV>(S + (int64_t)l * kp + pt * NV);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) acc[c] += sc * (x.v[c] - mu.v[c] / cnt);
     }
-    Z[d * kp + lane] = acc;
+    vstore<T, NV>(Z + d * kp + pt * NV, acc);
   }
+}
+template <typename T>
+static void launch_combine(hipStream_t st, int64_t m, const uint32_t* dptr, const uint32_t* dloc, const T* scale,
+                           const uint32_t* loc_sub, const uint32_t* loc_ptr, const T* S, const T* sum3, T* Z,
+                           int kp) {
+  if (m <= 0) return;
+  WA_NV_SWITCH(T, kp, (k_combine<T, NV><<<grid_stride_blocks(m, kp / NV), kBlock, 0, st>>>(
+                          m, dptr, dloc, scale, loc_sub, loc_ptr, S, sum3, Z, kp)));
 }
 
 // ===========================================================================
@@ -446,9 +504,15 @@ struct OpKind {
     for (int t = 0; t < QM; ++t) id[t] = other[(int64_t)e * QM + t];
     if (W) c = ew[e];
   }
-  __device__ void finish(uint32_t r, int col, T acc) const {
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
     const int64_t o = (int64_t)(off + r) * kp + col;
-    Y[o] = diag[r] * X[o] + acc;
+    const T d = diag[r];
+    const VecT<T, NV> x = vload<T, NV>(X + o);
+    VecT<T, NV> y;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) y.v[c] = d * x.v[c] + acc[c];
+    vstore<T, NV>(Y + o, y);
   }
 };
 
@@ -495,7 +559,10 @@ struct ObsKind {
     id[0] = i;
     if (W) c = w[i];
   }
-  __device__ void finish(uint32_t r, int col, double acc) const { out[(int64_t)(off + r) * kp + col] = acc; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const double (&acc)[NV]) const {
+    vstore<double, NV>(out + (int64_t)(off + r) * kp + col, acc);
+  }
 };
 
 void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* w, const double* V, double* out,
@@ -526,7 +593,10 @@ struct SpmmKind {
     id[0] = idx[e];
     c = val[e];
   }
-  __device__ void finish(uint32_t r, int col, T acc) const { Y[(int64_t)r * kp + col] = acc; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+    vstore<T, NV>(Y + (int64_t)r * kp + col, acc);
+  }
 };
 
 template <typename T>
@@ -549,7 +619,10 @@ struct SegKind {
   __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
   __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
   __device__ void load(uint32_t, uint32_t e, uint32_t* id, T&) const { id[0] = e; }
-  __device__ void finish(uint32_t r, int col, T acc) const { out[(int64_t)r * kp + col] = acc; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+    vstore<T, NV>(out + (int64_t)r * kp + col, acc);
+  }
 };
 
 // Schur right-hand side rows (local_solver.rs:375-389):
@@ -571,10 +644,19 @@ struct KeepKind {
     id[0] = idx[e];
     c = val[e];
   }
-  __device__ void finish(uint32_t j, int col, T acc) const {
+  template <int NV>
+  __device__ void finish(uint32_t j, int col, const T (&acc)[NV]) const {
     const uint32_t s = red_sub[j];
-    const T mu = sum1[(int64_t)s * kp + col] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
-    Rr[(int64_t)j * kp + col] = X[(int64_t)red_loc[j] * kp + col] - mu + acc - mu * rowsum[j];
+    const T cnt = (T)(loc_ptr[s + 1] - loc_ptr[s]), rs = rowsum[j];
+    const VecT<T, NV> m = vload<T, NV>(sum1 + (int64_t)s * kp + col);
+    const VecT<T, NV> x = vload<T, NV>(X + (int64_t)red_loc[j] * kp + col);
+    VecT<T, NV> o;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const T mu = m.v[c] / cnt;
+      o.v[c] = x.v[c] - mu + acc[c] - mu * rs;
+    }
+    vstore<T, NV>(Rr + (int64_t)j * kp + col, o);
   }
 };
 
@@ -597,12 +679,20 @@ struct ElimKind {
     id[0] = idx[e];
     c = val[e];
   }
-  __device__ void finish(uint32_t i, int col, T acc) const {
+  template <int NV>
+  __device__ void finish(uint32_t i, int col, const T (&acc)[NV]) const {
     const uint32_t l = eloc[i];
     const uint32_t s = loc_sub[l];
-    const T mu = sum1[(int64_t)s * kp + col] / (T)(loc_ptr[s + 1] - loc_ptr[s]);
-    const T iv = inv[i];
-    S[(int64_t)l * kp + col] = (B[(int64_t)l * kp + col] - mu) * iv + iv * acc;
+    const T cnt = (T)(loc_ptr[s + 1] - loc_ptr[s]), iv = inv[i];
+    const VecT<T, NV> m = vload<T, NV>(sum1 + (int64_t)s * kp + col);
+    const VecT<T, NV> b = vload<T, NV>(B + (int64_t)l * kp + col);
+    VecT<T, NV> o;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const T mu = m.v[c] / cnt;
+      o.v[c] = (b.v[c] - mu) * iv + iv * acc[c];
+    }
+    vstore<T, NV>(S + (int64_t)l * kp + col, o);
   }
 };
 
@@ -621,7 +711,14 @@ struct FwdKind {
     id[0] = idx[e];
     c = val[e];
   }
-  __device__ void finish(uint32_t r, int col, T acc) const { X[(int64_t)rows[r] * kp + col] += acc; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+    T* p = X + (int64_t)rows[r] * kp + col;
+    VecT<T, NV> x = vload<T, NV>(p);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) x.v[c] += acc[c];
+    vstore<T, NV>(p, x);
+  }
 };
 
 // backward level: x_v = z_v / d_v + sum_{u later} (w_vu / d_v) x_u  (in place)
@@ -639,9 +736,15 @@ struct BwdKind {
     id[0] = idx[e];
     c = val[e];
   }
-  __device__ void finish(uint32_t r, int col, T acc) const {
-    const int64_t o = (int64_t)rows[r] * kp + col;
-    X[o] = inv_piv[rows[r]] * X[o] + acc;
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+    const uint32_t v = rows[r];
+    const T ip = inv_piv[v];
+    T* p = X + (int64_t)v * kp + col;
+    VecT<T, NV> x = vload<T, NV>(p);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) x.v[c] = ip * x.v[c] + acc[c];
+    vstore<T, NV>(p, x);
   }
 };
 
@@ -665,7 +768,10 @@ struct CoreKind {
     id[0] = cidx[b + j];
     c = mat[moff[cc] + (uint64_t)(r - b) * n + j];
   }
-  __device__ void finish(uint32_t r, int col, T acc) const { out[(int64_t)r * kp + col] = acc; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+    vstore<T, NV>(out + (int64_t)r * kp + col, acc);
+  }
 };
 
 // ---- dense cores ------------------------------------------------------------
@@ -840,7 +946,7 @@ template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m) {
   // 1. restrict (signed, partition-weighted) and per-subdomain sums -> mu1
-  k_restrict<T><<<grid_stride_blocks(p.n_loc, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, R, w.B, 0, p.n_loc, kp);
+  launch_restrict<T>(st, p.loc_gidx, p.loc_scale, R, w.B, 0, p.n_loc, kp);
   run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, p.s_loc, w.part, kp);
   if (p.n_red > 0) {
     // 2. Schur right-hand side, centred (mu2)
@@ -849,8 +955,7 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                          p.red_sub, p.loc_ptr, w.sum1, kp},
                              p.s_keep, w.part, kp);
     run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, p.s_red, w.part, kp);
-    k_sub_mean<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(0, p.n_red, p.red_sub, p.red_ptr, w.sum2,
-                                                                      w.Rr, kp);
+    launch_sub_mean<T>(st, 0, p.n_red, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
     FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
     for (const Segment& g : p.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, p.s_fwd, g, w.part, kp);
@@ -860,15 +965,14 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                                            p.core_mat_off, p.core_mat, kp},
                                p.s_core, w.part, kp);
       for (const DenseCore<T>& d : p.big) run_dense_core<T>(st, d, p.core_idx, w.Rr, w.C, w.Zc, w.P, kp);
-      k_scatter_rows<T><<<grid_stride_blocks(p.n_core_rows, kp), kBlock, 0, st>>>(0, p.n_core_rows, p.core_idx,
-                                                                                  w.C, w.Rr, kp);
+      launch_scatter_rows<T>(st, 0, p.n_core_rows, p.core_idx, w.C, w.Rr, kp);
     }
     BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
     // the last level gathers the dense cores: no backward step
     for (int64_t t = (int64_t)p.s_bwd.segs.size() - 2; t >= 0; --t)
       run_seg<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.segs[t], w.part, kp);
     // 4. kept block copied into the local solution
-    k_scatter_rows<T><<<grid_stride_blocks(p.n_red, kp), kBlock, 0, st>>>(0, p.n_red, p.red_loc, w.Rr, w.S, kp);
+    launch_scatter_rows<T>(st, 0, p.n_red, p.red_loc, w.Rr, w.S, kp);
   }
   // 5. back-substitution for the eliminated block
   if (p.n_elim > 0)
@@ -878,8 +982,7 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
                              p.s_elim, w.part, kp);
   // 6. centre (mu3), sign and partition weights, combine per DOF
   run_rows<SegKind<T>, T>(st, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, p.s_loc, w.part, kp);
-  k_combine<T><<<grid_stride_blocks(m, kp), kBlock, 0, st>>>(m, p.dof_ptr, p.dof_loc, p.loc_scale, p.loc_sub,
-                                                             p.loc_ptr, w.S, w.sum3, Z, kp);
+  launch_combine<T>(st, m, p.dof_ptr, p.dof_loc, p.loc_scale, p.loc_sub, p.loc_ptr, w.S, w.sum3, Z, kp);
 }
 template void launch_precond<float>(hipStream_t, const DevPrecond<float>&, const float*, float*,
                                     PrecondWork<float>&, int, int64_t);
@@ -982,8 +1085,7 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
   (void)hipMemsetAsync(Z, 0, vbytes, st);
   for (const PairSched& ps : pairs) {
     const int64_t nl = ps.loc1 - ps.loc0;
-    k_restrict<T><<<grid_stride_blocks(nl, kp), kBlock, 0, st>>>(p.loc_gidx, p.loc_scale, rwork, w.B, ps.loc0,
-                                                                 ps.loc1, kp);
+    launch_restrict<T>(st, p.loc_gidx, p.loc_scale, rwork, w.B, ps.loc0, ps.loc1, kp);
     run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, ps.s_loc, w.part, kp);
     const int64_t nr = ps.red1 - ps.red0;
     if (nr > 0) {
@@ -992,8 +1094,7 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
                                            p.red_sub, p.loc_ptr, w.sum1, kp},
                                ps.s_keep, w.part, kp);
       run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, w.part, kp);
-      k_sub_mean<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2,
-                                                                   w.Rr, kp);
+      launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
       FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
       for (const Segment& sg : ps.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, ps.s_fwd, sg, w.part, kp);
       if (ps.core1 > ps.core0) {
@@ -1002,13 +1103,12 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
                                              p.core_mat_off, p.core_mat, kp},
                                  ps.s_core, w.part, kp);
         for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, w.Zc, w.P, kp);
-        k_scatter_rows<T><<<grid_stride_blocks(ps.core1 - ps.core0, kp), kBlock, 0, st>>>(
-            ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
+        launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
       }
       BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
       for (int64_t t = (int64_t)ps.s_bwd.segs.size() - 2; t >= 0; --t)
         run_seg<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.s_bwd.segs[t], w.part, kp);
-      k_scatter_rows<T><<<grid_stride_blocks(nr, kp), kBlock, 0, st>>>(ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
+      launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
     }
     run_rows<ElimKind<T>, T>(st,
                              ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
