@@ -160,8 +160,9 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
     if (r >= (int64_t)r1) return;
     const uint32_t e0 = k.rbegin((uint32_t)r), e1 = k.rend((uint32_t)r);
     if (e1 - e0 > short_max) return;
+    const auto q = k.template pre<NV>((uint32_t)r, pt * NV);  // finish inputs, loaded before the gathers
     gather_row<K, T, KP, NV>(k, (uint32_t)r, e0, e1, pt, acc);
-    k.template finish<NV>((uint32_t)r, pt * NV, acc);
+    k.template finish<NV>((uint32_t)r, pt * NV, acc, q);
     return;
   }
   g -= n_sw;
@@ -169,8 +170,9 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
   const Item it = items[g];
   if (it.slot < 0) {
     for (uint32_t r = it.row0; r < it.row1; ++r) {
+      const auto q = k.template pre<NV>(r, pt * NV);
       gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
-      if (slot == 0) k.template finish<NV>(r, pt * NV, acc);
+      if (slot == 0) k.template finish<NV>(r, pt * NV, acc, q);
     }
   } else {
     gather_vec<K, T, V, NV>(k, it.row0, it.e0, it.e1, slot, pt, acc);
@@ -205,7 +207,7 @@ __global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __rest
   }
   if (s0 == 0) {
     const T a[1] = {sh[threadIdx.x]};
-    k.template finish<1>(f.row, col, a);
+    k.template finish<1>(f.row, col, a, k.template pre<1>(f.row, col));
   }
 }
 

@@ -115,24 +115,57 @@ __device__ inline double red_term(const T* A, const T* B, int64_t idx) {
   return (double)A[idx] * (double)A[idx];
 }
 
-// part[b * kp + c] = sum over this block's rows of A[:,c] * B[:,c]
+// Lanes own NV consecutive values of a kp-wide row (16-byte vectors when the
+// row is at least 16 bytes); V = kp / NV lanes per row.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_dot_part(const T* __restrict__ A, const T* __restrict__ B,
-                                                     int64_t m, int kp, double* __restrict__ part) {
-  __shared__ double sh[kBlock];
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  double acc = 0.0;
-#pragma unroll 4
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb)
-    acc += (double)A[i * kp + lane] * (double)B[i * kp + lane];
-  sh[threadIdx.x] = acc;
+static inline int vnv(int kp) {
+  const int full = 16 / (int)sizeof(T);
+  return kp >= full ? full : kp;
+}
+#define WA_VEC_SWITCH(T, kp, CALL)       \
+  do {                                   \
+    const int nv_ = vnv<T>(kp);          \
+    if (nv_ == 16 / (int)sizeof(T)) {    \
+      constexpr int NV = 16 / sizeof(T); \
+      CALL;                              \
+    } else if (nv_ == 2) {               \
+      constexpr int NV = 2;              \
+      CALL;                              \
+    } else {                             \
+      constexpr int NV = 1;              \
+      CALL;                              \
+    }                                    \
+  } while (0)
+
+// per-block column sums of a thread's NV accumulators (sh: kBlock * NV)
+template <int NV>
+__device__ inline void block_col_sums(double (&acc)[NV], double* sh, int kp, int gpb, double* out) {
+#pragma unroll
+  for (int c = 0; c < NV; ++c) sh[threadIdx.x * NV + c] = acc[c];
   __syncthreads();
   if (threadIdx.x < kp) {
     double s = 0.0;
     for (int g = 0; g < gpb; ++g) s += sh[g * kp + threadIdx.x];
-    part[(int64_t)blockIdx.x * kp + threadIdx.x] = s;
+    out[(int64_t)blockIdx.x * kp + threadIdx.x] = s;
   }
+}
+
+// part[b * kp + c] = sum over this block's rows of A[:,c] * B[:,c]
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock) k_dot_part(const T* __restrict__ A, const T* __restrict__ B,
+                                                     int64_t m, int kp, double* __restrict__ part) {
+  __shared__ double sh[kBlock * NV];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+#pragma unroll 2
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < m; i += (int64_t)gridDim.x * gpb) {
+    const VecT<T, NV> a = vload<T, NV>(A + i * kp + pt * NV), b = vload<T, NV>(B + i * kp + pt * NV);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] += (double)a.v[c] * (double)b.v[c];
+  }
+  block_col_sums<NV>(acc, sh, kp, gpb, part);
 }
 
 // out[c] = sum_b part[b * kp + c]: threads stride over the blocks, then a
@@ -155,45 +188,49 @@ __global__ void __launch_bounds__(kBlock) k_dot_fin(const double* __restrict__ p
 template <typename T>
 void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out) {
   unsigned nb = red_blocks(m, kp);
-  k_dot_part<T><<<nb, kBlock, 0, st>>>(A, B, m, kp, part);
+  WA_VEC_SWITCH(T, kp, (k_dot_part<T, NV><<<nb, kBlock, 0, st>>>(A, B, m, kp, part)));
   k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
 }
 template void launch_dot<float>(hipStream_t, const float*, const float*, int64_t, int, double*, double*);
 template void launch_dot<double>(hipStream_t, const double*, const double*, int64_t, int, double*, double*);
 
 // X += alpha P ; R -= alpha AP ; part = sum R^2   (alpha per column, FP64)
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T* __restrict__ P,
                                                       T* __restrict__ R, const T* __restrict__ AP,
                                                       const double* __restrict__ alpha, int64_t m, int kp,
                                                       double* __restrict__ part) {
-  __shared__ double sh[kBlock];
-  const int lane = threadIdx.x & (kp - 1);
-  const int gpb = kBlock / kp;
-  const T a = (T)alpha[lane];
-  double acc = 0.0;
-#pragma unroll 4
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
-    const int64_t id = i * kp + lane;
-    X[id] += a * P[id];
-    const T r = R[id] - a * AP[id];
-    R[id] = r;
-    acc += (double)r * (double)r;
+  __shared__ double sh[kBlock * NV];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  T a[NV];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    a[c] = (T)alpha[pt * NV + c];
+    acc[c] = 0.0;
   }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  if (threadIdx.x < kp) {
-    double s = 0.0;
-    for (int g = 0; g < gpb; ++g) s += sh[g * kp + threadIdx.x];
-    part[(int64_t)blockIdx.x * kp + threadIdx.x] = s;
+#pragma unroll 2
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + pt * NV;
+    VecT<T, NV> x = vload<T, NV>(X + id), r = vload<T, NV>(R + id);
+    const VecT<T, NV> p = vload<T, NV>(P + id), ap = vload<T, NV>(AP + id);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      x.v[c] += a[c] * p.v[c];
+      r.v[c] = r.v[c] - a[c] * ap.v[c];
+      acc[c] += (double)r.v[c] * (double)r.v[c];
+    }
+    vstore<T, NV>(X + id, x);
+    vstore<T, NV>(R + id, r);
   }
+  block_col_sums<NV>(acc, sh, kp, gpb, part);
 }
 
 template <typename T>
 void launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m,
                       int kp, double* part, double* out) {
   unsigned nb = red_blocks(m, kp);
-  k_cg_update<T><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part);
+  WA_VEC_SWITCH(T, kp, (k_cg_update<T, NV><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part)));
   k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
 }
 template void launch_cg_update<float>(hipStream_t, float*, const float*, float*, const float*, const double*,
@@ -202,22 +239,33 @@ template void launch_cg_update<double>(hipStream_t, double*, const double*, doub
                                        const double*, int64_t, int, double*, double*);
 
 // P = Z + beta P (beta per column; columns with mask 0 untouched)
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_xpby(T* __restrict__ P, const T* __restrict__ Z,
                                                  const double* __restrict__ beta, const int* __restrict__ mask,
                                                  int64_t m, int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  if (!mask[lane]) return;
-  const T b = (T)beta[lane];
-  const int gpb = kBlock / kp;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
-    const int64_t id = i * kp + lane;
-    P[id] = Z[id] + b * P[id];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  T b[NV];
+  bool on[NV], any = false;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    b[c] = (T)beta[pt * NV + c];
+    on[c] = mask[pt * NV + c] != 0;
+    any = any || on[c];
+  }
+  if (!any) return;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + pt * NV;
+    VecT<T, NV> p = vload<T, NV>(P + id);
+    const VecT<T, NV> z = vload<T, NV>(Z + id);
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
+      if (on[c]) p.v[c] = z.v[c] + b[c] * p.v[c];
+    vstore<T, NV>(P + id, p);
   }
 }
 template <typename T>
 void launch_xpby(hipStream_t st, T* P, const T* Z, const double* beta, const int* mask, int64_t m, int kp) {
-  k_xpby<T><<<red_blocks(m, kp), kBlock, 0, st>>>(P, Z, beta, mask, m, kp);
+  WA_VEC_SWITCH(T, kp, (k_xpby<T, NV><<<red_blocks(m, kp), kBlock, 0, st>>>(P, Z, beta, mask, m, kp)));
 }
 template void launch_xpby<float>(hipStream_t, float*, const float*, const double*, const int*, int64_t, int);
 template void launch_xpby<double>(hipStream_t, double*, const double*, const double*, const int*, int64_t, int);
@@ -360,28 +408,7 @@ void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* be
   k_cg_beta<<<1, 64, 0, st>>>(s, rznew, beta, mask, n_active, itn, kp);
 }
 
-// Elementwise kernels over kp-wide rows: each lane owns NV consecutive
-// values (a 16-byte vector when the row is at least 16 bytes), V = kp / NV
-// lanes per row.
-template <typename T>
-static inline int nv_for(int kp) {
-  const int full = 16 / (int)sizeof(T);
-  return kp >= full ? full : kp;
-}
-#define WA_NV_SWITCH(T, kp, CALL)                   \
-  do {                                              \
-    const int nv_ = nv_for<T>(kp);                  \
-    if (nv_ == 16 / (int)sizeof(T)) {               \
-      constexpr int NV = 16 / sizeof(T);            \
-      CALL;                                         \
-    } else if (nv_ == 2) {                          \
-      constexpr int NV = 2;                         \
-      CALL;                                         \
-    } else {                                        \
-      constexpr int NV = 1;                         \
-      CALL;                                         \
-    }                                               \
-  } while (0)
+// Preconditioner elementwise kernels (same lane mapping as the vector ops).
 
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_restrict(const uint32_t* __restrict__ gidx,
@@ -400,7 +427,7 @@ template <typename T>
 static void launch_restrict(hipStream_t st, const uint32_t* gidx, const T* scale, const T* R, T* B, int64_t b0,
                             int64_t n, int kp) {
   if (n <= b0) return;
-  WA_NV_SWITCH(T, kp, (k_restrict<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(gidx, scale, R, B,
+  WA_VEC_SWITCH(T, kp, (k_restrict<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(gidx, scale, R, B,
                                                                                                    b0, n, kp)));
 }
 
@@ -424,7 +451,7 @@ template <typename T>
 static void launch_sub_mean(hipStream_t st, int64_t b0, int64_t n, const uint32_t* sub, const uint32_t* ptr,
                             const T* sums, T* V, int kp) {
   if (n <= b0) return;
-  WA_NV_SWITCH(T, kp, (k_sub_mean<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, sub, ptr,
+  WA_VEC_SWITCH(T, kp, (k_sub_mean<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, sub, ptr,
                                                                                                    sums, V, kp)));
 }
 
@@ -440,7 +467,7 @@ template <typename T>
 static void launch_scatter_rows(hipStream_t st, int64_t b0, int64_t n, const uint32_t* idx, const T* src, T* dst,
                                 int kp) {
   if (n <= b0) return;
-  WA_NV_SWITCH(T, kp, (k_scatter_rows<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, idx, src,
+  WA_VEC_SWITCH(T, kp, (k_scatter_rows<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(b0, n, idx, src,
                                                                                                        dst, kp)));
 }
 
@@ -475,7 +502,7 @@ static void launch_combine(hipStream_t st, int64_t m, const uint32_t* dptr, cons
                            const uint32_t* loc_sub, const uint32_t* loc_ptr, const T* S, const T* sum3, T* Z,
                            int kp) {
   if (m <= 0) return;
-  WA_NV_SWITCH(T, kp, (k_combine<T, NV><<<grid_stride_blocks(m, kp / NV), kBlock, 0, st>>>(
+  WA_VEC_SWITCH(T, kp, (k_combine<T, NV><<<grid_stride_blocks(m, kp / NV), kBlock, 0, st>>>(
                           m, dptr, dloc, scale, loc_sub, loc_ptr, S, sum3, Z, kp)));
 }
 
@@ -505,14 +532,20 @@ struct OpKind {
     if (W) c = ew[e];
   }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
-    const int64_t o = (int64_t)(off + r) * kp + col;
-    const T d = diag[r];
-    const VecT<T, NV> x = vload<T, NV>(X + o);
+  struct Pre {
+    T d;
+    VecT<T, NV> x;
+  };
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t r, int col) const {
+    return {diag[r], vload<T, NV>(X + (int64_t)(off + r) * kp + col)};
+  }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>& q) const {
     VecT<T, NV> y;
 #pragma unroll
-    for (int c = 0; c < NV; ++c) y.v[c] = d * x.v[c] + acc[c];
-    vstore<T, NV>(Y + o, y);
+    for (int c = 0; c < NV; ++c) y.v[c] = q.d * q.x.v[c] + acc[c];
+    vstore<T, NV>(Y + (int64_t)(off + r) * kp + col, y);
   }
 };
 
@@ -560,7 +593,11 @@ struct ObsKind {
     if (W) c = w[i];
   }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const double (&acc)[NV]) const {
+  struct Pre {};
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const double (&acc)[NV], const Pre<NV>&) const {
     vstore<double, NV>(out + (int64_t)(off + r) * kp + col, acc);
   }
 };
@@ -594,7 +631,11 @@ struct SpmmKind {
     c = val[e];
   }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+  struct Pre {};
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>&) const {
     vstore<T, NV>(Y + (int64_t)r * kp + col, acc);
   }
 };
@@ -620,7 +661,11 @@ struct SegKind {
   __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
   __device__ void load(uint32_t, uint32_t e, uint32_t* id, T&) const { id[0] = e; }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+  struct Pre {};
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>&) const {
     vstore<T, NV>(out + (int64_t)r * kp + col, acc);
   }
 };
@@ -645,16 +690,23 @@ struct KeepKind {
     c = val[e];
   }
   template <int NV>
-  __device__ void finish(uint32_t j, int col, const T (&acc)[NV]) const {
+  struct Pre {
+    T cnt, rs;
+    VecT<T, NV> m, x;
+  };
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t j, int col) const {
     const uint32_t s = red_sub[j];
-    const T cnt = (T)(loc_ptr[s + 1] - loc_ptr[s]), rs = rowsum[j];
-    const VecT<T, NV> m = vload<T, NV>(sum1 + (int64_t)s * kp + col);
-    const VecT<T, NV> x = vload<T, NV>(X + (int64_t)red_loc[j] * kp + col);
+    return {(T)(loc_ptr[s + 1] - loc_ptr[s]), rowsum[j], vload<T, NV>(sum1 + (int64_t)s * kp + col),
+            vload<T, NV>(X + (int64_t)red_loc[j] * kp + col)};
+  }
+  template <int NV>
+  __device__ void finish(uint32_t j, int col, const T (&acc)[NV], const Pre<NV>& q) const {
     VecT<T, NV> o;
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      const T mu = m.v[c] / cnt;
-      o.v[c] = x.v[c] - mu + acc[c] - mu * rs;
+      const T mu = q.m.v[c] / q.cnt;
+      o.v[c] = q.x.v[c] - mu + acc[c] - mu * q.rs;
     }
     vstore<T, NV>(Rr + (int64_t)j * kp + col, o);
   }
@@ -680,17 +732,26 @@ struct ElimKind {
     c = val[e];
   }
   template <int NV>
-  __device__ void finish(uint32_t i, int col, const T (&acc)[NV]) const {
+  struct Pre {
+    uint32_t l;
+    T cnt, iv;
+    VecT<T, NV> m, b;
+  };
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t i, int col) const {
     const uint32_t l = eloc[i];
     const uint32_t s = loc_sub[l];
-    const T cnt = (T)(loc_ptr[s + 1] - loc_ptr[s]), iv = inv[i];
-    const VecT<T, NV> m = vload<T, NV>(sum1 + (int64_t)s * kp + col);
-    const VecT<T, NV> b = vload<T, NV>(B + (int64_t)l * kp + col);
+    return {l, (T)(loc_ptr[s + 1] - loc_ptr[s]), inv[i], vload<T, NV>(sum1 + (int64_t)s * kp + col),
+            vload<T, NV>(B + (int64_t)l * kp + col)};
+  }
+  template <int NV>
+  __device__ void finish(uint32_t, int col, const T (&acc)[NV], const Pre<NV>& q) const {
+    const uint32_t l = q.l;
     VecT<T, NV> o;
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      const T mu = m.v[c] / cnt;
-      o.v[c] = (b.v[c] - mu) * iv + iv * acc[c];
+      const T mu = q.m.v[c] / q.cnt;
+      o.v[c] = (q.b.v[c] - mu) * q.iv + q.iv * acc[c];
     }
     vstore<T, NV>(S + (int64_t)l * kp + col, o);
   }
@@ -711,13 +772,24 @@ struct FwdKind {
     id[0] = idx[e];
     c = val[e];
   }
+  // a level's rows are not gathered by the same level: X[rows[r]] can be
+  // read before the gathers
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+  struct Pre {
+    T* p;
+    VecT<T, NV> x;
+  };
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t r, int col) const {
     T* p = X + (int64_t)rows[r] * kp + col;
-    VecT<T, NV> x = vload<T, NV>(p);
+    return {p, vload<T, NV>(p)};
+  }
+  template <int NV>
+  __device__ void finish(uint32_t, int, const T (&acc)[NV], const Pre<NV>& q) const {
+    VecT<T, NV> x = q.x;
 #pragma unroll
     for (int c = 0; c < NV; ++c) x.v[c] += acc[c];
-    vstore<T, NV>(p, x);
+    vstore<T, NV>(q.p, x);
   }
 };
 
@@ -737,14 +809,23 @@ struct BwdKind {
     c = val[e];
   }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+  struct Pre {
+    T* p;
+    T ip;
+    VecT<T, NV> x;
+  };
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t r, int col) const {
     const uint32_t v = rows[r];
-    const T ip = inv_piv[v];
     T* p = X + (int64_t)v * kp + col;
-    VecT<T, NV> x = vload<T, NV>(p);
+    return {p, inv_piv[v], vload<T, NV>(p)};
+  }
+  template <int NV>
+  __device__ void finish(uint32_t, int, const T (&acc)[NV], const Pre<NV>& q) const {
+    VecT<T, NV> x;
 #pragma unroll
-    for (int c = 0; c < NV; ++c) x.v[c] = ip * x.v[c] + acc[c];
-    vstore<T, NV>(p, x);
+    for (int c = 0; c < NV; ++c) x.v[c] = q.ip * q.x.v[c] + acc[c];
+    vstore<T, NV>(q.p, x);
   }
 };
 
@@ -769,7 +850,11 @@ struct CoreKind {
     c = mat[moff[cc] + (uint64_t)(r - b) * n + j];
   }
   template <int NV>
-  __device__ void finish(uint32_t r, int col, const T (&acc)[NV]) const {
+  struct Pre {};
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>&) const {
     vstore<T, NV>(out + (int64_t)r * kp + col, acc);
   }
 };
