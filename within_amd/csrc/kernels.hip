@@ -31,28 +31,6 @@ static inline unsigned grid_stride_blocks(int64_t n, int kp) {
 // observation space (FP64): D'W V and V = Y - D X, row-major n x kp
 // (rmatvec_wdt / matvec_d, domain.rs:390-418)
 // ===========================================================================
-// E[i] = Y[i] - sum_q X[off_q + code_q(i)]   (Y == nullptr: E = D X)
-__global__ void __launch_bounds__(kBlock) k_demean(int64_t n, int Q, const uint32_t* __restrict__ codes,
-                                                   const uint32_t* __restrict__ offs,
-                                                   const double* __restrict__ X,
-                                                   const double* __restrict__ Y, double* __restrict__ E,
-                                                   int kp) {
-  const int lane = threadIdx.x & (kp - 1);
-  const int64_t gpb = kBlock / kp;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * gpb) {
-    double s = 0.0;
-    for (int q = 0; q < Q; ++q) s += X[(int64_t)(offs[q] + codes[(int64_t)q * n + i]) * kp + lane];
-    E[i * kp + lane] = Y ? Y[i * kp + lane] - s : s;
-  }
-}
-
-void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
-                   const double* X, const double* Y, double* E, int kp) {
-  int64_t groups = n;
-  unsigned nb = (unsigned)std::min<int64_t>(blocks_for(groups, kp), 65536);
-  k_demean<<<nb, kBlock, 0, st>>>(n, Q, codes, offs, X, Y, E, kp);
-}
-
 // ===========================================================================
 // layout conversion: column-major (ld) <-> row-major kp blocks, via LDS tiles
 // ===========================================================================
@@ -136,6 +114,45 @@ static inline int vnv(int kp) {
       CALL;                              \
     }                                    \
   } while (0)
+
+// E[i] = Y[i] - sum_q X[off_q + code_q(i)]   (Y == nullptr: E = D X)
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_demean(int64_t n, int Q, const uint32_t* __restrict__ codes,
+                                                   const uint32_t* __restrict__ offs,
+                                                   const double* __restrict__ X,
+                                                   const double* __restrict__ Y, double* __restrict__ E,
+                                                   int kp) {
+  const int V = kp / NV, pt = threadIdx.x % V;
+  const int64_t gpb = kBlock / V;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < n; i += (int64_t)gridDim.x * gpb) {
+    double s[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) s[c] = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      const VecT<double, NV> x =
+          vload<double, NV>(X + (int64_t)(offs[q] + codes[(int64_t)q * n + i]) * kp + pt * NV);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) s[c] += x.v[c];
+    }
+    VecT<double, NV> e;
+    if (Y) {
+      const VecT<double, NV> y = vload<double, NV>(Y + i * kp + pt * NV);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) e.v[c] = y.v[c] - s[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) e.v[c] = s[c];
+    }
+    vstore<double, NV>(E + i * kp + pt * NV, e);
+  }
+}
+
+void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
+                   const double* X, const double* Y, double* E, int kp) {
+  if (n <= 0) return;
+  WA_VEC_SWITCH(double, kp, (k_demean<NV><<<grid_stride_blocks(n, kp / NV), kBlock, 0, st>>>(n, Q, codes, offs,
+                                                                                               X, Y, E, kp)));
+}
 
 // per-block column sums of a thread's NV accumulators (sh: kBlock * NV)
 template <int NV>

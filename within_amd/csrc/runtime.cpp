@@ -837,6 +837,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
   std::vector<int64_t> its(kb, 0), cits(kb, 0);
   const int64_t vbytes = sizeof(double) * m * kp;
   const bool mixed = params.precision == 1;
+  bool fresh = false;  // E and R64 hold y - Dx and D'W(y - Dx) for the current X64
   if (!mixed) {
     std::vector<double> tol(kb, params.tol);
     krylov<double>(kb, kp, B64, cx64, tol, conv, its);
@@ -845,6 +846,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
     for (int64_t it = 0; it < params.max_refinements; ++it) {
       launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
       rhs64(E, R64, kp);
+      fresh = true;
       std::vector<double> cn = col_norms(R64, m, kp);
       std::vector<double> scale(KMAX, 0.0), ctol(kb, 1.0);
       bool anyc = false;
@@ -864,6 +866,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       for (int c = 0; c < kb; ++c) msk[c] = live[c];
       HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
       launch_axpy_acc<double>(st, X64, cx64, nullptr, imask + KMAX + 1, m, kp);
+      fresh = false;
       for (int c = 0; c < kb; ++c)
         if (live[c]) {
           its[c] += cits[c];
@@ -883,6 +886,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       } else {
         launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
         rhs64(E, R64, kp);
+        fresh = true;
         rn = col_norms(R64, m, kp);
       }
       std::vector<double> scale(KMAX, 0.0), itol(kb, 1.0);
@@ -905,13 +909,17 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       HIPCHK(hipMemcpyAsync(s_(9), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
       HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
       launch_axpy_acc<float>(st, X64, cx32, s_(9), imask + KMAX + 1, m, kp);
+      fresh = false;
       for (int c = 0; c < kb; ++c)
         if (live[c]) its[c] += cits[c];
     }
   }
-  // final demeaning and observation-space residual (FP64)
-  launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
-  rhs64(E, R64, kp);
+  // final demeaning and observation-space residual (FP64), unless the last
+  // refinement check already computed them for the final X
+  if (!fresh) {
+    launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+    rhs64(E, R64, kp);
+  }
   std::vector<double> fr = col_norms(R64, m, kp);
   tic(3, ev);
   if (dem) launch_row_to_col(st, E, kp, n, kb, dem + (int64_t)c0 * ldd, ldd);
