@@ -129,12 +129,25 @@ def main():
     ap.add_argument("--operator", choices=["implicit", "explicit"], default="explicit")
     ap.add_argument("--dense-core", type=int, default=0,
                     help="reduced-system size factored densely (0 = library default)")
+    ap.add_argument("--profile", action="store_true",
+                    help="headline solves only (no reference, baseline or CPU legs): for counter runs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB FP32 buffer 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 << 28}), flush=True)
+        return
 
     import within_amd as W
     from within_amd import _native as N
@@ -177,6 +190,10 @@ def main():
         L.wa_timing_get(fast.s._h, c, C.byref(nl), C.byref(ms))
         cls[name] = (nl.value, ms.value)
     log(f"bench: kernel classes {cls}")
+    if args.profile:
+        print(json.dumps({"ms_per_step": round(t_step * 1e3, 3), "kernel_ms": {k_: v[1] for k_, v in cls.items()}}),
+              flush=True)
+        return
     D_fast = D.clone()
 
     # --- FP64 reference solution (tight) for the 1e-8 check ---------------------
@@ -218,6 +235,12 @@ def main():
         op_name = f"implicit operator apply (k={k} FP32, all factor passes)"
     op_ms = ms_op / max(n_op, 1)
     achieved = b_op / (op_ms * 1e-3) / 1e9 if n_op else None
+    # HBM bytes per dispatch of this kernel from the committed rocprofv3 PMC
+    # summary (FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy's readings)
+    traffic = None
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "spmm_fp32_k32_akm20m_traffic.json")
+    if args.operator == "explicit" and args.scale == 1.0 and k == 32 and os.path.exists(tf):
+        traffic = round(json.load(open(tf))["traffic_bytes"])
     out = {
         "metric": "RHS solved/sec to 1e-8 (FP64 oracle), 20M-obs 3-FE k=32 vs plain FP64 PCG",
         "value": round(value, 3),
@@ -241,7 +264,7 @@ def main():
         "kernel_ms": {k_: round(v[1], 3) for k_, v in cls.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": op_name,
+                     "traffic": traffic, "kernel": op_name,
                      "algorithmic_bytes": b_op, "avg_ms": round(op_ms, 4)},
     }
     if not args.no_cpu:

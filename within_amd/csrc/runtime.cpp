@@ -6,6 +6,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -160,6 +162,24 @@ struct wa_solver {
     std::vector<T> c(v.begin(), v.end());
     return upload(c);
   }
+  // schedule shape on stderr when WITHIN_AMD_VERBOSE is set
+  static void sched_report(const char* name, const Schedule& s, const std::vector<uint32_t>& ptr) {
+    if (!std::getenv("WITHIN_AMD_VERBOSE")) return;
+    int64_t rows = 0, shortr = 0, elems = 0, short_e = 0, whole = 0, split = 0;
+    for (const Segment& g : s.segs)
+      for (uint32_t r = g.r0; r < g.r1; ++r) {
+        const int64_t len = ptr[r + 1] - ptr[r];
+        ++rows;
+        elems += len;
+        if (len <= (int64_t)s.short_max) { ++shortr; short_e += len; }
+      }
+    for (const Item& it : s.items) (it.slot < 0 ? whole : split)++;
+    std::fprintf(stderr,
+                 "[within_amd] schedule %-8s segs %6zu rows %9lld (short %9lld, %10lld elems) elems %10lld "
+                 "items %8lld whole + %8lld chunks, fixups %6zu\n",
+                 name, s.segs.size(), (long long)rows, (long long)shortr, (long long)short_e, (long long)elems,
+                 (long long)whole, (long long)split, s.fixups.size());
+  }
   DevSched upload_sched(const Schedule& s) {
     DevSched d;
     d.items = upload(s.items);
@@ -285,7 +305,11 @@ void wa_solver::setup_device() {
     a.off = b.off = (uint32_t)d.offset[fp.q];
     a.levels = b.levels = d.levels[fp.q];
     a.n_elem = b.n_elem = n;
-    a.sched = b.sched = upload_sched(make_schedule(fp.row_ptr, kChunk, kShortPass));
+    {
+      Schedule sc = make_schedule(fp.row_ptr, kChunk, kShortPass);
+      sched_report("pass", sc, fp.row_ptr);
+      a.sched = b.sched = upload_sched(sc);
+    }
     max_slots = std::max(max_slots, a.sched.n_slots);
     p64.push_back(a);
     p32.push_back(b);
@@ -298,7 +322,11 @@ void wa_solver::setup_device() {
     g32.val = upload_as<float>(g.data);
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
-    g64.sched = g32.sched = upload_sched(make_schedule(g.indptr, kChunk, kShortRow));
+    {
+      Schedule sc = make_schedule(g.indptr, kChunk, kShortRow);
+      sched_report("gramian", sc, g.indptr);
+      g64.sched = g32.sched = upload_sched(sc);
+    }
     max_slots = std::max(max_slots, g64.sched.n_slots);
     has_gram = true;
   }
@@ -400,20 +428,26 @@ void wa_solver::setup_device() {
       }
     }
     const uint32_t* d_core_row_ptr = upload(core_row_ptr);
-    auto level_sched = [&](const std::vector<uint32_t>& ptr) {
+    auto level_sched = [&](const char* name, const std::vector<uint32_t>& ptr) {
       Schedule sc;
       sc.short_max = kShortRow;
       for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) append_schedule(sc, ptr, f.lvl_ptr[t], f.lvl_ptr[t + 1], kChunk);
+      sched_report(name, sc, ptr);
       return upload_sched(sc);
     };
-    DevSched s_loc = upload_sched(make_schedule(f.loc_ptr, kChunk, kShortRow));
-    DevSched s_red = upload_sched(make_schedule(f.red_ptr, kChunk, kShortRow));
-    DevSched s_keep = upload_sched(make_schedule(f.keep_ptr, kChunk, kShortRow));
-    DevSched s_elim = upload_sched(make_schedule(f.elim_ptr, kChunk, kShortRow));
-    DevSched s_fwd = level_sched(f.fin_ptr);
-    DevSched s_bwd = level_sched(f.bout_ptr);
+    auto whole_sched = [&](const char* name, const std::vector<uint32_t>& ptr, const std::vector<char>* skip) {
+      Schedule sc = make_schedule(ptr, kChunk, kShortRow, skip);
+      sched_report(name, sc, ptr);
+      return upload_sched(sc);
+    };
+    DevSched s_loc = whole_sched("local", f.loc_ptr, nullptr);
+    DevSched s_red = whole_sched("reduced", f.red_ptr, nullptr);
+    DevSched s_keep = whole_sched("keep", f.keep_ptr, nullptr);
+    DevSched s_elim = whole_sched("elim", f.elim_ptr, nullptr);
+    DevSched s_fwd = level_sched("forward", f.fin_ptr);
+    DevSched s_bwd = level_sched("backward", f.bout_ptr);
     static_assert(kDenseCore > kShortRow, "dense-core rows must not be short rows");
-    DevSched s_core = upload_sched(make_schedule(core_row_ptr, kChunk, kShortRow, &big_row));
+    DevSched s_core = whole_sched("core", core_row_ptr, &big_row);
     for (const DevSched* d : {&s_loc, &s_red, &s_keep, &s_elim, &s_fwd, &s_bwd, &s_core})
       max_slots = std::max(max_slots, d->n_slots);
     auto scheds = [&](auto& P) {
