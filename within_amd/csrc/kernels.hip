@@ -203,10 +203,18 @@ __global__ void __launch_bounds__(kBlock) k_dot_fin(const double* __restrict__ p
 }
 
 template <typename T>
-void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out) {
+int launch_dot_part(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part) {
   unsigned nb = red_blocks(m, kp);
   WA_VEC_SWITCH(T, kp, (k_dot_part<T, NV><<<nb, kBlock, 0, st>>>(A, B, m, kp, part)));
-  k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
+  return (int)nb;
+}
+template int launch_dot_part<float>(hipStream_t, const float*, const float*, int64_t, int, double*);
+template int launch_dot_part<double>(hipStream_t, const double*, const double*, int64_t, int, double*);
+
+template <typename T>
+void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out) {
+  const int nb = launch_dot_part<T>(st, A, B, m, kp, part);
+  k_dot_fin<<<1, kBlock, 0, st>>>(part, nb, kp, out);
 }
 template void launch_dot<float>(hipStream_t, const float*, const float*, int64_t, int, double*, double*);
 template void launch_dot<double>(hipStream_t, const double*, const double*, int64_t, int, double*, double*);
@@ -244,16 +252,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T
 }
 
 template <typename T>
-void launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m,
-                      int kp, double* part, double* out) {
+int launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m, int kp,
+                     double* part) {
   unsigned nb = red_blocks(m, kp);
   WA_VEC_SWITCH(T, kp, (k_cg_update<T, NV><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part)));
-  k_dot_fin<<<1, kBlock, 0, st>>>(part, (int)nb, kp, out);
+  return (int)nb;
 }
-template void launch_cg_update<float>(hipStream_t, float*, const float*, float*, const float*, const double*,
-                                      int64_t, int, double*, double*);
-template void launch_cg_update<double>(hipStream_t, double*, const double*, double*, const double*,
-                                       const double*, int64_t, int, double*, double*);
+template int launch_cg_update<float>(hipStream_t, float*, const float*, float*, const float*, const double*,
+                                     int64_t, int, double*);
+template int launch_cg_update<double>(hipStream_t, double*, const double*, double*, const double*, const double*,
+                                      int64_t, int, double*);
 
 // P = Z + beta P (beta per column; columns with mask 0 untouched)
 template <typename T, int NV>
@@ -349,31 +357,55 @@ __global__ void k_cg_start(ColState* __restrict__ s, const double* __restrict__ 
   q.converged = 0;
 }
 
+// Column totals of per-block partials (part[b * kp + c], b < nb): threads
+// stride over the blocks, then a fixed-order tree; the result is valid in
+// threads [0, kp).  Launched with kBlock threads.
+__device__ inline double col_total(const double* __restrict__ part, int nb, int kp, double* sh) {
+  const int col = threadIdx.x % kp, str = kBlock / kp, s0 = threadIdx.x / kp;
+  double acc = 0.0;
+  for (int b = s0; b < nb; b += str) acc += part[(int64_t)b * kp + col];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = str / 2; w >= 1; w >>= 1) {
+    if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
+    __syncthreads();
+  }
+  return sh[threadIdx.x];
+}
+
 // after A p: pap <= 0 -> stop (not converged); else alpha = rz / pap
-__global__ void k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ pap, double* __restrict__ alpha,
-                           int itn, int kp) {
+__global__ void __launch_bounds__(kBlock) k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ part,
+                                                     int nb, double* __restrict__ pap, double* __restrict__ alpha,
+                                                     int itn, int kp) {
+  __shared__ double sh[kBlock];
+  const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c >= kp) return;
+  pap[c] = v;
   ColState& q = s[c];
   alpha[c] = 0.0;
   if (!q.active) return;
-  if (!(pap[c] > 0.0)) {
+  if (!(v > 0.0)) {
     q.active = 0;
     q.converged = 0;
     q.iters = itn;
     return;
   }
-  alpha[c] = q.rz / pap[c];
+  alpha[c] = q.rz / v;
   q.alpha = alpha[c];
 }
 
 // after the update: ||r|| / ||b|| <= tol -> converged
-__global__ void k_cg_check(ColState* __restrict__ s, const double* __restrict__ rr, int itn, int kp) {
+__global__ void __launch_bounds__(kBlock) k_cg_check(ColState* __restrict__ s, const double* __restrict__ part,
+                                                     int nb, double* __restrict__ rr, int itn, int kp) {
+  __shared__ double sh[kBlock];
+  const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c >= kp) return;
+  rr[c] = v;
   ColState& q = s[c];
   if (!q.active) return;
-  q.rnorm = sqrt(rr[c]);
+  q.rnorm = sqrt(v);
   q.iters = itn;
   if (q.rnorm / q.bnorm <= q.tol) {
     q.active = 0;
@@ -382,22 +414,27 @@ __global__ void k_cg_check(ColState* __restrict__ s, const double* __restrict__ 
 }
 
 // after z = M r: stagnation guard |rz_new| < EPS^2 max(|rz0|, MIN); beta
-__global__ void k_cg_beta(ColState* __restrict__ s, const double* __restrict__ rznew, double* __restrict__ beta,
-                          int* __restrict__ mask, int* __restrict__ n_active, int itn, int kp) {
+__global__ void __launch_bounds__(kBlock) k_cg_beta(ColState* __restrict__ s, const double* __restrict__ part,
+                                                    int nb, double* __restrict__ rznew, double* __restrict__ beta,
+                                                    int* __restrict__ mask, int* __restrict__ n_active, int itn,
+                                                    int kp) {
+  __shared__ double sh[kBlock];
+  const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c < kp) {
+    rznew[c] = v;
     ColState& q = s[c];
     beta[c] = 0.0;
     mask[c] = 0;
     if (q.active) {
       const double thr = 4.930380657631324e-32 * fmax(fabs(q.rz0), 2.2250738585072014e-308);
-      if (fabs(rznew[c]) < thr) {
+      if (fabs(v) < thr) {
         q.active = 0;
         q.converged = q.rnorm / q.bnorm <= q.tol;
         q.iters = itn;
       } else {
-        beta[c] = rznew[c] / q.rz;
-        q.rz = rznew[c];
+        beta[c] = v / q.rz;
+        q.rz = v;
         mask[c] = 1;
       }
     }
@@ -414,15 +451,16 @@ void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const do
                      int kp) {
   k_cg_start<<<1, 64, 0, st>>>(s, bnorm2, rz, tol, k, kp);
 }
-void launch_cg_alpha(hipStream_t st, ColState* s, const double* pap, double* alpha, int itn, int kp) {
-  k_cg_alpha<<<1, 64, 0, st>>>(s, pap, alpha, itn, kp);
+void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int itn,
+                     int kp) {
+  k_cg_alpha<<<1, kBlock, 0, st>>>(s, part, nb, pap, alpha, itn, kp);
 }
-void launch_cg_check(hipStream_t st, ColState* s, const double* rr, int itn, int kp) {
-  k_cg_check<<<1, 64, 0, st>>>(s, rr, itn, kp);
+void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int itn, int kp) {
+  k_cg_check<<<1, kBlock, 0, st>>>(s, part, nb, rr, itn, kp);
 }
-void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* beta, int* mask, int* n_active,
-                    int itn, int kp) {
-  k_cg_beta<<<1, 64, 0, st>>>(s, rznew, beta, mask, n_active, itn, kp);
+void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, double* rznew, double* beta, int* mask,
+                    int* n_active, int itn, int kp) {
+  k_cg_beta<<<1, kBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, itn, kp);
 }
 
 // Preconditioner elementwise kernels (same lane mapping as the vector ops).

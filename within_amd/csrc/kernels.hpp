@@ -109,9 +109,12 @@ void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n,
 void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld);
 template <typename T>
 void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out);
+// per-block partials only; returns their count (finished by the CG scalar kernels)
 template <typename T>
-void launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m,
-                      int kp, double* part, double* out);
+int launch_dot_part(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part);
+template <typename T>
+int launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m, int kp,
+                     double* part);
 template <typename T>
 void launch_xpby(hipStream_t st, T* P, const T* Z, const double* beta, const int* mask, int64_t m, int kp);
 template <typename S, typename D>
@@ -121,10 +124,12 @@ void launch_axpy_acc(hipStream_t st, double* X, const S* Dl, const double* scale
                      int kp);
 void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const double* rz, const double* tol, int k,
                      int kp);
-void launch_cg_alpha(hipStream_t st, ColState* s, const double* pap, double* alpha, int itn, int kp);
-void launch_cg_check(hipStream_t st, ColState* s, const double* rr, int itn, int kp);
-void launch_cg_beta(hipStream_t st, ColState* s, const double* rznew, double* beta, int* mask, int* n_active,
-                    int itn, int kp);
+// the CG scalar steps take the per-block partials of their dot product
+void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int itn,
+                     int kp);
+void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int itn, int kp);
+void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, double* rznew, double* beta, int* mask,
+                    int* n_active, int itn, int kp);
 template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m);

@@ -665,16 +665,16 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
     ++itn;
     op_apply<T>(p, ap, kp);
     tic(2, ev);
-    launch_dot<T>(st, p, ap, m, kp, redp, pap);
-    launch_cg_alpha(st, cs, pap, alpha, (int)itn, kp);
-    launch_cg_update<T>(st, x, p, r, ap, alpha, m, kp, redp, rr);
-    launch_cg_check(st, cs, rr, (int)itn, kp);
+    int nb = launch_dot_part<T>(st, p, ap, m, kp, redp);
+    launch_cg_alpha(st, cs, redp, nb, pap, alpha, (int)itn, kp);
+    nb = launch_cg_update<T>(st, x, p, r, ap, alpha, m, kp, redp);
+    launch_cg_check(st, cs, redp, nb, rr, (int)itn, kp);
     toc(2, ev);
     if (has_pre && params.preconditioner != 0) pre_apply<T>(r, z, kp);
     else HIPCHK(hipMemcpyAsync(z, r, bytes, hipMemcpyDeviceToDevice, st));
     tic(2, ev);
-    launch_dot<T>(st, r, z, m, kp, redp, rzn);
-    launch_cg_beta(st, cs, rzn, beta, mask, nact, (int)itn, kp);
+    nb = launch_dot_part<T>(st, r, z, m, kp, redp);
+    launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, (int)itn, kp);
     launch_xpby<T>(st, p, z, beta, mask, m, kp);
     toc(2, ev);
     HIPCHK(hipMemcpyAsync(hbuf, nact, sizeof(int), hipMemcpyDeviceToHost, st));
