@@ -225,13 +225,14 @@ def main():
     value = sum(solved) / t_step
     n_op, ms_op = cls["explicit" if args.operator == "explicit" else "operator"]
     Q = 3
+    m_dev = int(fast.s.info()[14]) or m  # DOFs the device works on (active levels)
     if args.operator == "explicit":
         nnz = int(fast.s.info()[8])
         # CSR indices + FP32 values once, row pointers, X read + GX written (FP32, k columns)
-        b_op = 8 * nnz + 4 * (m + 1) + 2 * 4 * k * m
+        b_op = 8 * nnz + 4 * (m_dev + 1) + 2 * 4 * k * m_dev
         op_name = f"explicit Gramian SpMM (nnz={nnz}, k={k} FP32)"
     else:
-        b_op = 4 * Q * n + 2 * 4 * k * m  # codes once + X read + GX written (FP32, k columns)
+        b_op = 4 * Q * n + 2 * 4 * k * m_dev  # codes once + X read + GX written (FP32, k columns)
         op_name = f"implicit operator apply (k={k} FP32, all factor passes)"
     op_ms = ms_op / max(n_op, 1)
     achieved = b_op / (op_ms * 1e-3) / 1e9 if n_op else None
@@ -255,7 +256,8 @@ def main():
         "dtype": "f32 inner Krylov + f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"AKM panel n={n}, worker 2M Zipf 1.1 / firm 200k mobility 0.05 / year 30, "
-                               f"k={k} (Y=[y,X])", "n_obs": n, "n_dofs": m, "k": k, "scale": args.scale},
+                               f"k={k} (Y=[y,X])", "n_obs": n, "n_dofs": m, "active_dofs": m_dev, "k": k,
+                   "scale": args.scale},
         "plain_fp64_k1_rhs_per_s": round(base_rate, 4),
         "speedup_vs_plain_fp64": round(value / base_rate, 3) if base_rate > 0 else None,
         "iterations": [s[1] for s in st],

@@ -135,7 +135,8 @@ wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, 
 /* ---- setup introspection (host data; no GPU needed) ------------------------
  * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
  * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,
- * operator_passes, dense_subdomains, max_core, reserved, reserved. */
+ * operator_passes, dense_subdomains, max_core, device_dofs (active levels
+ * the device solves over; 0 for a host-only solver), reserved. */
 wa_status wa_solver_info(const wa_solver* s, int64_t* info16);
 /* Subdomain i: meta[0..7] = q, r, n_q, n_r, eliminate_q, dense, rounds, core
  * size; gidx (n_q + n_r), pou (n_q + n_r) may be NULL. */

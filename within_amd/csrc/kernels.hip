@@ -34,15 +34,19 @@ static inline unsigned grid_stride_blocks(int64_t n, int kp) {
 // ===========================================================================
 // layout conversion: column-major (ld) <-> row-major kp blocks, via LDS tiles
 // ===========================================================================
+// dst row i <- src row (map ? map[i] : i)
 template <typename S, typename D>
 __global__ void __launch_bounds__(kBlock) k_col_to_row(const S* __restrict__ src, int64_t ld, int64_t n,
-                                                       int k, D* __restrict__ dst, int kp) {
+                                                       int k, D* __restrict__ dst, int kp,
+                                                       const uint32_t* __restrict__ map) {
   __shared__ double tile[32][65];
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   for (int t = threadIdx.x; t < 64 * 32; t += kBlock) {
     int r = t & 63, c = t >> 6;
     int64_t i = r0 + r;
-    tile[c][r] = (c < k && i < n) ? (double)src[i + c * ld] : 0.0;
+    double v = 0.0;
+    if (c < k && i < n) v = (double)src[(map ? (int64_t)map[i] : i) + c * ld];
+    tile[c][r] = v;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < 64 * kp; t += kBlock) {
@@ -52,15 +56,22 @@ __global__ void __launch_bounds__(kBlock) k_col_to_row(const S* __restrict__ src
   }
 }
 
+// dst row i <- src row (map ? map[i] : i); a map entry of UINT32_MAX gives zeros
 template <typename S, typename D>
 __global__ void __launch_bounds__(kBlock) k_row_to_col(const S* __restrict__ src, int kp, int64_t n, int k,
-                                                       D* __restrict__ dst, int64_t ld) {
+                                                       D* __restrict__ dst, int64_t ld,
+                                                       const uint32_t* __restrict__ map) {
   __shared__ double tile[32][65];
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   for (int t = threadIdx.x; t < 64 * kp; t += kBlock) {
     int c = t % kp, r = t / kp;
     int64_t i = r0 + r;
-    tile[c][r] = i < n ? (double)src[i * kp + c] : 0.0;
+    double v = 0.0;
+    if (i < n) {
+      const uint32_t j = map ? map[i] : (uint32_t)i;
+      if (j != UINT32_MAX) v = (double)src[(int64_t)j * kp + c];
+    }
+    tile[c][r] = v;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < 64 * 32; t += kBlock) {
@@ -70,11 +81,15 @@ __global__ void __launch_bounds__(kBlock) k_row_to_col(const S* __restrict__ src
   }
 }
 
-void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp) {
-  if (n > 0) k_col_to_row<double, double><<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(src, ld, n, k, dst, kp);
+void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp,
+                       const uint32_t* map) {
+  if (n > 0)
+    k_col_to_row<double, double><<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(src, ld, n, k, dst, kp, map);
 }
-void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld) {
-  if (n > 0) k_row_to_col<double, double><<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(src, kp, n, k, dst, ld);
+void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld,
+                       const uint32_t* map) {
+  if (n > 0)
+    k_row_to_col<double, double><<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(src, kp, n, k, dst, ld, map);
 }
 
 // ===========================================================================

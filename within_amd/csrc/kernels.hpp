@@ -105,8 +105,12 @@ void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* 
                         double* part, int kp);
 void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
                    const double* X, const double* Y, double* E, int kp);
-void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp);
-void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld);
+// column-major (ld) <-> row-major kp blocks; with `map`, row i of the
+// destination comes from row map[i] of the source (UINT32_MAX: zeros)
+void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp,
+                       const uint32_t* map = nullptr);
+void launch_row_to_col(hipStream_t st, const double* src, int kp, int64_t n, int k, double* dst, int64_t ld,
+                       const uint32_t* map = nullptr);
 template <typename T>
 void launch_dot(hipStream_t st, const T* A, const T* B, int64_t m, int kp, double* part, double* out);
 // per-block partials only; returns their count (finished by the CG scalar kernels)

@@ -913,6 +913,160 @@ static void flatten(Plan& p) {
   }
 }
 
+DeviceLayout make_device_layout(const Plan& p) {
+  const Design& d = p.d;
+  const int Q = d.Q;
+  const i64 n = d.n_obs;
+  DeviceLayout L;
+  // active levels
+  std::vector<char> act(d.n_dofs, 0);
+  for (int q = 0; q < Q; ++q) {
+    const u32* c = d.codes[q].data();
+    const i64 o = d.offset[q];
+    for (i64 i = 0; i < n; ++i) act[o + c[i]] = 1;
+  }
+  // factor with the most levels, ordered by the factor with the next-most
+  int big = 0, ref = -1;
+  for (int q = 1; q < Q; ++q)
+    if (d.levels[q] > d.levels[big]) big = q;
+  for (int q = 0; q < Q; ++q)
+    if (q != big && (ref < 0 || d.levels[q] > d.levels[ref])) ref = q;
+  L.int_of.assign(d.n_dofs, UINT32_MAX);
+  for (int q = 0; q < Q; ++q) {
+    std::vector<u32> lv;
+    for (i64 l = 0; l < d.levels[q]; ++l)
+      if (act[d.offset[q] + l]) lv.push_back((u32)l);
+    if (q == big && ref >= 0) {
+      std::vector<u32> key(d.levels[q], UINT32_MAX);
+      const u32* cb = d.codes[q].data();
+      const u32* cr = d.codes[ref].data();
+      for (i64 i = 0; i < n; ++i) key[cb[i]] = std::min(key[cb[i]], cr[i]);
+      std::stable_sort(lv.begin(), lv.end(), [&](u32 a, u32 b) { return key[a] < key[b]; });
+    }
+    L.off.push_back((i64)L.ext_of.size());
+    L.levels.push_back((i64)lv.size());
+    for (u32 l : lv) {
+      L.int_of[d.offset[q] + l] = (u32)L.ext_of.size();
+      L.ext_of.push_back((u32)(d.offset[q] + l));
+    }
+  }
+  L.m = (i64)L.ext_of.size();
+  // observations -> internal DOFs
+  L.dof_of_obs.resize((size_t)Q * n);
+  for (int q = 0; q < Q; ++q) {
+    const u32* c = d.codes[q].data();
+    const i64 o = d.offset[q];
+    for (i64 i = 0; i < n; ++i) L.dof_of_obs[(size_t)q * n + i] = L.int_of[o + c[i]];
+  }
+  // factor passes: active rows in internal order; elements re-sorted by the
+  // internal index of the largest other factor (gather locality)
+  const int Qm = Q - 1;
+  L.passes.resize(Q);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < Q; ++q) {
+    const FactorPass& fp = p.passes[q];
+    FactorPass& np = L.passes[q];
+    np.q = q;
+    int sec = -1;
+    for (int r = 0; r < Q; ++r)
+      if (r != q && (sec < 0 || d.levels[r] > d.levels[sec])) sec = r;
+    const int ts = sec < 0 ? 0 : (sec < q ? sec : sec - 1);
+    np.row_ptr.assign(1, 0);
+    np.obs.reserve(fp.obs.size());
+    np.other.reserve(fp.other.size());
+    if (!fp.w.empty()) np.w.reserve(fp.w.size());
+    std::vector<u32> idx;
+    for (i64 r = 0; r < L.levels[q]; ++r) {
+      const u32 l = L.ext_of[L.off[q] + r] - (u32)d.offset[q];
+      idx.clear();
+      for (u32 e = fp.row_ptr[l]; e < fp.row_ptr[l + 1]; ++e) idx.push_back(e);
+      if (Qm > 0)
+        std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) {
+          return L.int_of[fp.other[(size_t)a * Qm + ts]] < L.int_of[fp.other[(size_t)b * Qm + ts]];
+        });
+      for (u32 e : idx) {
+        np.obs.push_back(fp.obs[e]);
+        for (int t = 0; t < Qm; ++t) np.other.push_back(L.int_of[fp.other[(size_t)e * Qm + t]]);
+        if (!fp.w.empty()) np.w.push_back(fp.w[e]);
+      }
+      np.row_ptr.push_back((u32)np.obs.size());
+      np.diag.push_back(fp.diag[l]);
+    }
+  }
+  // explicit Gramian: rows and columns renumbered, columns ascending
+  if (p.has_gram) {
+    const Csr& g = p.gram;
+    Csr& ng = L.gram;
+    ng.nrows = ng.ncols = L.m;
+    ng.indptr.assign(1, 0);
+    ng.indices.reserve(g.indices.size());
+    ng.data.reserve(g.data.size());
+    std::vector<std::pair<u32, double>> row;
+    for (i64 r = 0; r < L.m; ++r) {
+      const u32 u = L.ext_of[r];
+      row.clear();
+      for (u32 k = g.indptr[u]; k < g.indptr[u + 1]; ++k) row.push_back({L.int_of[g.indices[k]], g.data[k]});
+      std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+      for (const auto& e : row) {
+        ng.indices.push_back(e.first);
+        ng.data.push_back(e.second);
+      }
+      ng.indptr.push_back((u32)ng.indices.size());
+    }
+  }
+  // preconditioner: DOF indices renumbered; each subdomain's eliminated block
+  // (its elimination rows) re-laid out in internal DOF order
+  if (!p.subs.empty()) {
+    const PrecondFlat& f = p.pre;
+    PrecondFlat nf = f;
+    const i64 ne = (i64)f.elim_loc.size();
+    std::vector<u32> lp(f.n_loc);
+    std::iota(lp.begin(), lp.end(), 0u);
+    std::vector<u32> order(ne);
+    std::iota(order.begin(), order.end(), 0u);
+    i64 a = 0;
+    while (a < ne) {  // runs of elimination rows of one subdomain
+      const u32 sub = f.loc_sub[f.elim_loc[a]];
+      i64 b = a;
+      u32 base = f.elim_loc[a];
+      while (b < ne && f.loc_sub[f.elim_loc[b]] == sub) base = std::min(base, f.elim_loc[b++]);
+      std::stable_sort(order.begin() + a, order.begin() + b, [&](u32 x, u32 y) {
+        return L.int_of[f.loc_gidx[f.elim_loc[x]]] < L.int_of[f.loc_gidx[f.elim_loc[y]]];
+      });
+      for (i64 t = a; t < b; ++t) lp[f.elim_loc[order[t]]] = base + (u32)(t - a);
+      a = b;
+    }
+    for (i64 l = 0; l < f.n_loc; ++l) {
+      nf.loc_gidx[lp[l]] = L.int_of[f.loc_gidx[l]];
+      nf.loc_scale[lp[l]] = f.loc_scale[l];
+      nf.loc_sub[lp[l]] = f.loc_sub[l];
+    }
+    nf.elim_ptr.assign(1, 0);
+    nf.elim_idx.clear();
+    nf.elim_val.clear();
+    for (i64 t = 0; t < ne; ++t) {
+      const u32 i = order[t];
+      nf.elim_loc[t] = lp[f.elim_loc[i]];
+      nf.elim_inv[t] = f.elim_inv[i];
+      for (u32 k = f.elim_ptr[i]; k < f.elim_ptr[i + 1]; ++k) {
+        nf.elim_idx.push_back(f.elim_idx[k]);
+        nf.elim_val.push_back(f.elim_val[k]);
+      }
+      nf.elim_ptr.push_back((u32)nf.elim_idx.size());
+    }
+    for (size_t k = 0; k < f.keep_idx.size(); ++k) nf.keep_idx[k] = lp[f.keep_idx[k]];
+    for (i64 j = 0; j < f.n_red; ++j) nf.red_loc[j] = lp[f.red_loc[j]];
+    nf.dof_ptr.assign(L.m + 1, 0);
+    for (u32 g : nf.loc_gidx) nf.dof_ptr[g + 1]++;
+    for (i64 j = 0; j < L.m; ++j) nf.dof_ptr[j + 1] += nf.dof_ptr[j];
+    nf.dof_loc.resize(nf.n_loc);
+    std::vector<u32> cur(nf.dof_ptr.begin(), nf.dof_ptr.end() - 1);
+    for (i64 l = 0; l < nf.n_loc; ++l) nf.dof_loc[cur[nf.loc_gidx[l]]++] = (u32)l;
+    L.pre = std::move(nf);
+  }
+  return L;
+}
+
 void build_plan(Plan& p) {
   const Design& d = p.d;
   p.passes.resize(d.Q);

@@ -174,6 +174,26 @@ struct Plan {
 // Builds everything the solver uploads (throws wa::Error).
 void build_plan(Plan& p);
 
+// The device works in an internal DOF numbering: only active levels (levels
+// with at least one observation; the others are zero rows and columns of G
+// and stay zero in every solve), factor blocks in factor order, the levels of
+// the factor with the most levels ordered by the smallest code they meet of
+// the factor with the next-most levels (workers grouped by firm), the rest in
+// level order.  Every device array indexed by DOF is built in this numbering;
+// the C ABI maps user DOF vectors in and out.  The local layout of each
+// subdomain's eliminated block follows the same order.
+struct DeviceLayout {
+  i64 m = 0;                        // internal DOFs
+  std::vector<u32> int_of;          // user DOF -> internal (UINT32_MAX = inactive)
+  std::vector<u32> ext_of;          // internal -> user DOF
+  std::vector<i64> off, levels;     // per factor: internal offset and active levels
+  std::vector<u32> dof_of_obs;      // Q x n, factor-major: internal DOF of each observation
+  std::vector<FactorPass> passes;   // rows = active levels in internal order, `other` internal
+  Csr gram;                         // explicit Gramian in internal numbering (if built)
+  PrecondFlat pre;                  // local / DOF arrays in internal numbering
+};
+DeviceLayout make_device_layout(const Plan& p);
+
 // Individual steps (exposed for tests through the C ABI).
 std::vector<Subdomain> build_subdomains(const Design& d);
 void build_local_solver(Subdomain& s, const LocalConfig& cfg);

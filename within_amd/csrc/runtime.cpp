@@ -97,7 +97,10 @@ struct wa_solver {
   bool host_only = false;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
-  int64_t n = 0, m = 0;
+  int64_t n = 0, m = 0;      // observations, internal DOFs (active levels)
+  int64_t mu = 0;            // user DOFs (every level of every factor)
+  const uint32_t* int_of = nullptr;  // user DOF -> internal (UINT32_MAX = inactive)
+  const uint32_t* ext_of = nullptr;  // internal -> user DOF
   int Q = 0;
   // design
   uint32_t* codes = nullptr;  // Q x n (original order)
@@ -284,23 +287,24 @@ static void check_device() {
 
 void wa_solver::setup_device() {
   const Design& d = plan.d;
-  n = d.n_obs;
-  m = d.n_dofs;
-  Q = d.Q;
   check_device();
+  const DeviceLayout lay = make_device_layout(plan);
+  n = d.n_obs;
+  m = lay.m;
+  mu = d.n_dofs;
+  Q = d.Q;
   HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  // design (original observation order) for D x and demeaning
+  int_of = upload(lay.int_of);
+  ext_of = upload(lay.ext_of);
+  // design (original observation order) for D x and demeaning: the internal
+  // DOF of every (factor, observation), so the factor offsets are zero
   {
-    std::vector<uint32_t> c((size_t)Q * n);
-    for (int q = 0; q < Q; ++q) std::memcpy(&c[(size_t)q * n], d.codes[q].data(), sizeof(uint32_t) * n);
-    codes = upload(c);
-    std::vector<uint32_t> o;
-    for (int q = 0; q < Q; ++q) o.push_back((uint32_t)d.offset[q]);
-    offs = upload(o);
+    codes = upload(lay.dof_of_obs);
+    offs = upload(std::vector<uint32_t>(Q, 0u));
     if (!d.weights.empty()) wobs = upload(d.weights);
   }
   int64_t max_slots = 1;
-  for (const FactorPass& fp : plan.passes) {
+  for (const FactorPass& fp : lay.passes) {
     DevPass<double> a;
     DevPass<float> b;
     a.row_ptr = b.row_ptr = upload(fp.row_ptr);
@@ -313,8 +317,8 @@ void wa_solver::setup_device() {
     a.diag = upload(fp.diag);
     b.diag = upload_as<float>(fp.diag);
     a.qm = b.qm = Q - 1;
-    a.off = b.off = (uint32_t)d.offset[fp.q];
-    a.levels = b.levels = d.levels[fp.q];
+    a.off = b.off = (uint32_t)lay.off[fp.q];
+    a.levels = b.levels = lay.levels[fp.q];
     a.n_elem = b.n_elem = n;
     {
       Schedule sc = make_schedule(fp.row_ptr, kChunk, kShortPass);
@@ -326,7 +330,7 @@ void wa_solver::setup_device() {
     p32.push_back(b);
   }
   if (plan.has_gram) {
-    const Csr& g = plan.gram;
+    const Csr& g = lay.gram;
     g64.ptr = g32.ptr = upload(g.indptr);
     g64.idx = g32.idx = upload(g.indices);
     g64.val = upload(g.data);
@@ -342,7 +346,7 @@ void wa_solver::setup_device() {
     has_gram = true;
   }
   if (!plan.subs.empty()) {
-    const PrecondFlat& f = plan.pre;
+    const PrecondFlat& f = lay.pre;
     // dense cores: the large ones are applied by the dense kernels from
     // zero-padded copies; the rest keep the gather path (their matrices packed)
     const size_t n_cores = f.core_ptr.empty() ? 0 : f.core_ptr.size() - 1;
@@ -987,7 +991,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
   std::vector<double> fr = col_norms(R64, m, kp);
   tic(3, ev);
   if (dem) launch_row_to_col(st, E, kp, n, kb, dem + (int64_t)c0 * ldd, ldd);
-  launch_row_to_col(st, X64, kp, m, kb, X + (int64_t)c0 * ldx, ldx);
+  launch_row_to_col(st, X64, kp, mu, kb, X + (int64_t)c0 * ldx, ldx, int_of);
   toc(3, ev);
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
@@ -1151,13 +1155,13 @@ static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ld
   try {
     need_device(s);
     if (k < 0 || !stats || (k > 0 && (!Y || !X))) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
-    if (ldy < s->n || ldx < s->m || (dem && ldd < s->n))
+    if (ldy < s->n || ldx < s->mu || (dem && ldd < s->n))
       throw Error(WA_ERR_OBS_COUNT_MISMATCH, "ObservationCountMismatch: leading dimension too small");
     if (k == 0) return WA_OK;
     const double* dY = Y;
     double *dX = X, *dD = dem;
     if (!device) {
-      int64_t need = ldy * k + s->m * (int64_t)k + (dem ? s->n * (int64_t)k : 0);
+      int64_t need = ldy * k + s->mu * (int64_t)k + (dem ? s->n * (int64_t)k : 0);
       if (need > s->ubuf_len) {
         if (s->ubuf) HIPCHK(hipFree(s->ubuf));
         s->ubuf = nullptr;
@@ -1168,9 +1172,9 @@ static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ld
       HIPCHK(hipMemcpyAsync(y, Y, sizeof(double) * ldy * k, hipMemcpyHostToDevice, s->st));
       dY = y;
       dX = y + ldy * k;
-      dD = dem ? dX + s->m * (int64_t)k : nullptr;
+      dD = dem ? dX + s->mu * (int64_t)k : nullptr;
     }
-    const int64_t lx = device ? ldx : s->m, ld = device ? ldd : s->n;
+    const int64_t lx = device ? ldx : s->mu, ld = device ? ldd : s->n;
     int blk = s->params.block <= 0 ? KMAX : std::min(s->params.block, KMAX);
     for (int c0 = 0; c0 < k; c0 += blk) {
       int kb = std::min(blk, k - c0);
@@ -1178,7 +1182,7 @@ static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ld
     }
     if (!device) {
       for (int c = 0; c < k; ++c)
-        HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, dX + (int64_t)c * s->m, sizeof(double) * s->m,
+        HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, dX + (int64_t)c * s->mu, sizeof(double) * s->mu,
                               hipMemcpyDeviceToHost, s->st));
       if (dem)
         for (int c = 0; c < k; ++c)
@@ -1202,13 +1206,13 @@ wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_
 }
 
 wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
-  return level_a(s, X, s ? s->m : 0, ld, Y, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+  return level_a(s, X, s ? s->mu : 0, ld, Y, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
     int saved = s->params.operator_repr;
     s->params.operator_repr = 0;
     s->op_apply<double>(s->cp64, s->cap64, kp);
     s->params.operator_repr = saved;
-    launch_row_to_col(s->st, s->cap64, kp, s->m, k, dout, s->m);
+    launch_row_to_col(s->st, s->cap64, kp, s->mu, k, dout, s->mu, s->int_of);
   });
 }
 
@@ -1217,10 +1221,10 @@ wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32
     g_err = "solver has no explicit Gramian (build with operator_repr = 1 or multiplicative Schwarz)";
     return WA_ERR_INVALID_ARGUMENT;
   }
-  return level_a(s, X, s ? s->m : 0, ld, Y, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+  return level_a(s, X, s ? s->mu : 0, ld, Y, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
     launch_spmm<double>(s->st, s->g64, s->cp64, s->cap64, s->part64, kp);
-    launch_row_to_col(s->st, s->cap64, kp, s->m, k, dout, s->m);
+    launch_row_to_col(s->st, s->cap64, kp, s->mu, k, dout, s->mu, s->int_of);
   });
 }
 
@@ -1229,26 +1233,26 @@ wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, 
     g_err = "solver has no preconditioner";
     return WA_ERR_INVALID_ARGUMENT;
   }
-  return level_a(s, R, s ? s->m : 0, ld, Z, s ? s->m : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->m, s->m, k, s->cr64, kp);
+  return level_a(s, R, s ? s->mu : 0, ld, Z, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cr64, kp, s->ext_of);
     s->pre_apply<double>(s->cr64, s->cz64, kp);
-    launch_row_to_col(s->st, s->cz64, kp, s->m, k, dout, s->m);
+    launch_row_to_col(s->st, s->cz64, kp, s->mu, k, dout, s->mu, s->int_of);
   });
 }
 
 wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k) {
-  return level_a(s, X, s ? s->m : 0, ldx, Y, s ? s->n : 0, ldy, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->m, s->m, k, s->cp64, kp);
+  return level_a(s, X, s ? s->mu : 0, ldx, Y, s ? s->n : 0, ldy, k, [&](double* din, double* dout, int kp) {
+    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
     launch_demean(s->st, s->n, s->Q, s->codes, s->offs, s->cp64, nullptr, s->E, kp);
     launch_row_to_col(s->st, s->E, kp, s->n, k, dout, s->n);
   });
 }
 
 wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
-  return level_a(s, R, s ? s->n : 0, ldr, X, s ? s->m : 0, ldx, k, [&](double* din, double* dout, int kp) {
+  return level_a(s, R, s ? s->n : 0, ldr, X, s ? s->mu : 0, ldx, k, [&](double* din, double* dout, int kp) {
     launch_col_to_row(s->st, din, s->n, s->n, k, s->Yr, kp);
     s->rhs64(s->Yr, s->B64, kp);
-    launch_row_to_col(s->st, s->B64, kp, s->m, k, dout, s->m);
+    launch_row_to_col(s->st, s->B64, kp, s->mu, k, dout, s->mu, s->int_of);
   });
 }
 
@@ -1270,6 +1274,7 @@ wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
   info[11] = (int64_t)s->plan.passes.size();
   info[12] = f.dense_subs;
   info[13] = f.max_core;
+  info[14] = s->host_only ? 0 : s->m;  // DOFs the device works on (active levels)
   return WA_OK;
 }
 
