@@ -216,7 +216,13 @@ inline unsigned engine_blocks(int64_t waves) { return (unsigned)((waves * kWave 
 template <class K, typename T, int V, int NV>
 void launch_rows(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T* part) {
   constexpr int S = kWave / V;
-  const int64_t n_sw = ((int64_t)g.r1 - g.r0 + S - 1) / S, ni = g.i1 - g.i0;
+  // short-row waves padded to whole XCD rounds, so that item i runs in
+  // workgroup (n_sw + i) / 4 with the XCD (i / 4) mod 8 the schedule assumed
+  static_assert(kBlock / kWave == (int)kWavesPerGroup, "schedules assume 4 waves per workgroup");
+  constexpr int64_t kRound = (int64_t)kXcds * kWavesPerGroup;
+  const int64_t raw = ((int64_t)g.r1 - g.r0 + S - 1) / S;
+  const int64_t n_sw = (raw + kRound - 1) / kRound * kRound;
+  const int64_t ni = g.i1 - g.i0;
   if (n_sw + ni == 0) return;
   k_rows<K, T, V, NV><<<engine_blocks(n_sw + ni), kBlock, 0, st>>>(k, g.r0, g.r1, sc.short_max, n_sw,
                                                                    sc.items + g.i0, ni, part);

@@ -66,20 +66,52 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
 void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk,
                      const std::vector<char>* skip) {
   Segment g{r0, r1, (i64)s.items.size(), 0, (i64)s.fixups.size(), 0};
+  std::vector<Item> whole;
+  std::vector<u32> split;
   for (u32 r = r0; r < r1; ++r) {
     const u32 len = row_ptr[r + 1] - row_ptr[r];
     if (len <= s.short_max || (skip && (*skip)[r])) continue;
-    if (len <= chunk) {
-      s.items.push_back({r, r + 1, row_ptr[r], row_ptr[r + 1], -1, 0});
-      continue;
-    }
-    const u32 slot0 = (u32)s.n_slots;
-    for (u32 e0 = row_ptr[r]; e0 < row_ptr[r + 1]; e0 += chunk) {
-      const u32 e1 = std::min(e0 + chunk, row_ptr[r + 1]);
-      s.items.push_back({r, r + 1, e0, e1, (int32_t)s.n_slots++, 0});
-    }
-    s.fixups.push_back({r, slot0, (u32)s.n_slots, 0});
+    if (len <= chunk) whole.push_back({r, r + 1, row_ptr[r], row_ptr[r + 1], -1, 0});
+    else split.push_back(r);
   }
+  // split rows: each row's slots are consecutive (its fix-up sums them in
+  // order); the chunk items are laid out XCD by XCD when several long rows
+  // share the segment
+  auto nchunks = [&](u32 r) { return (row_ptr[r + 1] - row_ptr[r] + chunk - 1) / chunk; };
+  std::stable_sort(split.begin(), split.end(), [&](u32 a, u32 b) { return nchunks(a) > nchunks(b); });
+  std::vector<u32> slot0(split.size());
+  for (size_t t = 0; t < split.size(); ++t) {
+    slot0[t] = (u32)s.n_slots;
+    s.n_slots += nchunks(split[t]);
+    s.fixups.push_back({split[t], slot0[t], (u32)s.n_slots, 0});
+  }
+  auto chunk_item = [&](size_t t, u32 j) {
+    const u32 r = split[t];
+    const u32 e0 = row_ptr[r] + j * chunk, e1 = std::min(e0 + chunk, row_ptr[r + 1]);
+    return Item{r, r + 1, e0, e1, (int32_t)(slot0[t] + j), 0};
+  };
+  if (split.size() >= kWavesPerGroup) {
+    // chunk index j goes to workgroups ≡ j (mod kXcds); rows with more chunks
+    // come first, so the rows still active at j are a prefix
+    const Item none{0, 0, 0, 0, -1, 0};
+    const u32 cmax = nchunks(split[0]);
+    size_t act = split.size();
+    for (u32 j0 = 0; j0 < cmax; j0 += kXcds) {
+      while (act > 0 && nchunks(split[act - 1]) <= j0) --act;
+      const size_t per = (act + kWavesPerGroup - 1) / kWavesPerGroup;  // workgroups per chunk index
+      for (size_t w = 0; w < per * kXcds; ++w) {
+        const u32 j = j0 + (u32)(w % kXcds);
+        for (u32 v = 0; v < kWavesPerGroup; ++v) {
+          const size_t t = (w / kXcds) * kWavesPerGroup + v;
+          s.items.push_back(t < act && j < nchunks(split[t]) ? chunk_item(t, j) : none);
+        }
+      }
+    }
+  } else {
+    for (size_t t = 0; t < split.size(); ++t)
+      for (u32 j = 0; j < nchunks(split[t]); ++j) s.items.push_back(chunk_item(t, j));
+  }
+  s.items.insert(s.items.end(), whole.begin(), whole.end());
   g.i1 = (i64)s.items.size();
   g.f1 = (i64)s.fixups.size();
   s.segs.push_back(g);

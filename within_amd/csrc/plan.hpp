@@ -57,6 +57,12 @@ struct Item { u32 row0, row1, e0, e1; int32_t slot; int32_t pad; };
 struct Fixup { u32 row, slot0, slot1, pad; };
 // one row range with its items [i0, i1) and fix-ups [f0, f1)
 struct Segment { u32 r0, r1; i64 i0, i1, f0, f1; };
+
+// Workgroups are dispatched to the MI355X's 8 XCDs (each with its own L2)
+// round-robin, and the engine runs 4 waves (items) per workgroup.  Chunks of
+// different split rows that cover the same stretch of columns are placed in
+// workgroups with the same index mod 8, so they share one L2.
+constexpr u32 kXcds = 8, kWavesPerGroup = 4;
 struct Schedule {
   std::vector<Item> items;
   std::vector<Fixup> fixups;
