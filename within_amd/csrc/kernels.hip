@@ -953,6 +953,9 @@ __global__ void __launch_bounds__(kBlock) k_core_tiled(const T* __restrict__ M, 
   constexpr int NV = 16 / sizeof(T);  // values per 16-byte vector
   constexpr int TK = 128 / sizeof(T); // columns per step (128-byte row segments)
   constexpr int TR = kCoreRows, CG = KP / 4, RT = KP / 8;
+  constexpr int VPR = TK / NV;                 // vectors per row segment
+  constexpr int MV = TR * VPR / kBlock;        // M vectors per thread per step
+  static_assert(TR * VPR % kBlock == 0, "tile shape");
   __shared__ T Ms[TK][TR + NV];
   __shared__ __align__(16) T Zs[TK * KP];
   const int tid = threadIdx.x;
@@ -964,23 +967,34 @@ __global__ void __launch_bounds__(kBlock) k_core_tiled(const T* __restrict__ M, 
   for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[i][c] = 0;
-  for (uint32_t k0 = ka; k0 < kb; k0 += TK) {
-    constexpr int VPR = TK / NV;  // vectors per row segment
-    for (int v = tid; v < TR * VPR; v += kBlock) {
+  // the next M tile is loaded into registers while the current one is used
+  VecT<T, NV> mr[MV];
+  auto load_m = [&](uint32_t k0) {
+#pragma unroll
+    for (int u = 0; u < MV; ++u) {
+      const int v = tid + u * kBlock;
       const int row = v / VPR, kv = (v % VPR) * NV;
-      VecT<T, NV> x;
       if (r0 + row < n) {
-        x = vload<T, NV>(M + (size_t)(r0 + row) * ld + k0 + kv);
+        mr[u] = vload<T, NV>(M + (size_t)(r0 + row) * ld + k0 + kv);
       } else {
 #pragma unroll
-        for (int c = 0; c < NV; ++c) x.v[c] = 0;
+        for (int c = 0; c < NV; ++c) mr[u].v[c] = 0;
       }
+    }
+  };
+  if (ka < kb) load_m(ka);
+  for (uint32_t k0 = ka; k0 < kb; k0 += TK) {
 #pragma unroll
-      for (int c = 0; c < NV; ++c) Ms[kv + c][row] = x.v[c];
+    for (int u = 0; u < MV; ++u) {
+      const int v = tid + u * kBlock;
+      const int row = v / VPR, kv = (v % VPR) * NV;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) Ms[kv + c][row] = mr[u].v[c];
     }
     for (int v = tid; v < TK * KP / NV; v += kBlock)
       *reinterpret_cast<VecT<T, NV>*>(&Zs[v * NV]) = vload<T, NV>(Zc + (size_t)k0 * KP + v * NV);
     __syncthreads();
+    if (k0 + TK < kb) load_m(k0 + TK);
 #pragma unroll 4
     for (int kk = 0; kk < TK; ++kk) {
       T mv[RT], zv[4];
@@ -1063,10 +1077,10 @@ __global__ void __launch_bounds__(kBlock) k_core_gemv(const T* __restrict__ M, c
         for (int c = 0; c < KP; ++c) C[(size_t)(r0 + i) * KP + c] = acc[i][c];
 }
 
-// column split for a core: enough workgroups to cover the chip
+// column split for a core: about eight workgroups per CU
 static inline int core_split(uint32_t n, uint32_t ld, int tk) {
   const int tiles = (int)((n + kCoreRows - 1) / kCoreRows);
-  int s = std::max(1, std::min(kCoreSplitMax, 1024 / std::max(tiles, 1)));
+  int s = std::max(1, std::min(kCoreSplitMax, 2048 / std::max(tiles, 1)));
   return std::min<int>(s, (int)(ld / tk));
 }
 

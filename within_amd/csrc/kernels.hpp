@@ -14,6 +14,7 @@ constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 constexpr int kRedBlocks = 512;   // fixed reduction grid (deterministic sums)
 constexpr uint32_t kChunk = 256;  // elements per work item (long rows are split)
 constexpr uint32_t kShortRow = 16;   // rows this short are summed by one lane group
+constexpr uint32_t kShortElim = 32;  // the same for elimination rows (their gathers hit L1)
 constexpr uint32_t kShortPass = 8;   // the same for factor passes (several gathers per element)
 
 struct DevSched {
@@ -52,7 +53,7 @@ struct DevCsr {
 // A dense core applied by the dense kernels: core rows [b, b + n), matrix
 // n x n row-major with leading dimension ld (a multiple of 32, zero padded).
 constexpr uint32_t kDenseCore = 512;  // cores at least this large
-constexpr int kCoreSplitMax = 16;     // split of the column range (partials)
+constexpr int kCoreSplitMax = 32;     // split of the column range (partials)
 template <typename T>
 struct DenseCore {
   uint32_t b = 0, n = 0, ld = 0;

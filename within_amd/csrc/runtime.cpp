@@ -450,15 +450,16 @@ void wa_solver::setup_device() {
       sched_report(name, sc, ptr);
       return upload_sched(sc);
     };
-    auto whole_sched = [&](const char* name, const std::vector<uint32_t>& ptr, const std::vector<char>* skip) {
-      Schedule sc = make_schedule(ptr, kChunk, kShortRow, skip);
+    auto whole_sched = [&](const char* name, const std::vector<uint32_t>& ptr, const std::vector<char>* skip,
+                           uint32_t short_max = kShortRow) {
+      Schedule sc = make_schedule(ptr, kChunk, short_max, skip);
       sched_report(name, sc, ptr);
       return upload_sched(sc);
     };
     DevSched s_loc = whole_sched("local", f.loc_ptr, nullptr);
     DevSched s_red = whole_sched("reduced", f.red_ptr, nullptr);
     DevSched s_keep = whole_sched("keep", f.keep_ptr, nullptr);
-    DevSched s_elim = whole_sched("elim", f.elim_ptr, nullptr);
+    DevSched s_elim = whole_sched("elim", f.elim_ptr, nullptr, kShortElim);
     DevSched s_fwd = level_sched("forward", f.fin_ptr);
     DevSched s_bwd = level_sched("backward", f.bout_ptr);
     static_assert(kDenseCore > kShortRow, "dense-core rows must not be short rows");
@@ -490,16 +491,16 @@ void wa_solver::setup_device() {
         ps.red0 = f.red_ptr[s0]; ps.red1 = f.red_ptr[s1];
         ps.core0 = core_row_sub[s0]; ps.core1 = core_row_sub[s1];
         auto range = [&](const std::vector<uint32_t>& ptr, int64_t r0, int64_t r1,
-                         const std::vector<char>* skip = nullptr) {
+                         const std::vector<char>* skip = nullptr, uint32_t short_max = kShortRow) {
           Schedule sc;
-          sc.short_max = kShortRow;
+          sc.short_max = short_max;
           append_schedule(sc, ptr, (u32)r0, (u32)r1, kChunk, skip);
           return upload_sched(sc);
         };
         ps.s_loc = range(f.loc_ptr, (int64_t)s0, (int64_t)s1);
         ps.s_red = range(f.red_ptr, (int64_t)s0, (int64_t)s1);
         ps.s_keep = range(f.keep_ptr, ps.red0, ps.red1);
-        ps.s_elim = range(f.elim_ptr, elim_ptr_sub[s0], elim_ptr_sub[s1]);
+        ps.s_elim = range(f.elim_ptr, elim_ptr_sub[s0], elim_ptr_sub[s1], nullptr, kShortElim);
         ps.s_core = range(core_row_ptr, ps.core0, ps.core1, &big_row);
         for (size_t bi = 0; bi < pre64.big.size(); ++bi)
           if (pre64.big[bi].b >= ps.core0 && pre64.big[bi].b < ps.core1) ps.big.push_back(bi);
