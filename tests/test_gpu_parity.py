@@ -209,7 +209,8 @@ def test_multiplicative_apply_matches_oracle(idx, oracle_mod):
 
 
 @pytest.mark.parametrize("idx", [1, 2, 3], ids=["3fe-weighted", "akm-3fe", "akm-4fe-weighted"])
-@pytest.mark.parametrize("kind", ["gmres-additive", "gmres-multiplicative", "gmres-mixed", "cg-explicit"])
+@pytest.mark.parametrize("kind", ["gmres-additive", "gmres-multiplicative", "gmres-mixed", "cg-explicit",
+                                  "cg-mixed-explicit"])
 def test_other_solver_paths_match_oracle(idx, kind, oracle_mod):
     codes, w, _ = PROBS[idx]
     n = codes.shape[0]
@@ -222,12 +223,19 @@ def test_other_solver_paths_match_oracle(idx, kind, oracle_mod):
         cfg, prec = W.GMRES(tol=1e-10, max_refinements=3), W.MultiplicativeSchwarz()
     elif kind == "gmres-mixed":
         cfg, prec = W.GMRES(tol=1e-10, max_refinements=3, precision="mixed"), None
-    else:
+    elif kind == "cg-explicit":
         cfg, prec = W.CG(tol=1e-10, max_refinements=3, operator=W.OperatorRepr.Explicit), None
+    else:  # residuals b - Gx from the FP64 Gramian, demeaned written from Y
+        cfg = W.CG(tol=1e-10, max_refinements=3, operator=W.OperatorRepr.Explicit, precision="mixed")
+        prec = None
     r = W.solve_batch(codes, Y, config=cfg, weights=w, preconditioner=prec)
     assert all(r.converged)
     for j in range(3):
         assert rel(r.demeaned[:, j], ro["demeaned"][:, j]) <= 1e-8
+        # the reported residual is the observation-space one, to FP64 rounding
+        res = o.rmatvec_wdt(r.demeaned[:, j])
+        b = o.rmatvec_wdt(Y[:, j])
+        assert abs(r.residual[j] - np.linalg.norm(res) / np.linalg.norm(b)) <= 1e-12 + 1e-3 * r.residual[j]
 
 
 def test_gmres_iterations_track_oracle(oracle_mod):

@@ -169,6 +169,38 @@ void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, cons
                                                                                                X, Y, E, kp)));
 }
 
+// D[i + c ldd] = Y[i + c ldy] - sum_q X[off_q + code_q(i)] for c < k: the
+// demeaning written straight into column-major output (same arithmetic as
+// k_demean), 64 rows per workgroup through an LDS tile
+__global__ void __launch_bounds__(kBlock) k_demean_cols(int64_t n, int Q, const uint32_t* __restrict__ codes,
+                                                        const uint32_t* __restrict__ offs,
+                                                        const double* __restrict__ X, const double* __restrict__ Y,
+                                                        int64_t ldy, double* __restrict__ D, int64_t ldd, int k,
+                                                        int kp) {
+  __shared__ double tile[32][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  for (int t = threadIdx.x; t < 64 * kp; t += kBlock) {
+    const int c = t % kp, r = t / kp;
+    const int64_t i = r0 + r;
+    double s = 0.0;
+    if (i < n)
+      for (int q = 0; q < Q; ++q) s += X[(int64_t)(offs[q] + codes[(int64_t)q * n + i]) * kp + c];
+    tile[c][r] = s;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 64 * 32; t += kBlock) {
+    const int r = t & 63, c = t >> 6;
+    const int64_t i = r0 + r;
+    if (c < k && i < n) D[i + c * ldd] = Y[i + c * ldy] - tile[c][r];
+  }
+}
+
+void launch_demean_cols(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
+                        const double* X, const double* Y, int64_t ldy, double* D, int64_t ldd, int k, int kp) {
+  if (n > 0)
+    k_demean_cols<<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(n, Q, codes, offs, X, Y, ldy, D, ldd, k, kp);
+}
+
 // per-block column sums of a thread's NV accumulators (sh: kBlock * NV)
 template <int NV>
 __device__ inline void block_col_sums(double (&acc)[NV], double* sh, int kp, int gpb, double* out) {

@@ -106,6 +106,9 @@ void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* 
                         double* part, int kp);
 void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
                    const double* X, const double* Y, double* E, int kp);
+// the same demeaning from column-major Y straight into column-major D (k columns)
+void launch_demean_cols(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
+                        const double* X, const double* Y, int64_t ldy, double* D, int64_t ldd, int k, int kp);
 // column-major (ld) <-> row-major kp blocks; with `map`, row i of the
 // destination comes from row map[i] of the source (UINT32_MAX: zeros)
 void launch_col_to_row(hipStream_t st, const double* src, int64_t ld, int64_t n, int k, double* dst, int kp,

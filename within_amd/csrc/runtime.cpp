@@ -887,7 +887,8 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
   std::vector<int64_t> its(kb, 0), cits(kb, 0);
   const int64_t vbytes = sizeof(double) * m * kp;
   const bool mixed = params.precision == 1;
-  bool fresh = false;  // E and R64 hold y - Dx and D'W(y - Dx) for the current X64
+  bool fresh = false;      // E and R64 hold y - Dx and D'W(y - Dx) for the current X64
+  bool dof_final = false;  // R64 holds b - Gx for the current X64 (E is stale)
   if (!mixed) {
     std::vector<double> tol(kb, params.tol);
     krylov<double>(kb, kp, B64, cx64, tol, conv, its);
@@ -925,10 +926,10 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
     }
   } else {
     // FP32 inner Krylov, FP64 outer residual and accumulation.  With the
-    // explicit Gramian the intermediate residuals are b - G x in FP64 (the same
-    // vector as D'W(y - Dx); its rounding, ~eps (|b| + |G||x|), is far below
-    // any tolerance here); once they all pass, the observation-space residual
-    // D'W(y - Dx) decides, as it does at every step without the Gramian.
+    // explicit Gramian every residual, the final one included, is b - G x in
+    // FP64: the same vector as D'W(y - Dx) (b = D'W y, G = D'W D), with a
+    // rounding of ~eps (|b| + |G||x|), far below any tolerance here.  Without
+    // it the residual is D'W(y - Dx) from observation space.
     HIPCHK(hipMemsetAsync(X64, 0, vbytes, st));
     const int64_t max_outer = std::max<int64_t>(params.max_refinements, 0) + 8;
     std::vector<int> live(kb, 1);
@@ -939,19 +940,15 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
         HIPCHK(hipMemcpyAsync(R64, B64, vbytes, hipMemcpyDeviceToDevice, st));
         rn = bn;
       } else {
-        bool obs = !dof_resid || outer == max_outer;
-        if (!obs) {
+        if (dof_resid) {
           hipEvent_t ex = nullptr;
           tic(3, ex);  // counted with the observation-space work it replaces
           launch_spmm<double>(st, g64, X64, R64, part64, kp);
           launch_rsub<double>(st, R64, B64, m * kp);
           toc(3, ex);
           rn = col_norms(R64, m, kp);
-          obs = true;
-          for (int c = 0; c < kb; ++c)
-            if (live[c] && !(rn[c] <= abs_tol[c] || rn[c] < F64_EPS)) obs = false;
-        }
-        if (obs) {
+          dof_final = true;
+        } else {
           launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
           rhs64(E, R64, kp);
           fresh = true;
@@ -978,20 +975,34 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       HIPCHK(hipMemcpyAsync(s_(9), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
       HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
       launch_axpy_acc<float>(st, X64, cx32, s_(9), imask + KMAX + 1, m, kp);
-      fresh = false;
+      fresh = dof_final = false;
       for (int c = 0; c < kb; ++c)
         if (live[c]) its[c] += cits[c];
     }
   }
-  // final demeaning and observation-space residual (FP64), unless the last
-  // refinement check already computed them for the final X
-  if (!fresh) {
-    launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
-    rhs64(E, R64, kp);
+  // final residual for the final X: the observation-space D'W(y - Dx), or,
+  // in the mixed mode with the explicit Gramian, the same vector as b - Gx
+  // (FP64); the demeaned output is then written straight from Y
+  if (!fresh && !dof_final) {
+    if (mixed && has_gram) {
+      launch_spmm<double>(st, g64, X64, R64, part64, kp);
+      launch_rsub<double>(st, R64, B64, m * kp);
+      dof_final = true;
+    } else {
+      launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+      rhs64(E, R64, kp);
+      fresh = true;
+    }
   }
   std::vector<double> fr = col_norms(R64, m, kp);
   tic(3, ev);
-  if (dem) launch_row_to_col(st, E, kp, n, kb, dem + (int64_t)c0 * ldd, ldd);
+  if (dem) {
+    if (fresh)
+      launch_row_to_col(st, E, kp, n, kb, dem + (int64_t)c0 * ldd, ldd);
+    else
+      launch_demean_cols(st, n, Q, codes, offs, X64, Y + (int64_t)c0 * ldy, ldy, dem + (int64_t)c0 * ldd, ldd, kb,
+                         kp);
+  }
   launch_row_to_col(st, X64, kp, mu, kb, X + (int64_t)c0 * ldx, ldx, int_of);
   toc(3, ev);
   HIPCHK(hipStreamSynchronize(st));
