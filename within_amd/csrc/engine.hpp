@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace wa {
@@ -41,6 +43,13 @@ template <typename T, int NV>
 __device__ __forceinline__ void vstore(T* p, const VecT<T, NV>& v) {
   *reinterpret_cast<VecT<T, NV>*>(p) = v;
 }
+
+// A kind with EMIT = true also receives each element's term coef * X[id]
+// (emit(e, col, v)), e.g. to write the restricted vector while summing it.
+template <class K, class = void>
+struct emits : std::false_type {};
+template <class K>
+struct emits<K, std::void_t<decltype(K::EMIT)>> : std::integral_constant<bool, K::EMIT> {};
 
 template <typename T, int NV>
 __device__ __forceinline__ void vstore(T* p, const T (&a)[NV]) {
@@ -82,14 +91,19 @@ __device__ __forceinline__ void gather_row(const K& k, uint32_t row, uint32_t e0
         }
       }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) {
+      VecT<T, NV> term;
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
         T s = 0;
 #pragma unroll
         for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
-        if (e + u < e1) acc[c] += K::COEF ? cf[u] * s : s;
+        term.v[c] = K::COEF ? cf[u] * s : s;
+        if (e + u < e1) acc[c] += term.v[c];
       }
+      if constexpr (emits<K>::value)
+        if (e + u < e1) k.template emit<NV>(e + u, part * NV, term);
+    }
   }
 }
 
@@ -127,14 +141,19 @@ __device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0
         }
       }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) {
+      VecT<T, NV> term;
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
         T s = 0;
 #pragma unroll
         for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
-        if (e + u * S < e1) acc[c] += K::COEF ? cf[u] * s : s;
+        term.v[c] = K::COEF ? cf[u] * s : s;
+        if (e + u * S < e1) acc[c] += term.v[c];
       }
+      if constexpr (emits<K>::value)
+        if (e + u * S < e1) k.template emit<NV>(e + u * S, part * NV, term);
+    }
   }
   // combine the S slots (lanes V apart) in a fixed order
 #pragma unroll

@@ -169,38 +169,6 @@ void launch_demean(hipStream_t st, int64_t n, int Q, const uint32_t* codes, cons
                                                                                                X, Y, E, kp)));
 }
 
-// D[i + c ldd] = Y[i + c ldy] - sum_q X[off_q + code_q(i)] for c < k: the
-// demeaning written straight into column-major output (same arithmetic as
-// k_demean), 64 rows per workgroup through an LDS tile
-__global__ void __launch_bounds__(kBlock) k_demean_cols(int64_t n, int Q, const uint32_t* __restrict__ codes,
-                                                        const uint32_t* __restrict__ offs,
-                                                        const double* __restrict__ X, const double* __restrict__ Y,
-                                                        int64_t ldy, double* __restrict__ D, int64_t ldd, int k,
-                                                        int kp) {
-  __shared__ double tile[32][65];
-  const int64_t r0 = (int64_t)blockIdx.x * 64;
-  for (int t = threadIdx.x; t < 64 * kp; t += kBlock) {
-    const int c = t % kp, r = t / kp;
-    const int64_t i = r0 + r;
-    double s = 0.0;
-    if (i < n)
-      for (int q = 0; q < Q; ++q) s += X[(int64_t)(offs[q] + codes[(int64_t)q * n + i]) * kp + c];
-    tile[c][r] = s;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < 64 * 32; t += kBlock) {
-    const int r = t & 63, c = t >> 6;
-    const int64_t i = r0 + r;
-    if (c < k && i < n) D[i + c * ldd] = Y[i + c * ldy] - tile[c][r];
-  }
-}
-
-void launch_demean_cols(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
-                        const double* X, const double* Y, int64_t ldy, double* D, int64_t ldd, int k, int kp) {
-  if (n > 0)
-    k_demean_cols<<<(unsigned)((n + 63) / 64), kBlock, 0, st>>>(n, Q, codes, offs, X, Y, ldy, D, ldd, k, kp);
-}
-
 // per-block column sums of a thread's NV accumulators (sh: kBlock * NV)
 template <int NV>
 __device__ inline void block_col_sums(double (&acc)[NV], double* sh, int kp, int gpb, double* out) {
@@ -247,6 +215,49 @@ __global__ void __launch_bounds__(kBlock) k_dot_fin(const double* __restrict__ p
     __syncthreads();
   }
   if (s0 == 0) out[col] = sh[threadIdx.x];
+}
+
+// D[i + c ldd] = Y[i + c ldy] - sum_q X[off_q + code_q(i)] for c < k: the
+// demeaning written straight into column-major output (same arithmetic as
+// k_demean).  One lane per row, so the column-major reads and writes are
+// coalesced along rows; the row's sums are formed 8 columns at a time.
+__global__ void __launch_bounds__(kBlock) k_demean_cols(int64_t n, int Q, const uint32_t* __restrict__ codes,
+                                                        const uint32_t* __restrict__ offs,
+                                                        const double* __restrict__ X, const double* __restrict__ Y,
+                                                        int64_t ldy, double* __restrict__ D, int64_t ldd, int k,
+                                                        int kp) {
+  constexpr int CB = 8, QMAX = 8;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    int64_t row[QMAX];
+    for (int q = 0; q < Q; ++q) row[q] = (int64_t)(offs[q] + codes[(int64_t)q * n + i]) * kp;
+    for (int c0 = 0; c0 < k; c0 += CB) {
+      double s[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) s[c] = 0.0;
+      const int w = min(CB, kp - c0);
+      for (int q = 0; q < Q; ++q) {
+        if (w == CB) {
+#pragma unroll
+          for (int h = 0; h < CB / 2; ++h) {
+            const VecT<double, 2> x = vload<double, 2>(X + row[q] + c0 + 2 * h);
+            s[2 * h] += x.v[0];
+            s[2 * h + 1] += x.v[1];
+          }
+        } else {
+          for (int c = 0; c < w; ++c) s[c] += X[row[q] + c0 + c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CB; ++c)
+        if (c0 + c < k) D[i + (int64_t)(c0 + c) * ldd] = Y[i + (int64_t)(c0 + c) * ldy] - s[c];
+    }
+  }
+}
+
+void launch_demean_cols(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const uint32_t* offs,
+                        const double* X, const double* Y, int64_t ldy, double* D, int64_t ldd, int k, int kp) {
+  if (n <= 0) return;
+  k_demean_cols<<<grid_stride_blocks(n, 1), kBlock, 0, st>>>(n, Q, codes, offs, X, Y, ldy, D, ldd, k, kp);
 }
 
 template <typename T>
@@ -406,9 +417,10 @@ __global__ void k_cg_start(ColState* __restrict__ s, const double* __restrict__ 
 
 // Column totals of per-block partials (part[b * kp + c], b < nb): threads
 // stride over the blocks, then a fixed-order tree; the result is valid in
-// threads [0, kp).  Launched with kBlock threads.
+// threads [0, kp).  Launched with kScalarBlock threads (more loads in flight).
+constexpr int kScalarBlock = 1024;
 __device__ inline double col_total(const double* __restrict__ part, int nb, int kp, double* sh) {
-  const int col = threadIdx.x % kp, str = kBlock / kp, s0 = threadIdx.x / kp;
+  const int col = threadIdx.x % kp, str = kScalarBlock / kp, s0 = threadIdx.x / kp;
   double acc = 0.0;
   for (int b = s0; b < nb; b += str) acc += part[(int64_t)b * kp + col];
   sh[threadIdx.x] = acc;
@@ -421,10 +433,10 @@ __device__ inline double col_total(const double* __restrict__ part, int nb, int 
 }
 
 // after A p: pap <= 0 -> stop (not converged); else alpha = rz / pap
-__global__ void __launch_bounds__(kBlock) k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ part,
+__global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ part,
                                                      int nb, double* __restrict__ pap, double* __restrict__ alpha,
                                                      int itn, int kp) {
-  __shared__ double sh[kBlock];
+  __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c >= kp) return;
@@ -443,9 +455,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_alpha(ColState* __restrict__ s, c
 }
 
 // after the update: ||r|| / ||b|| <= tol -> converged
-__global__ void __launch_bounds__(kBlock) k_cg_check(ColState* __restrict__ s, const double* __restrict__ part,
+__global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict__ s, const double* __restrict__ part,
                                                      int nb, double* __restrict__ rr, int itn, int kp) {
-  __shared__ double sh[kBlock];
+  __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c >= kp) return;
@@ -461,11 +473,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_check(ColState* __restrict__ s, c
 }
 
 // after z = M r: stagnation guard |rz_new| < EPS^2 max(|rz0|, MIN); beta
-__global__ void __launch_bounds__(kBlock) k_cg_beta(ColState* __restrict__ s, const double* __restrict__ part,
+__global__ void __launch_bounds__(kScalarBlock) k_cg_beta(ColState* __restrict__ s, const double* __restrict__ part,
                                                     int nb, double* __restrict__ rznew, double* __restrict__ beta,
                                                     int* __restrict__ mask, int* __restrict__ n_active, int itn,
                                                     int kp) {
-  __shared__ double sh[kBlock];
+  __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
   if (c < kp) {
@@ -500,38 +512,17 @@ void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const do
 }
 void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int itn,
                      int kp) {
-  k_cg_alpha<<<1, kBlock, 0, st>>>(s, part, nb, pap, alpha, itn, kp);
+  k_cg_alpha<<<1, kScalarBlock, 0, st>>>(s, part, nb, pap, alpha, itn, kp);
 }
 void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int itn, int kp) {
-  k_cg_check<<<1, kBlock, 0, st>>>(s, part, nb, rr, itn, kp);
+  k_cg_check<<<1, kScalarBlock, 0, st>>>(s, part, nb, rr, itn, kp);
 }
 void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, double* rznew, double* beta, int* mask,
                     int* n_active, int itn, int kp) {
-  k_cg_beta<<<1, kBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, itn, kp);
+  k_cg_beta<<<1, kScalarBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, itn, kp);
 }
 
 // Preconditioner elementwise kernels (same lane mapping as the vector ops).
-
-template <typename T, int NV>
-__global__ void __launch_bounds__(kBlock) k_restrict(const uint32_t* __restrict__ gidx,
-                                                     const T* __restrict__ scale, const T* __restrict__ R,
-                                                     T* __restrict__ B, int64_t b0, int64_t n, int kp) {
-  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
-  for (int64_t l = b0 + (int64_t)blockIdx.x * gpb + threadIdx.x / V; l < n; l += (int64_t)gridDim.x * gpb) {
-    const T sc = scale[l];
-    VecT<T, NV> x = vload<T, NV>(R + (int64_t)gidx[l] * kp + pt * NV);
-#pragma unroll
-    for (int c = 0; c < NV; ++c) x.v[c] = sc * x.v[c];
-    vstore<T, NV>(B + l * kp + pt * NV, x);
-  }
-}
-template <typename T>
-static void launch_restrict(hipStream_t st, const uint32_t* gidx, const T* scale, const T* R, T* B, int64_t b0,
-                            int64_t n, int kp) {
-  if (n <= b0) return;
-  WA_VEC_SWITCH(T, kp, (k_restrict<T, NV><<<grid_stride_blocks(n - b0, kp / NV), kBlock, 0, st>>>(gidx, scale, R, B,
-                                                                                                   b0, n, kp)));
-}
 
 // Rr[j] -= sum2[s] / n_red(s)
 template <typename T, int NV>
@@ -762,6 +753,39 @@ struct SegKind {
   __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
   __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
   __device__ void load(uint32_t, uint32_t e, uint32_t* id, T&) const { id[0] = e; }
+  template <int NV>
+  struct Pre {};
+  template <int NV>
+  __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
+  template <int NV>
+  __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>&) const {
+    vstore<T, NV>(out + (int64_t)r * kp + col, acc);
+  }
+};
+
+// restriction fused with its per-subdomain sums (mu1): element l of row s
+// (subdomain s) is B[l] = scale_l R[gidx_l], written as it is summed
+template <typename T>
+struct RestrictSegKind {
+  static constexpr int QM = 1;
+  static constexpr bool COEF = true;
+  static constexpr bool EMIT = true;
+  const T* X;  // R
+  T* B;
+  T* out;
+  const uint32_t *ptr, *gidx;
+  const T* scale;
+  int kp;
+  __device__ uint32_t rbegin(uint32_t r) const { return ptr[r]; }
+  __device__ uint32_t rend(uint32_t r) const { return ptr[r + 1]; }
+  __device__ void load(uint32_t, uint32_t e, uint32_t* id, T& c) const {
+    id[0] = gidx[e];
+    c = scale[e];
+  }
+  template <int NV>
+  __device__ void emit(uint32_t e, int col, const VecT<T, NV>& v) const {
+    vstore<T, NV>(B + (int64_t)e * kp + col, v);
+  }
   template <int NV>
   struct Pre {};
   template <int NV>
@@ -1147,8 +1171,8 @@ template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m) {
   // 1. restrict (signed, partition-weighted) and per-subdomain sums -> mu1
-  launch_restrict<T>(st, p.loc_gidx, p.loc_scale, R, w.B, 0, p.n_loc, kp);
-  run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, p.s_loc, w.part, kp);
+  run_rows<RestrictSegKind<T>, T>(st, RestrictSegKind<T>{R, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp},
+                                  p.s_loc, w.part, kp);
   if (p.n_red > 0) {
     // 2. Schur right-hand side, centred (mu2)
     run_rows<KeepKind<T>, T>(st,
@@ -1286,8 +1310,8 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
   (void)hipMemsetAsync(Z, 0, vbytes, st);
   for (const PairSched& ps : pairs) {
     const int64_t nl = ps.loc1 - ps.loc0;
-    launch_restrict<T>(st, p.loc_gidx, p.loc_scale, rwork, w.B, ps.loc0, ps.loc1, kp);
-    run_rows<SegKind<T>, T>(st, SegKind<T>{w.B, w.sum1, p.loc_ptr, kp}, ps.s_loc, w.part, kp);
+    run_rows<RestrictSegKind<T>, T>(
+        st, RestrictSegKind<T>{rwork, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp}, ps.s_loc, w.part, kp);
     const int64_t nr = ps.red1 - ps.red0;
     if (nr > 0) {
       run_rows<KeepKind<T>, T>(st,
