@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -162,17 +163,16 @@ __device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0
     for (int c = 0; c < NV; ++c) acc[c] += __shfl_xor(acc[c], off, kWave);
 }
 
-// Waves [0, n_sw) sum the short rows of [r0, r1), S rows per wave; the
-// remaining waves run one work item each.
+// Work of virtual wave g over rows [r0, r1): waves [0, n_sw) sum the short
+// rows, S per wave; the remaining waves run one work item each.
 template <class K, typename T, int V, int NV>
-__global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, uint32_t short_max, int64_t n_sw,
-                                                 const Item* __restrict__ items, int64_t n_items,
-                                                 T* __restrict__ part) {
+__device__ __forceinline__ void rows_wave(const K& k, int64_t g, uint32_t r0, uint32_t r1, uint32_t short_max,
+                                          int64_t n_sw, const Item* __restrict__ items, int64_t n_items,
+                                          T* __restrict__ part) {
   constexpr int S = kWave / V;
   constexpr int KP = V * NV;
   const int lane = threadIdx.x & (kWave - 1);
   const int slot = lane / V, pt = lane % V;
-  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   T acc[NV];
   if (g < n_sw) {
     const int64_t r = (int64_t)r0 + g * S + slot;
@@ -201,14 +201,19 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
   }
 }
 
-// ---- fix-up: one workgroup per split row; threads stride over the slots
-// with four independent sums, then a fixed-order tree over the workgroup -----
-template <class K, typename T>
-__global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __restrict__ fx,
-                                                        const T* __restrict__ part, int kp) {
-  __shared__ T sh[kFixBlock];
-  const Fixup f = fx[blockIdx.x];
-  const int col = threadIdx.x % kp, str = kFixBlock / kp, s0 = threadIdx.x / kp;
+template <class K, typename T, int V, int NV>
+__global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, uint32_t short_max, int64_t n_sw,
+                                                 const Item* __restrict__ items, int64_t n_items,
+                                                 T* __restrict__ part) {
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  rows_wave<K, T, V, NV>(k, g, r0, r1, short_max, n_sw, items, n_items, part);
+}
+
+// ---- fix-up of one split row by a workgroup of NT threads: threads stride
+// over the slots with four independent sums, then a fixed-order tree -------
+template <class K, typename T, int NT>
+__device__ __forceinline__ void fix_row(const K& k, const Fixup& f, const T* __restrict__ part, int kp, T* sh) {
+  const int col = threadIdx.x % kp, str = NT / kp, s0 = threadIdx.x / kp;
   T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   uint32_t s = f.slot0 + s0;
   for (; s + 3 * str < f.slot1; s += 4 * str) {
@@ -228,6 +233,14 @@ __global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __rest
     const T a[1] = {sh[threadIdx.x]};
     k.template finish<1>(f.row, col, a, k.template pre<1>(f.row, col));
   }
+  __syncthreads();  // sh is reused by the next call
+}
+
+template <class K, typename T>
+__global__ void __launch_bounds__(kFixBlock) k_rows_fix(K k, const Fixup* __restrict__ fx,
+                                                        const T* __restrict__ part, int kp) {
+  __shared__ T sh[kFixBlock];
+  fix_row<K, T, kFixBlock>(k, fx[blockIdx.x], part, kp, sh);
 }
 
 inline unsigned engine_blocks(int64_t waves) { return (unsigned)((waves * kWave + kBlock - 1) / kBlock); }
@@ -271,6 +284,15 @@ void run_seg(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T
 template <class K, typename T>
 void run_rows(hipStream_t st, const K& k, const DevSched& sc, T* part, int kp) {
   for (const Segment& g : sc.segs) run_seg<K, T>(st, k, sc, g, part, kp);
+}
+
+// Levels t0..t1-1 of a level schedule, ascending when dir > 0 (one launch
+// per level: a grid-wide barrier inside one cooperative launch costs more than
+// a launch boundary here).
+template <class K, typename T>
+void run_levels(hipStream_t st, const K& k, const DevSched& sc, int t0, int t1, int dir, T* part, int kp) {
+  for (int step = 0; step < t1 - t0; ++step)
+    run_seg<K, T>(st, k, sc, sc.segs[dir > 0 ? t0 + step : t1 - 1 - step], part, kp);
 }
 
 }  // namespace wa

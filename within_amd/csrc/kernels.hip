@@ -1183,7 +1183,7 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
     launch_sub_mean<T>(st, 0, p.n_red, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
     // 3. reduced solve: forward levels, dense cores, backward levels
     FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-    for (const Segment& g : p.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, p.s_fwd, g, w.part, kp);
+    run_levels<FwdKind<T>, T>(st, fk, p.s_fwd, 0, (int)p.s_fwd.segs.size(), 1, w.part, kp);
     if (p.n_core_rows > 0) {
       run_rows<CoreKind<T>, T>(st,
                                CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
@@ -1194,8 +1194,7 @@ void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, Pr
     }
     BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
     // the last level gathers the dense cores: no backward step
-    for (int64_t t = (int64_t)p.s_bwd.segs.size() - 2; t >= 0; --t)
-      run_seg<BwdKind<T>, T>(st, bk, p.s_bwd, p.s_bwd.segs[t], w.part, kp);
+    run_levels<BwdKind<T>, T>(st, bk, p.s_bwd, 0, (int)p.s_bwd.segs.size() - 1, -1, w.part, kp);
     // 4. kept block copied into the local solution
     launch_scatter_rows<T>(st, 0, p.n_red, p.red_loc, w.Rr, w.S, kp);
   }
@@ -1321,7 +1320,7 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
       run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, w.part, kp);
       launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
       FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-      for (const Segment& sg : ps.s_fwd.segs) run_seg<FwdKind<T>, T>(st, fk, ps.s_fwd, sg, w.part, kp);
+      run_levels<FwdKind<T>, T>(st, fk, ps.s_fwd, 0, (int)ps.s_fwd.segs.size(), 1, w.part, kp);
       if (ps.core1 > ps.core0) {
         run_rows<CoreKind<T>, T>(st,
                                  CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
@@ -1331,8 +1330,7 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
         launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
       }
       BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
-      for (int64_t t = (int64_t)ps.s_bwd.segs.size() - 2; t >= 0; --t)
-        run_seg<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.s_bwd.segs[t], w.part, kp);
+      run_levels<BwdKind<T>, T>(st, bk, ps.s_bwd, 0, (int)ps.s_bwd.segs.size() - 1, -1, w.part, kp);
       launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
     }
     run_rows<ElimKind<T>, T>(st,
