@@ -399,6 +399,21 @@ static void sample_clique_tree(const std::vector<std::pair<u32, double>>& nbrs, 
   }
 }
 
+// y[j] -= c * x[j] and y[j] += c * x[j] over a range: the dense-core inner
+// loops, built for AVX2 as well as the baseline ISA (chosen at load time).
+// Either build does one multiply and one add or subtract per element (no
+// contraction), so the results are the same bit for bit.
+__attribute__((target_clones("avx2", "default"))) static void axpy_sub(double* __restrict y,
+                                                                       const double* __restrict x, double c,
+                                                                       i64 n) {
+  for (i64 j = 0; j < n; ++j) y[j] -= c * x[j];
+}
+__attribute__((target_clones("avx2", "default"))) static void axpy_add(double* __restrict y,
+                                                                       const double* __restrict x, double c,
+                                                                       i64 n) {
+  for (i64 j = 0; j < n; ++j) y[j] += c * x[j];
+}
+
 // Cholesky of the anchored minor (drop the last index) of a dense n x n
 // Laplacian-like matrix, then its inverse.  Entry by entry the arithmetic is
 // fixed:
@@ -454,7 +469,7 @@ std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n) {
         for (i64 kk = 0; kk < k1 - k0; ++kk) {
           const double c = li[k0 + kk];
           const double* __restrict p = &pt[kk * m];
-          for (i64 j = j0; j < j1; ++j) li[j] -= c * p[j];
+          axpy_sub(li + j0, p + j0, c, j1 - j0);
         }
       }
     }
@@ -479,7 +494,7 @@ std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n) {
         for (i64 k = j0; k < i; ++k) {
           const double c = li[k];
           const double* __restrict uk = &ub[(k - j0) * jb];
-          for (i64 jj = 0; jj < jb; ++jj) sp[jj] -= c * uk[jj];
+          axpy_sub(sp, uk, c, jb);
         }
         const double lii = li[i];
         double* ui = &ub[(i - j0) * jb];
@@ -504,7 +519,7 @@ std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n) {
         for (i64 i = i0; i < std::min(i1, k + 1); ++i) {
           const double c = uk[i];
           double* __restrict ai = &acc[(i - i0) * i1];
-          for (i64 j = 0; j <= i; ++j) ai[j] += c * uk[j];
+          axpy_add(ai, uk, c, i + 1);
         }
       }
       for (i64 i = i0; i < i1; ++i)
@@ -771,37 +786,57 @@ Csr build_gramian(const Design& d) {
   Csr g;
   g.nrows = g.ncols = d.n_dofs;
   g.indptr.assign(d.n_dofs + 1, 0);
-  std::vector<std::vector<std::pair<u32, double>>> rows(d.n_dofs);
+  // per factor: observations grouped by level, in observation order
+  std::vector<std::vector<u32>> ptr(d.Q), ob(d.Q);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int q = 0; q < d.Q; ++q) {
-    std::vector<u32> ptr(d.levels[q] + 1, 0);
-    for (i64 i = 0; i < d.n_obs; ++i) ptr[d.codes[q][i] + 1]++;
-    for (i64 j = 0; j < d.levels[q]; ++j) ptr[j + 1] += ptr[j];
-    std::vector<u32> ob(d.n_obs);
-    {
-      std::vector<u32> cur(ptr.begin(), ptr.end() - 1);
-      for (i64 i = 0; i < d.n_obs; ++i) ob[cur[d.codes[q][i]]++] = (u32)i;
-    }
-    std::map<u32, double> acc;
-    for (i64 j = 0; j < d.levels[q]; ++j) {
-      if (ptr[j] == ptr[j + 1]) continue;
-      u32 self = (u32)(d.offset[q] + j);
-      acc.clear();
-      for (u32 t = ptr[j]; t < ptr[j + 1]; ++t) {
-        u32 i = ob[t];
-        double w = d.w(i);
-        acc[self] += w;
+    ptr[q].assign(d.levels[q] + 1, 0);
+    for (i64 i = 0; i < d.n_obs; ++i) ptr[q][d.codes[q][i] + 1]++;
+    for (i64 j = 0; j < d.levels[q]; ++j) ptr[q][j + 1] += ptr[q][j];
+    ob[q].resize(d.n_obs);
+    std::vector<u32> cur(ptr[q].begin(), ptr[q].end() - 1);
+    for (i64 i = 0; i < d.n_obs; ++i) ob[q][cur[d.codes[q][i]]++] = (u32)i;
+  }
+  // one row per DOF: its weight sums with every DOF it shares an observation
+  // with, accumulated in observation order, columns ascending
+  std::vector<std::vector<std::pair<u32, double>>> rows(d.n_dofs);
+#pragma omp parallel
+  {
+    std::vector<double> acc(d.n_dofs, 0.0);
+    std::vector<char> hit(d.n_dofs, 0);
+    std::vector<u32> touched;
+    auto add = [&](u32 k, double w) {
+      if (!hit[k]) { hit[k] = 1; touched.push_back(k); }
+      acc[k] += w;
+    };
+#pragma omp for schedule(dynamic, 64)
+    for (i64 a = 0; a < d.n_dofs; ++a) {
+      int q = 0;
+      while (q + 1 < d.Q && d.offset[q + 1] <= a) ++q;
+      const i64 j = a - d.offset[q];
+      if (ptr[q][j] == ptr[q][j + 1]) continue;
+      touched.clear();
+      for (u32 t = ptr[q][j]; t < ptr[q][j + 1]; ++t) {
+        const u32 i = ob[q][t];
+        const double w = d.w(i);
+        add((u32)a, w);
         for (int r = 0; r < d.Q; ++r)
-          if (r != q) acc[(u32)(d.offset[r] + d.codes[r][i])] += w;
+          if (r != q) add((u32)(d.offset[r] + d.codes[r][i]), w);
       }
-      auto& row = rows[self];
-      for (const auto& kv : acc)
-        if (kv.second != 0.0 || kv.first == self) row.push_back(kv);
+      std::sort(touched.begin(), touched.end());
+      auto& row = rows[a];
+      row.reserve(touched.size());
+      for (u32 k : touched) {
+        if (acc[k] != 0.0 || k == (u32)a) row.push_back({k, acc[k]});
+        acc[k] = 0.0;
+        hit[k] = 0;
+      }
     }
   }
   for (i64 a = 0; a < d.n_dofs; ++a) g.indptr[a + 1] = g.indptr[a] + (u32)rows[a].size();
   g.indices.resize(g.indptr[d.n_dofs]);
   g.data.resize(g.indptr[d.n_dofs]);
+#pragma omp parallel for schedule(static)
   for (i64 a = 0; a < d.n_dofs; ++a) {
     u32 p = g.indptr[a];
     for (const auto& kv : rows[a]) { g.indices[p] = kv.first; g.data[p] = kv.second; ++p; }
