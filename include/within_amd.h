@@ -132,6 +132,25 @@ wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int
 wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx,
                          int32_t k);
 
+/* ---- serialized preconditioners (pickling of FePreconditioner) -----------
+ * Replaces the postcard bytes of FePreconditioner.__reduce__ / FePreconditioner
+ * (data) (crates/within-py/src/lib.rs:892-913) and Solver(categories,
+ * preconditioner=p) (lib.rs:925-982) with this library's own format (not
+ * byte-compatible with the reference's).
+ * wa_solver_precond_bytes: buf == NULL -> *len = size; else copies (needs
+ * *len >= size) and sets *len.  wa_solver_create_with_precond: a solver for the
+ * design whose preconditioner (kind, local config, factors) comes from the
+ * bytes; no subdomain setup runs.  wa_precond_create: the preconditioner on
+ * its own (no design): only wa_precond_apply, wa_solver_info,
+ * wa_solver_precond_bytes and wa_solver_params work on it. */
+wa_status wa_solver_precond_bytes(const wa_solver* s, uint8_t* buf, int64_t* len);
+wa_status wa_solver_create_with_precond(const wa_design* d, const wa_params* p, const uint8_t* bytes,
+                                        int64_t len, wa_solver** out);
+wa_status wa_precond_create(const uint8_t* bytes, int64_t len, int32_t host_only, wa_solver** out);
+/* The effective parameters of a built solver (the preconditioner kind and
+ * local config come from the bytes when it was built from them). */
+wa_status wa_solver_params(const wa_solver* s, wa_params* out);
+
 /* ---- setup introspection (host data; no GPU needed) ------------------------
  * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
  * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,

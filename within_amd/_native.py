@@ -61,7 +61,8 @@ EXPORTS = [
     "wa_design_n_obs", "wa_design_n_factors", "wa_design_n_levels", "wa_solver_create", "wa_solver_destroy", "wa_solver_set_params",
     "wa_solve_batch", "wa_solve_batch_device", "wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply",
     "wa_matvec_d", "wa_rmatvec_wdt", "wa_solver_info", "wa_solver_subdomain", "wa_solver_subdomain_factor",
-    "wa_timing_enable", "wa_timing_reset", "wa_timing_get",
+    "wa_timing_enable", "wa_timing_reset", "wa_timing_get", "wa_solver_precond_bytes",
+    "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params",
 ]
 
 _lib = None
@@ -110,6 +111,10 @@ def load():
     L.wa_timing_reset.argtypes = [P]
     L.wa_timing_reset.restype = None
     L.wa_timing_get.argtypes = [P, i32, P, P]
+    L.wa_solver_precond_bytes.argtypes = [P, P, C.POINTER(i64)]
+    L.wa_solver_create_with_precond.argtypes = [P, C.POINTER(Params), P, i64, C.POINTER(P)]
+    L.wa_precond_create.argtypes = [P, i64, i32, C.POINTER(P)]
+    L.wa_solver_params.argtypes = [P, C.POINTER(Params)]
     _lib = L
     return L
 

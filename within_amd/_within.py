@@ -276,52 +276,105 @@ class _Design:
 
 
 class FePreconditioner:
-    """The preconditioner of a built ``Solver`` (device resident).
+    """A built Schwarz preconditioner (device resident)
+    (crates/within-py/src/lib.rs:820-913).
 
-    ``apply`` runs the GPU additive / multiplicative Schwarz apply.  Passing it
-    to ``Solver(..., preconditioner=p)`` reuses its factors when the design
-    matches.  The byte serialisation of the reference (postcard) is not
-    provided; pickling raises TypeError.
+    Obtained from ``Solver.preconditioner()`` or rebuilt from its bytes with
+    ``FePreconditioner(data)``; pickling stores the bytes (this library's own
+    format, not the reference's postcard encoding).  ``apply`` runs the GPU
+    additive / multiplicative Schwarz apply.  ``Solver(categories,
+    preconditioner=p)`` uploads these factors for a design with the same DOFs
+    instead of building new ones (solver.rs:135-155).
     """
 
-    def __init__(self, data=None, *, _solver=None):
-        if _solver is None:
-            raise TypeError("FePreconditioner objects are obtained from Solver.preconditioner()")
-        self._solver = _solver
+    def __init__(self, data=None, *, _solver=None, _host_only=False):
+        self._bytes = None
+        self._own = None
+        if _solver is not None:
+            self._solver = _solver
+            self._h = _solver._h
+            return
+        if data is None:
+            raise TypeError("FePreconditioner(data) needs the bytes of a serialised preconditioner")
+        b = bytes(data)
+        h = C.c_void_p()
+        rc = N.load().wa_precond_create(_byte_buffer(b), len(b), 1 if _host_only else 0, C.byref(h))
+        if rc != 0:
+            raise ValueError(f"failed to deserialize preconditioner: {N.last_error()}")
+        self._own = h
+        self._h = h
+        self._solver = None
+        self._bytes = b
+
+    def __del__(self):
+        if getattr(self, "_own", None):
+            N.load().wa_solver_destroy(self._own)
+            self._own = None
+
+    def _info(self):
+        out = np.zeros(16, dtype=np.int64)
+        N.check(N.load().wa_solver_info(self._h, N.ptr(out)))
+        return out
+
+    def _kind(self):
+        p = N.Params()
+        N.check(N.load().wa_solver_params(self._h, C.byref(p)))
+        return int(p.preconditioner)
+
+    def to_bytes(self):
+        """The serialised preconditioner (kind, local config, factors)."""
+        if self._bytes is None:
+            self._bytes = _precond_bytes(self._h)
+        return self._bytes
 
     def apply(self, x):
-        s = self._solver
-        x = _vector(x, s.n_dofs, "x")
-        y = np.zeros(s.n_dofs)
-        N.check(N.load().wa_precond_apply(s._h, N.ptr(x), N.ptr(y), 1, s.n_dofs))
+        n = self.nrows
+        x = _vector(x, n, "x")
+        y = np.zeros(n)
+        N.check(N.load().wa_precond_apply(self._h, N.ptr(x), N.ptr(y), 1, n))
         return y
 
     @property
     def nrows(self):
-        return self._solver.n_dofs
+        return int(self._info()[9])
 
     @property
     def ncols(self):
-        return self._solver.n_dofs
+        return self.nrows
 
     @property
     def n_subdomains(self):
-        return int(self._solver.info()[0])
+        return int(self._info()[0])
 
     @property
     def subdomain_inner_parallel_work(self):
         return [0] * self.n_subdomains
 
     def additive_schwarz_diagnostics(self):
-        if self._solver._kind != 1:
+        if self._kind() != 1:
             return None
-        return AdditiveSchwarzDiagnostics(self._solver.info(), self._solver._reduction)
+        red = self._solver._reduction if self._solver is not None else ReductionStrategy.Auto
+        return AdditiveSchwarzDiagnostics(self._info(), red)
 
     def __reduce__(self):
-        raise TypeError("FePreconditioner serialisation is not provided by within_amd")
+        return (FePreconditioner, (self.to_bytes(),))
 
     def __repr__(self):
-        return f"FePreconditioner(n_dofs={self.nrows}, n_subdomains={self.n_subdomains})"
+        variant = "Additive" if self._kind() == 1 else "Multiplicative"
+        return f"FePreconditioner({variant}, n={self.nrows})"
+
+
+def _byte_buffer(b):
+    return (C.c_uint8 * max(len(b), 1)).from_buffer_copy(b if b else b"\0")
+
+
+def _precond_bytes(h):
+    L = N.load()
+    n = C.c_int64(0)
+    N.check(L.wa_solver_precond_bytes(h, None, C.byref(n)))
+    buf = (C.c_uint8 * max(n.value, 1))()
+    N.check(L.wa_solver_precond_bytes(h, buf, C.byref(n)))
+    return bytes(buf)[: n.value]
 
 
 class Solver:
@@ -331,19 +384,26 @@ class Solver:
     def __init__(self, categories, config=None, weights=None, preconditioner=None, *, _host_only=False,
                  _dense_core=0):
         t0 = time.perf_counter()
-        if isinstance(preconditioner, FePreconditioner):
-            base = preconditioner._solver
-            preconditioner = base._precond_arg
-        self._precond_arg = preconditioner
+        L = N.load()
+        given = preconditioner if isinstance(preconditioner, FePreconditioner) else None
+        self._precond_arg = None if given is not None else preconditioner
         self._design = _Design(categories, weights)
-        p, self._reduction = _params(config, preconditioner)
+        p, self._reduction = _params(config, self._precond_arg)
         p.host_only = 1 if _host_only else 0
         p.local.dense_core = int(_dense_core)
-        self._kind = p.preconditioner
-        self._params = p
         h = C.c_void_p()
-        N.check(N.load().wa_solver_create(self._design.h, C.byref(p), C.byref(h)))
+        if given is None:
+            N.check(L.wa_solver_create(self._design.h, C.byref(p), C.byref(h)))
+        else:
+            if given._solver is not None:
+                self._reduction = given._solver._reduction
+            b = given.to_bytes()
+            N.check(L.wa_solver_create_with_precond(self._design.h, C.byref(p), _byte_buffer(b), len(b),
+                                                    C.byref(h)))
+            N.check(L.wa_solver_params(h, C.byref(p)))
         self._h = h
+        self._kind = int(p.preconditioner)
+        self._params = p
         self.time_setup = time.perf_counter() - t0
 
     def __del__(self):
@@ -356,7 +416,8 @@ class Solver:
         precision, block) of this built solver; the preconditioner stays."""
         p, _ = _params(config, self._precond_arg)
         p.host_only = self._params.host_only
-        p.local.dense_core = self._params.local.dense_core
+        p.preconditioner = self._params.preconditioner
+        p.local = self._params.local
         N.check(N.load().wa_solver_set_params(self._h, C.byref(p)))
         self._params = p
 

@@ -985,6 +985,16 @@ DeviceLayout make_device_layout(const Plan& p) {
   const int Q = d.Q;
   const i64 n = d.n_obs;
   DeviceLayout L;
+  if (Q == 0) {
+    // a preconditioner on its own (no design): the user numbering as is
+    L.m = d.n_dofs;
+    L.int_of.resize(L.m);
+    std::iota(L.int_of.begin(), L.int_of.end(), 0u);
+    L.ext_of = L.int_of;
+    L.gram = p.gram;
+    L.pre = p.pre;
+    return L;
+  }
   // active levels
   std::vector<char> act(d.n_dofs, 0);
   for (int q = 0; q < Q; ++q) {
@@ -1083,8 +1093,13 @@ DeviceLayout make_device_layout(const Plan& p) {
   }
   // preconditioner: DOF indices renumbered; each subdomain's eliminated block
   // (its elimination rows) re-laid out in internal DOF order
-  if (!p.subs.empty()) {
+  if (!p.meta.empty()) {
     const PrecondFlat& f = p.pre;
+    // (bytes built for another design with the same DOF count may name
+    // levels this design never observes)
+    for (u32 g : f.loc_gidx)
+      if (L.int_of[g] == UINT32_MAX)
+        throw Error(E_INVALID, "the preconditioner does not match the design (it covers an unobserved level)");
     PrecondFlat nf = f;
     const i64 ne = (i64)f.elim_loc.size();
     std::vector<u32> lp(f.n_loc);
@@ -1147,7 +1162,7 @@ void build_plan(Plan& p) {
     p.gram = build_gramian(d);
     p.has_gram = true;
   }
-  if (p.precond == 1 || p.precond == 2) {
+  if ((p.precond == 1 || p.precond == 2) && !p.pre_given) {
     {
       PhaseTimer t("subdomains");
       p.subs = build_subdomains(d);
@@ -1180,7 +1195,202 @@ void build_plan(Plan& p) {
       if (code[i]) throw Error(code[i], err[i]);
     PhaseTimer tf("flatten");
     flatten(p);
+    p.meta.clear();
+    for (const Subdomain& sd : p.subs)
+      p.meta.push_back({sd.q, sd.r, sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r(), (i64)sd.f.core.size()});
   }
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner bytes
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kMagic = 0x43504157u;  // "WAPC"
+constexpr uint32_t kVersion = 1;
+
+struct Writer {
+  std::vector<uint8_t> b;
+  template <typename T>
+  void put(const T& v) {
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+  }
+  template <typename T>
+  void vec(const std::vector<T>& v) {
+    put<uint64_t>(v.size());
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(v.data());
+    b.insert(b.end(), p, p + sizeof(T) * v.size());
+  }
+};
+
+struct Reader {
+  const uint8_t* p;
+  i64 left;
+  void need(i64 n) {
+    if (n < 0 || n > left) throw Error(E_INVALID, "preconditioner bytes are truncated or malformed");
+  }
+  template <typename T>
+  T get() {
+    need((i64)sizeof(T));
+    T v;
+    std::memcpy(&v, p, sizeof(T));
+    p += sizeof(T);
+    left -= (i64)sizeof(T);
+    return v;
+  }
+  template <typename T>
+  void vec(std::vector<T>& v) {
+    const uint64_t n = get<uint64_t>();
+    if (n > (uint64_t)left / sizeof(T)) throw Error(E_INVALID, "preconditioner bytes are truncated or malformed");
+    v.resize(n);
+    std::memcpy(v.data(), p, sizeof(T) * n);
+    p += sizeof(T) * n;
+    left -= (i64)(sizeof(T) * n);
+  }
+};
+
+template <class IO>
+void flat_fields(IO& io, PrecondFlat& f) {
+  io.vec(f.loc_gidx); io.vec(f.loc_scale); io.vec(f.loc_ptr); io.vec(f.red_ptr); io.vec(f.loc_sub);
+  io.vec(f.red_sub); io.vec(f.red_loc); io.vec(f.elim_loc); io.vec(f.elim_inv); io.vec(f.elim_ptr);
+  io.vec(f.elim_idx); io.vec(f.elim_val); io.vec(f.keep_ptr); io.vec(f.keep_idx); io.vec(f.keep_val);
+  io.vec(f.keep_rowsum); io.vec(f.lvl_ptr); io.vec(f.lvl_rows); io.vec(f.fin_ptr); io.vec(f.fin_idx);
+  io.vec(f.fin_val); io.vec(f.bout_ptr); io.vec(f.bout_idx); io.vec(f.bout_val); io.vec(f.inv_pivot);
+  io.vec(f.core_ptr); io.vec(f.core_idx); io.vec(f.core_mat_off); io.vec(f.core_mat); io.vec(f.core_row_core);
+  io.vec(f.dof_ptr); io.vec(f.dof_loc);
+}
+}  // namespace
+
+std::vector<uint8_t> serialize_precond(const Plan& p) {
+  if (p.precond != 1 && p.precond != 2) throw Error(E_INVALID, "solver has no preconditioner");
+  Writer w;
+  w.put(kMagic);
+  w.put(kVersion);
+  w.put<int32_t>(p.precond);
+  w.put<i64>(p.d.n_dofs);
+  w.put<uint64_t>(p.cfg.ac_seed); w.put<int32_t>(p.cfg.ac_split); w.put<int32_t>(p.cfg.approx_schur ? 1 : 0);
+  w.put<uint64_t>(p.cfg.schur_seed); w.put<int32_t>(p.cfg.schur_split); w.put<int32_t>(p.cfg.dense_threshold);
+  w.put<i64>(p.cfg.dense_core);
+  w.vec(p.meta);
+  const PrecondFlat& f = p.pre;
+  w.put<i64>(f.n_loc); w.put<i64>(f.n_red); w.put<i64>(f.n_sub);
+  w.put<i64>(f.max_core); w.put<i64>(f.ac_nnz); w.put<i64>(f.c_nnz); w.put<i64>(f.dense_subs);
+  flat_fields(w, const_cast<PrecondFlat&>(f));
+  if (p.precond == 2) {
+    w.put<i64>(p.gram.nrows);
+    w.vec(p.gram.indptr); w.vec(p.gram.indices); w.vec(p.gram.data);
+  }
+  return std::move(w.b);
+}
+
+i64 deserialize_precond(const uint8_t* buf, i64 len, Plan& p) {
+  if (!buf || len <= 0) throw Error(E_INVALID, "empty preconditioner bytes");
+  Reader r{buf, len};
+  if (r.get<uint32_t>() != kMagic) throw Error(E_INVALID, "not a within_amd preconditioner");
+  if (r.get<uint32_t>() != kVersion) throw Error(E_INVALID, "unsupported preconditioner format version");
+  p.precond = r.get<int32_t>();
+  if (p.precond != 1 && p.precond != 2) throw Error(E_INVALID, "bad preconditioner kind");
+  const i64 n_dofs = r.get<i64>();
+  p.cfg.ac_seed = r.get<uint64_t>(); p.cfg.ac_split = r.get<int32_t>(); p.cfg.approx_schur = r.get<int32_t>() != 0;
+  p.cfg.schur_seed = r.get<uint64_t>(); p.cfg.schur_split = r.get<int32_t>(); p.cfg.dense_threshold = r.get<int32_t>();
+  p.cfg.dense_core = r.get<i64>();
+  r.vec(p.meta);
+  PrecondFlat& f = p.pre;
+  f.n_loc = r.get<i64>(); f.n_red = r.get<i64>(); f.n_sub = r.get<i64>();
+  f.max_core = r.get<i64>(); f.ac_nnz = r.get<i64>(); f.c_nnz = r.get<i64>(); f.dense_subs = r.get<i64>();
+  flat_fields(r, f);
+  // consistency of the sizes every kernel relies on
+  bool ok = n_dofs > 0 && (i64)f.loc_ptr.size() == f.n_sub + 1 && (i64)f.red_ptr.size() == f.n_sub + 1 &&
+            (i64)p.meta.size() == f.n_sub && (i64)f.loc_gidx.size() == f.n_loc && (i64)f.loc_scale.size() == f.n_loc &&
+            (i64)f.loc_sub.size() == f.n_loc && (i64)f.red_sub.size() == f.n_red && (i64)f.red_loc.size() == f.n_red &&
+            (i64)f.dof_ptr.size() == n_dofs + 1 && (i64)f.dof_loc.size() == f.n_loc &&
+            f.loc_ptr.back() == (u32)f.n_loc && f.red_ptr.back() == (u32)f.n_red;
+  auto below = [&](const std::vector<u32>& v, i64 n) {
+    for (u32 x : v) ok = ok && (i64)x < n;
+  };
+  auto csr = [&](const std::vector<u32>& ptr, i64 rows, size_t nnz) {
+    ok = ok && (i64)ptr.size() == rows + 1 && ptr[0] == 0 && ptr.back() == nnz;
+    for (size_t i = 0; ok && i + 1 < ptr.size(); ++i) ok = ptr[i] <= ptr[i + 1];
+  };
+  if (ok) {
+    const i64 n_elim = (i64)f.elim_loc.size(), n_lv = (i64)f.lvl_rows.size();
+    const i64 n_cores = f.core_ptr.empty() ? 0 : (i64)f.core_ptr.size() - 1;
+    below(f.loc_gidx, n_dofs); below(f.loc_sub, f.n_sub); below(f.red_sub, f.n_sub); below(f.red_loc, f.n_loc);
+    below(f.elim_loc, f.n_loc); below(f.elim_idx, f.n_red); below(f.keep_idx, f.n_loc); below(f.lvl_rows, f.n_red);
+    below(f.fin_idx, f.n_red); below(f.bout_idx, f.n_red); below(f.core_idx, f.n_red); below(f.dof_loc, f.n_loc);
+    below(f.core_row_core, n_cores);
+    csr(f.loc_ptr, f.n_sub, (size_t)f.n_loc); csr(f.red_ptr, f.n_sub, (size_t)f.n_red);
+    csr(f.elim_ptr, n_elim, f.elim_idx.size()); csr(f.keep_ptr, f.n_red, f.keep_idx.size());
+    csr(f.fin_ptr, n_lv, f.fin_idx.size()); csr(f.bout_ptr, n_lv, f.bout_idx.size());
+    csr(f.dof_ptr, n_dofs, (size_t)f.n_loc);
+    ok = ok && f.elim_inv.size() == f.elim_loc.size() && f.elim_val.size() == f.elim_idx.size() &&
+         f.keep_val.size() == f.keep_idx.size() && (i64)f.keep_rowsum.size() == f.n_red &&
+         f.fin_val.size() == f.fin_idx.size() && f.bout_val.size() == f.bout_idx.size() &&
+         (i64)f.inv_pivot.size() == f.n_red && (i64)f.core_mat_off.size() == n_cores &&
+         f.core_row_core.size() == f.core_idx.size() && !f.lvl_ptr.empty() && f.lvl_ptr.back() == (u32)n_lv;
+    for (size_t i = 0; ok && i + 1 < f.lvl_ptr.size(); ++i) ok = f.lvl_ptr[i] <= f.lvl_ptr[i + 1];
+    if (n_cores > 0) csr(f.core_ptr, n_cores, f.core_idx.size());
+    i64 max_core = 0;
+    for (i64 c = 0; ok && c < n_cores; ++c) {
+      const uint64_t nc = f.core_ptr[c + 1] - f.core_ptr[c];
+      ok = f.core_mat_off[c] <= f.core_mat.size() && nc * nc <= f.core_mat.size() - f.core_mat_off[c];
+      for (u32 i = f.core_ptr[c]; ok && i < f.core_ptr[c + 1]; ++i) ok = f.core_row_core[i] == (u32)c;
+      max_core = std::max<i64>(max_core, (i64)nc);
+    }
+    ok = ok && max_core == f.max_core;
+    // local / reduced entries belong to the subdomain whose range holds them
+    for (i64 s = 0; ok && s < f.n_sub; ++s) {
+      for (u32 l = f.loc_ptr[s]; ok && l < f.loc_ptr[s + 1]; ++l) ok = f.loc_sub[l] == (u32)s;
+      for (u32 j = f.red_ptr[s]; ok && j < f.red_ptr[s + 1]; ++j)
+        ok = f.red_sub[j] == (u32)s && f.red_loc[j] >= f.loc_ptr[s] && f.red_loc[j] < f.loc_ptr[s + 1];
+    }
+    // elimination rows: grouped by subdomain in order, meta[s].n_elim of them,
+    // their local entries a contiguous block of that subdomain, each once
+    if (ok) {
+      std::vector<char> seen(f.n_loc, 0);
+      i64 a = 0;
+      for (i64 s = 0; ok && s < f.n_sub; ++s) {
+        const i64 b = a + p.meta[s].n_elim;
+        ok = b <= n_elim;
+        u32 lo = UINT32_MAX, hi = 0;
+        for (i64 t = a; ok && t < b; ++t) {
+          const u32 l = f.elim_loc[t];
+          ok = f.loc_sub[l] == (u32)s && !seen[l];
+          seen[l] = 1;
+          lo = std::min(lo, l);
+          hi = std::max(hi, l);
+        }
+        ok = ok && (a == b || (i64)(hi - lo) + 1 == b - a);
+        a = b;
+      }
+      ok = ok && a == n_elim;
+    }
+  }
+  {
+    i64 se = 0, sc = 0;
+    for (const SubMeta& m : p.meta) {
+      ok = ok && m.q >= 0 && m.r >= 0 && m.n_elim >= 0 && m.n_core >= 0;
+      se += m.n_elim;
+      sc += m.n_core;
+    }
+    ok = ok && se == (i64)f.elim_loc.size() && sc == (i64)f.core_idx.size();
+  }
+  if (!ok) throw Error(E_INVALID, "preconditioner bytes are inconsistent");
+  if (p.precond == 2) {
+    Csr& g = p.gram;
+    g.nrows = g.ncols = r.get<i64>();
+    r.vec(g.indptr); r.vec(g.indices); r.vec(g.data);
+    bool gok = g.nrows == n_dofs && (i64)g.indptr.size() == n_dofs + 1 && g.indices.size() == g.data.size() &&
+               g.indptr[0] == 0 && g.indptr.back() == g.indices.size();
+    for (i64 i = 0; gok && i < n_dofs; ++i) gok = g.indptr[i] <= g.indptr[i + 1];
+    if (!gok) throw Error(E_INVALID, "preconditioner bytes are inconsistent");
+    for (u32 j : g.indices)
+      if ((i64)j >= n_dofs) throw Error(E_INVALID, "preconditioner bytes are inconsistent");
+    p.has_gram = true;
+  }
+  if (r.left != 0) throw Error(E_INVALID, "trailing bytes after the preconditioner");
+  p.pre_given = true;
+  return n_dofs;
 }
 
 }  // namespace wa

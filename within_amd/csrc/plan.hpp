@@ -165,17 +165,34 @@ struct PrecondFlat {
   i64 max_core = 0, ac_nnz = 0, c_nnz = 0, dense_subs = 0;
 };
 
+// What the device setup needs of each subdomain (kept also when the
+// preconditioner comes from serialized bytes and the subdomain objects are
+// not rebuilt): its factor pair, eliminated count and dense-core size.
+struct SubMeta { int32_t q = 0, r = 0; i64 n_elim = 0, n_core = 0; };
+
 struct Plan {
   Design d;
   LocalConfig cfg;
   int precond = 1;            // 0 none, 1 additive, 2 multiplicative
   bool explicit_op = false;
+  bool pre_given = false;     // pre / meta (and gram for precond 2) came from bytes
   std::vector<FactorPass> passes;  // one per factor
-  std::vector<Subdomain> subs;
+  std::vector<Subdomain> subs;     // empty when pre_given
+  std::vector<SubMeta> meta;       // one per subdomain
   PrecondFlat pre;
   Csr gram;                   // explicit Gramian (when needed)
   bool has_gram = false;
 };
+
+// Serialized preconditioner (this library's own format, host byte order):
+// the preconditioner kind, local config, DOF count, per-subdomain metadata,
+// the flattened arrays and, for multiplicative Schwarz, the Gramian its
+// residual updates use.  Everything a device apply needs, in the user's DOF
+// numbering.
+std::vector<uint8_t> serialize_precond(const Plan& p);
+// Fills p.precond, p.cfg, p.meta, p.pre (and p.gram), sets p.pre_given and
+// returns the serialized DOF count (throws wa::Error on malformed bytes).
+i64 deserialize_precond(const uint8_t* buf, i64 len, Plan& p);
 
 // Builds everything the solver uploads (throws wa::Error).
 void build_plan(Plan& p);

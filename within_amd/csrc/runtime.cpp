@@ -142,6 +142,8 @@ struct wa_solver {
   GmresBufs<double> gm64;
   GmresBufs<float> gm32;
   template <typename T> GmresBufs<T>& gm(int restart);
+  // serialized preconditioner (built on first request)
+  mutable std::vector<uint8_t> precond_bytes;
   // device user buffers for host-pointer calls
   double* ubuf = nullptr;
   int64_t ubuf_len = 0;
@@ -345,7 +347,7 @@ void wa_solver::setup_device() {
     max_slots = std::max(max_slots, g64.sched.n_slots);
     has_gram = true;
   }
-  if (!plan.subs.empty()) {
+  if (!plan.meta.empty()) {
     const PrecondFlat& f = lay.pre;
     // dense cores: the large ones are applied by the dense kernels from
     // zero-padded copies; the rest keep the gather path (their matrices packed)
@@ -478,14 +480,15 @@ void wa_solver::setup_device() {
     if (plan.precond == 2) {
       // per-pair subranges (subdomains are ordered by factor pair)
       std::vector<int64_t> elim_ptr_sub(1, 0), core_row_sub(1, 0);
-      for (const Subdomain& sd : plan.subs) {
-        elim_ptr_sub.push_back(elim_ptr_sub.back() + (sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r()));
-        core_row_sub.push_back(core_row_sub.back() + (int64_t)sd.f.core.size());
+      for (const SubMeta& sd : plan.meta) {
+        elim_ptr_sub.push_back(elim_ptr_sub.back() + sd.n_elim);
+        core_row_sub.push_back(core_row_sub.back() + sd.n_core);
       }
+      const std::vector<SubMeta>& sm = plan.meta;
       size_t s0 = 0;
-      while (s0 < plan.subs.size()) {
+      while (s0 < sm.size()) {
         size_t s1 = s0;
-        while (s1 < plan.subs.size() && plan.subs[s1].q == plan.subs[s0].q && plan.subs[s1].r == plan.subs[s0].r) ++s1;
+        while (s1 < sm.size() && sm[s1].q == sm[s0].q && sm[s1].r == sm[s0].r) ++s1;
         PairSched ps;
         ps.loc0 = f.loc_ptr[s0]; ps.loc1 = f.loc_ptr[s1];
         ps.red0 = f.red_ptr[s0]; ps.red1 = f.red_ptr[s1];
@@ -1025,6 +1028,10 @@ static void need_device(wa_solver* s) {
   if (!s) throw Error(WA_ERR_INVALID_ARGUMENT, "null solver");
   if (s->host_only) throw DevError("solver was built host-only (no device state)");
 }
+static void need_design(wa_solver* s) {
+  need_device(s);
+  if (s->plan.d.Q == 0) throw Error(WA_ERR_INVALID_ARGUMENT, "a preconditioner on its own has no design");
+}
 
 // host column-major in/out around a DOF-block device op
 template <typename F>
@@ -1105,7 +1112,19 @@ int64_t wa_design_n_levels(const wa_design* d, int32_t q) {
   return (d && q >= 0 && q < d->d.Q) ? d->d.levels[q] : 0;
 }
 
-wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** out) {
+static void local_from_cfg(const LocalConfig& c, wa_local_config& l) {
+  l.ac_seed = c.ac_seed;
+  l.ac_split = c.ac_split;
+  l.approx_schur = c.approx_schur ? 1 : 0;
+  l.schur_seed = c.schur_seed;
+  l.schur_split = c.schur_split;
+  l.dense_threshold = c.dense_threshold;
+  l.dense_core = c.dense_core;
+}
+
+// solver from a design, optionally with a serialized preconditioner
+static wa_status create_impl(const wa_design* d, const wa_params* p, const uint8_t* pre, int64_t pre_len,
+                             wa_solver** out) {
   try {
     if (!d || !p || !out) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -1132,6 +1151,23 @@ wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** o
     pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
     pl.precond = p->preconditioner;
     pl.explicit_op = p->operator_repr == 1;
+    if (pre) {
+      const int64_t nd = deserialize_precond(pre, pre_len, pl);
+      if (nd != pl.d.n_dofs)
+        throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner was built for a design with a different number of DOFs");
+      if (pl.precond == 2 && p->krylov == 0)
+        throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
+      std::vector<char> seen(pl.d.n_dofs, 0);
+      for (int q = 0; q < pl.d.Q; ++q)
+        for (u32 c : pl.d.codes[q]) seen[pl.d.offset[q] + c] = 1;
+      for (u32 g : pl.pre.loc_gidx)
+        if (!seen[g])
+          throw Error(WA_ERR_INVALID_ARGUMENT,
+                      "the preconditioner does not match the design (it covers an unobserved level)");
+      if (pl.precond == 2) pl.gram = Csr(), pl.has_gram = false;  // rebuilt from the design
+      s->params.preconditioner = pl.precond;
+      local_from_cfg(pl.cfg, s->params.local);
+    }
     build_plan(pl);
     if (!s->host_only) s->setup_device();
     *out = s.release();
@@ -1139,6 +1175,65 @@ wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** o
   } catch (const std::exception& e) {
     return (wa_status)set_error(e);
   }
+}
+
+wa_status wa_solver_create(const wa_design* d, const wa_params* p, wa_solver** out) {
+  return create_impl(d, p, nullptr, 0, out);
+}
+
+wa_status wa_solver_create_with_precond(const wa_design* d, const wa_params* p, const uint8_t* bytes, int64_t len,
+                                        wa_solver** out) {
+  if (!bytes) {
+    g_err = "null preconditioner bytes";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  return create_impl(d, p, bytes, len, out);
+}
+
+wa_status wa_precond_create(const uint8_t* bytes, int64_t len, int32_t host_only, wa_solver** out) {
+  try {
+    if (!bytes || !out) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<wa_solver>();
+    wa_params_default(&s->params);
+    s->host_only = host_only != 0;
+    Plan& pl = s->plan;
+    pl.d.n_dofs = deserialize_precond(bytes, len, pl);
+    s->params.preconditioner = pl.precond;
+    s->params.krylov = pl.precond == 2 ? 1 : 0;
+    local_from_cfg(pl.cfg, s->params.local);
+    if (!s->host_only) s->setup_device();
+    *out = s.release();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_solver_precond_bytes(const wa_solver* s, uint8_t* buf, int64_t* len) {
+  try {
+    if (!s || !len) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->plan.meta.empty()) throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no preconditioner");
+    if (s->precond_bytes.empty()) s->precond_bytes = serialize_precond(s->plan);
+    const int64_t need = (int64_t)s->precond_bytes.size();
+    if (buf) {
+      if (*len < need) throw Error(WA_ERR_INVALID_ARGUMENT, "buffer too small for the preconditioner bytes");
+      std::memcpy(buf, s->precond_bytes.data(), (size_t)need);
+    }
+    *len = need;
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_solver_params(const wa_solver* s, wa_params* out) {
+  if (!s || !out) {
+    g_err = "null argument";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  *out = s->params;
+  return WA_OK;
 }
 
 void wa_solver_destroy(wa_solver* s) { delete s; }
@@ -1165,7 +1260,7 @@ wa_status wa_solver_set_params(wa_solver* s, const wa_params* p) {
 static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ldy, double* X, int64_t ldx,
                             double* dem, int64_t ldd, wa_rhs_stats* stats, bool device) {
   try {
-    need_device(s);
+    need_design(s);
     if (k < 0 || !stats || (k > 0 && (!Y || !X))) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
     if (ldy < s->n || ldx < s->mu || (dem && ldd < s->n))
       throw Error(WA_ERR_OBS_COUNT_MISMATCH, "ObservationCountMismatch: leading dimension too small");
@@ -1218,6 +1313,10 @@ wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_
 }
 
 wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
   return level_a(s, X, s ? s->mu : 0, ld, Y, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
     launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
     int saved = s->params.operator_repr;
@@ -1253,6 +1352,10 @@ wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, 
 }
 
 wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
   return level_a(s, X, s ? s->mu : 0, ldx, Y, s ? s->n : 0, ldy, k, [&](double* din, double* dout, int kp) {
     launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
     launch_demean(s->st, s->n, s->Q, s->codes, s->offs, s->cp64, nullptr, s->E, kp);
@@ -1261,6 +1364,10 @@ wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int
 }
 
 wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
   return level_a(s, R, s ? s->n : 0, ldr, X, s ? s->mu : 0, ldx, k, [&](double* din, double* dout, int kp) {
     launch_col_to_row(s->st, din, s->n, s->n, k, s->Yr, kp);
     s->rhs64(s->Yr, s->B64, kp);
@@ -1272,7 +1379,7 @@ wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
   if (!s || !info) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
   const PrecondFlat& f = s->plan.pre;
   std::memset(info, 0, sizeof(int64_t) * 16);
-  info[0] = (int64_t)s->plan.subs.size();
+  info[0] = (int64_t)s->plan.meta.size();
   info[1] = f.n_loc;
   info[2] = f.n_red;
   info[3] = f.lvl_ptr.empty() ? 0 : (int64_t)f.lvl_ptr.size() - 2;
