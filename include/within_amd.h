@@ -34,7 +34,7 @@ typedef enum {
   WA_ERR_ITERATIVE_SOLVE = 8,      /* WithinError::IterativeSolve               */
   WA_ERR_INVALID_ARGUMENT = 9,     /* bad sizes / pointers / config values      */
   WA_ERR_DEVICE = 10,              /* HIP runtime failure, or no gfx950 device  */
-  WA_ERR_UNSUPPORTED = 11          /* KrylovMethod::Lsmr and other unbuilt paths */
+  WA_ERR_UNSUPPORTED = 11          /* configurations this library does not build */
 } wa_status;
 
 /* LocalSolverConfig + ApproxCholConfig + ApproxSchurConfig
@@ -55,7 +55,7 @@ typedef struct {
 
 /* SolverParams + Preconditioner (config.rs:157-237). */
 typedef struct {
-  int32_t krylov;          /* 0 = Cg, 1 = Gmres{restart}, 2 = Lsmr (unsupported)   */
+  int32_t krylov;          /* 0 = Cg, 1 = Gmres{restart}, 2 = Lsmr{local_size}     */
   int32_t restart;         /* GMRES restart (default 30)                           */
   int32_t operator_repr;   /* 0 = Implicit, 1 = Explicit                           */
   int32_t preconditioner;  /* 0 = none, 1 = Additive, 2 = Multiplicative           */
@@ -68,6 +68,7 @@ typedef struct {
                               RHS per solve                                        */
   int32_t host_only;       /* 1 = build the host-side setup only (no GPU); apply
                               and solve calls then fail with WA_ERR_DEVICE        */
+  int32_t local_size;      /* Lsmr reorthogonalisation window (0 = None)           */
   wa_local_config local;
 } wa_params;
 

@@ -46,6 +46,7 @@ class _Params(C.Structure):
         ("schur_split", C.c_int),
         ("dense_threshold", C.c_int),
         ("dense_core", C.c_int64),
+        ("local_size", C.c_int64),
     ]
 
 
@@ -86,6 +87,7 @@ def lib():
         L.oracle_schur_dense.argtypes = [P, C.c_int64, C.c_int64, P, P, C.c_int, C.c_uint64, C.c_int, P, P]
         L.oracle_time_solve_batch.argtypes = [P, P, C.c_int, C.c_int64, C.c_int, P]
         L.oracle_time_solve_batch.restype = C.c_double
+        L.oracle_lsmr_dense.argtypes = [C.c_int64, C.c_int64, P, P, P, C.c_double, C.c_int64, C.c_int64, P, P]
         _lib = L
     return _lib
 
@@ -104,14 +106,14 @@ class OracleSolver:
     """FP64 CPU solver restating within's ``Solver`` (solver.rs:82-447).
 
     precond: 0 none, 1 additive Schwarz, 2 multiplicative Schwarz.
-    krylov:  0 CG, 1 GMRES(restart).
+    krylov:  0 CG, 1 GMRES(restart), 2 LSMR (window ``local_size``, 0 = None).
     Local solver defaults follow ``LocalSolverConfig::default()`` (config.rs:96-104).
     """
 
     def __init__(self, categories, weights=None, *, krylov=0, restart=30, explicit_op=False,
                  tol=1e-8, maxiter=1000, max_refinements=2, precond=1, ac_seed=0, ac_split=1,
                  approx_schur=True, schur_seed=0, schur_split=1, dense_threshold=24,
-                 dense_core=8192):
+                 dense_core=8192, local_size=0):
         L = lib()
         cats = np.asfortranarray(np.asarray(categories, dtype=np.uint32))
         if cats.ndim != 2:
@@ -121,7 +123,7 @@ class OracleSolver:
         self._w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         p = _Params(krylov, restart, int(explicit_op), tol, maxiter, max_refinements, precond,
                     ac_seed, ac_split, int(approx_schur), schur_seed, schur_split, dense_threshold,
-                    dense_core)
+                    dense_core, int(local_size or 0))
         h = C.c_void_p()
         rc = L.oracle_solver_new(_ptr(cats), self.n_obs, self.n_factors, max(self.n_obs, 1),
                                  _ptr(self._w), C.byref(p), C.byref(h))
@@ -284,3 +286,23 @@ def schur_dense(c_dense, diag_q, diag_r, approx=False, seed=0, split=1):
     if rc < 0:
         raise OracleError(-rc, lib().oracle_last_error().decode())
     return S, bool(rc), inv
+
+
+LSMR_STOPS = ("zero_rhs", "atb_zero", "residual_tol", "normal_tol", "breakdown", "maxiter")
+
+
+def lsmr_dense(a, b, minv=None, tol=1e-10, maxiter=100, local_size=None):
+    """LSMR (lsmr.rs:122-316) on a dense matrix; ``minv`` is a diagonal M^-1."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    rows, cols = a.shape
+    mv = None if minv is None else np.ascontiguousarray(minv, dtype=np.float64)
+    x = np.zeros(cols)
+    info = np.zeros(4)
+    L = lib()
+    rc = L.oracle_lsmr_dense(rows, cols, _ptr(a), _ptr(b), _ptr(mv), tol, maxiter, int(local_size or 0),
+                             _ptr(x), _ptr(info))
+    if rc != 0:
+        raise OracleError(rc, L.oracle_last_error().decode())
+    return dict(x=x, converged=bool(info[0]), iterations=int(info[1]), residual_norm=float(info[2]),
+                stop=LSMR_STOPS[int(info[3])])

@@ -1000,6 +1000,227 @@ KrylovResult pgmres(const LinOp& A, const double* b, i64 n, const LinOp* M, doub
 }
 
 // ===========================================================================
+// LSMR -- schwarz-precond/src/solve/lsmr.rs:122-316 (driver),
+// lsmr/bidiag.rs:86-590 (Golub-Kahan / modified Golub-Kahan, windowed MGS),
+// lsmr/recurrence.rs:15-285 (the two Givens chains, the x/h/hbar recurrence,
+// the two stopping tests)
+// ===========================================================================
+namespace {
+struct Givens {  // recurrence.rs:20-30
+  double c, s, r;
+  Givens(double a, double b) {
+    r = std::hypot(a, b);
+    if (r > 0.0) { c = a / r; s = b / r; } else { c = 1.0; s = 0.0; }
+  }
+};
+struct Rot { double rho = 1.0, rho_bar = 1.0, theta_new = 0.0, theta_bar = 0.0, zeta = 0.0; };
+
+// ring of past v (and, preconditioned, p~ = M v) vectors (bidiag.rs:86-262)
+struct Ring {
+  i64 n = 0, cap = 0, next = 0, count = 0;
+  std::vector<double> v, p;
+  Ring(i64 rows, i64 cols, i64 local_size, bool paired) : n(cols) {
+    cap = std::max<i64>(0, std::min(local_size, std::min(rows, cols)));
+    v.assign(cap * n, 0.0);
+    if (paired) p.assign(cap * n, 0.0);
+  }
+  // modified Gram-Schmidt, oldest first; the coefficient is <y, v_j> (plain)
+  // or <y, p_j> (M-weighted, p~ updated in lockstep)
+  void reorth(double* y, double* pt) const {
+    const i64 start = count < cap ? 0 : next;
+    for (i64 i = 0; i < count; ++i) {
+      const i64 j = (start + i) % cap;
+      const double* vj = v.data() + j * n;
+      const double* pj = pt ? p.data() + j * n : vj;
+      const double c = dot(y, pj, n);
+      for (i64 t = 0; t < n; ++t) y[t] = -c * vj[t] + y[t];
+      if (pt)
+        for (i64 t = 0; t < n; ++t) pt[t] = -c * pj[t] + pt[t];
+    }
+  }
+  void push(const double* y, const double* pt, double inv_alpha) {
+    std::copy(y, y + n, v.begin() + next * n);
+    if (pt)
+      for (i64 t = 0; t < n; ++t) p[next * n + t] = pt[t] * inv_alpha;
+    next = (next + 1) % cap;
+    if (count < cap) ++count;
+  }
+};
+
+struct Bidiag {  // GolubKahan (M == nullptr) or ModifiedGolubKahan
+  const RectOp& A;
+  const LinOp* M;
+  i64 m, n;
+  std::vector<double> u, v, pt, av, atu;
+  Ring ring;
+  double alpha = 0.0, beta_prev_inv = 1.0;
+  Bidiag(const RectOp& a, const LinOp* mm, i64 local_size)
+      : A(a), M(mm), m(a.rows()), n(a.cols()), u(m, 0.0), v(n, 0.0), pt(mm ? n : 0, 0.0), av(m), atu(n),
+        ring(a.rows(), a.cols(), local_size, mm != nullptr) {}
+
+  // bidiag.rs:476-505 / 551-589; returns (alpha_1, beta_1)
+  std::pair<double, double> init(const double* b) {
+    const double beta = std::sqrt(dot(b, b, m));
+    if (beta > 0.0) {
+      const double inv = 1.0 / beta;
+      for (i64 i = 0; i < m; ++i) u[i] = b[i] * inv;
+    }
+    if (!M) {
+      A.adjoint(u.data(), v.data());
+      alpha = std::sqrt(dot(v.data(), v.data(), n));
+      if (alpha > 0.0) scale(v, 1.0 / alpha);
+      if (ring.cap) ring.push(v.data(), nullptr, 0.0);
+    } else {
+      A.adjoint(u.data(), pt.data());
+      M->apply(pt.data(), v.data());
+      const double vp = dot(v.data(), pt.data(), n);
+      alpha = vp > 0.0 ? std::sqrt(vp) : 0.0;
+      if (alpha > 0.0) scale(v, 1.0 / alpha);
+      if (ring.cap) ring.push(v.data(), pt.data(), alpha > 0.0 ? 1.0 / alpha : 0.0);
+      beta_prev_inv = 1.0;
+    }
+    return {alpha, beta};
+  }
+  static void scale(std::vector<double>& y, double s) {
+    for (double& t : y) t *= s;
+  }
+  // y = x + s y, returns ||y||^2 (bidiag.rs:19-38)
+  static double axpy_sq(std::vector<double>& y, const std::vector<double>& x, double s) {
+    double acc = 0.0;
+    for (size_t i = 0; i < y.size(); ++i) {
+      const double val = x[i] + s * y[i];
+      y[i] = val;
+      acc += val * val;
+    }
+    return acc;
+  }
+  // one step (bidiag.rs:286-340 / 342-430): returns (alpha_{k+1}, beta_{k+1})
+  std::pair<double, double> step() {
+    if (!M) {
+      A.apply(v.data(), av.data());
+      const double beta = std::sqrt(axpy_sq(u, av, -alpha));
+      if (beta == 0.0) {
+        std::fill(v.begin(), v.end(), 0.0);
+        alpha = 0.0;
+        return {0.0, beta};
+      }
+      scale(u, 1.0 / beta);
+      A.adjoint(u.data(), atu.data());
+      double a2 = axpy_sq(v, atu, -beta);
+      if (ring.cap) {
+        ring.reorth(v.data(), nullptr);
+        a2 = dot(v.data(), v.data(), n);
+      }
+      const double a = std::sqrt(a2);
+      if (a > 0.0) scale(v, 1.0 / a);
+      if (ring.cap) ring.push(v.data(), nullptr, 0.0);
+      alpha = a;
+      return {a, beta};
+    }
+    const double s = -(alpha * beta_prev_inv);
+    A.apply(v.data(), av.data());
+    const double beta = std::sqrt(axpy_sq(u, av, s));
+    if (beta == 0.0) {
+      std::fill(v.begin(), v.end(), 0.0);
+      std::fill(pt.begin(), pt.end(), 0.0);
+      alpha = 0.0;
+      beta_prev_inv = 0.0;
+      return {0.0, beta};
+    }
+    const double beta_inv = beta > 0.0 ? 1.0 / beta : 0.0;
+    A.adjoint(u.data(), atu.data());
+    const double p_coeff = beta / alpha;
+    for (i64 i = 0; i < n; ++i) pt[i] = beta_inv * atu[i] + (-p_coeff) * pt[i];
+    M->apply(pt.data(), v.data());
+    if (ring.cap) ring.reorth(v.data(), pt.data());
+    const double vp = dot(v.data(), pt.data(), n);
+    const double a = vp > 0.0 ? std::sqrt(vp) : 0.0;
+    if (a > 0.0) scale(v, 1.0 / a);
+    if (ring.cap) ring.push(v.data(), pt.data(), a > 0.0 ? 1.0 / a : 0.0);
+    alpha = a;
+    beta_prev_inv = beta_inv;
+    return {a, beta};
+  }
+};
+}  // namespace
+
+LsmrResult lsmr(const RectOp& A, const double* b, const LinOp* M, double tol, i64 maxiter, i64 local_size) {
+  const i64 m = A.rows(), n = A.cols();
+  // lsmr.rs:254-283 input validation
+  if (!std::isfinite(tol) || tol < 0.0) throw WithinError(ERR_SOLVE, "lsmr: tolerance must be finite and nonnegative");
+  for (i64 i = 0; i < m; ++i)
+    if (!std::isfinite(b[i])) throw WithinError(ERR_SOLVE, "lsmr: rhs entries must be finite");
+  LsmrResult res;
+  const double b_norm = vnorm(b, m);
+  if (b_norm == 0.0) {  // lsmr.rs:193-201
+    res.x.assign(n, 0.0); res.converged = true; res.iterations = 0; res.residual_norm = 0.0;
+    res.stop = LSMR_ZERO_RHS;
+    return res;
+  }
+  Bidiag bd(A, M, local_size);
+  const auto s1 = bd.init(b);
+  if (s1.first == 0.0) {  // lsmr.rs:214-222
+    res.x.assign(n, 0.0); res.converged = true; res.iterations = 0; res.residual_norm = b_norm;
+    res.stop = LSMR_ATB_ZERO;
+    return res;
+  }
+  // recurrence state (recurrence.rs:78-86), solution state (:168-174),
+  // convergence state (:240-246)
+  double alpha_bar = s1.first, phi_bar = s1.second, c_bar = 1.0, s_bar = 0.0, zeta_bar = s1.first * s1.second;
+  std::vector<double> x(n, 0.0), h(bd.v), hb(n, 0.0);
+  const double abs_tol = tol * b_norm, rel_tol = tol;
+  double a_norm_sq = s1.first * s1.first;
+  Rot prev;
+  const double eps = std::numeric_limits<double>::epsilon(), tiny = std::numeric_limits<double>::min();
+  for (i64 itn = 1; itn <= maxiter; ++itn) {
+    const auto st = bd.step();
+    const double alpha = st.first, beta = st.second;
+    a_norm_sq += alpha * alpha + beta * beta;
+    // P-hat then P-bar (recurrence.rs:89-138)
+    const Givens ph(alpha_bar, beta);
+    Rot cur;
+    cur.theta_new = ph.s * alpha;
+    const double alpha_bar_new = -ph.c * alpha, phi_bar_new = ph.s * phi_bar;
+    cur.theta_bar = s_bar * ph.r;
+    const Givens pb(c_bar * ph.r, cur.theta_new);
+    cur.zeta = pb.c * zeta_bar;
+    const double zeta_bar_new = -pb.s * zeta_bar;
+    alpha_bar = alpha_bar_new; phi_bar = phi_bar_new; c_bar = pb.c; s_bar = pb.s; zeta_bar = zeta_bar_new;
+    cur.rho = ph.r;
+    cur.rho_bar = pb.r;
+    // x / h / hbar (recurrence.rs:181-229)
+    const double dx = cur.rho * cur.rho_bar, dh = prev.rho * prev.rho_bar;
+    const double t_x = std::fabs(dx) > eps ? cur.zeta / dx : 0.0;
+    const double t_hbar = std::fabs(dh) > eps ? cur.theta_bar * cur.rho / dh : 0.0;
+    const double t_h = std::fabs(cur.rho) > eps ? cur.theta_new / cur.rho : 0.0;
+    for (i64 i = 0; i < n; ++i) {
+      const double h_old = h[i];
+      const double hbi = h_old - t_hbar * hb[i];
+      hb[i] = hbi;
+      x[i] += t_x * hbi;
+      h[i] = bd.v[i] - t_h * h_old;
+    }
+    // stopping tests (recurrence.rs:263-275); convergence wins over breakdown
+    const double resid = std::fabs(phi_bar);
+    int stop = -1;
+    if (resid <= abs_tol) stop = LSMR_RESIDUAL_TOL;
+    else {
+      const double a_norm = std::max(std::sqrt(a_norm_sq), tiny);
+      if (std::fabs(zeta_bar) / (a_norm * std::max(resid, tiny)) <= rel_tol) stop = LSMR_NORMAL_TOL;
+    }
+    if (stop < 0 && alpha == 0.0) stop = LSMR_BREAKDOWN;
+    if (stop >= 0) {
+      res.x = std::move(x); res.converged = true; res.iterations = itn; res.residual_norm = resid; res.stop = stop;
+      return res;
+    }
+    prev = cur;
+  }
+  res.x = std::move(x); res.converged = false; res.iterations = maxiter; res.residual_norm = std::fabs(phi_bar);
+  res.stop = LSMR_MAXITER;
+  return res;
+}
+
+// ===========================================================================
 // Solver -- crates/within/src/solver.rs:98-132 (build), :158-239 (solve),
 // :296-347 (iterative refinement), :353-398 (dispatch)
 // ===========================================================================
@@ -1042,9 +1263,60 @@ Solver::Solver(Design d, const Params& p) : design(std::move(d)), params(p) {
   time_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// W^(1/2) D and its adjoint (operator.rs:99-181)
+struct WeightedDesignOp : RectOp {
+  const Design& d;
+  std::vector<double> sw;  // sqrt of the weights (empty = unit)
+  explicit WeightedDesignOp(const Design& dd) : d(dd) {
+    if (!d.weights.empty()) {
+      sw.resize(d.n_obs);
+      for (i64 i = 0; i < d.n_obs; ++i) sw[i] = std::sqrt(d.weights[i]);
+    }
+  }
+  i64 rows() const override { return d.n_obs; }
+  i64 cols() const override { return d.n_dofs; }
+  void apply(const double* x, double* y) const override {
+    matvec_d(d, x, y);
+    if (!sw.empty())
+      for (i64 i = 0; i < d.n_obs; ++i) y[i] *= sw[i];
+  }
+  void adjoint(const double* u, double* x) const override {
+    std::vector<double> t(u, u + d.n_obs);
+    if (!sw.empty())
+      for (i64 i = 0; i < d.n_obs; ++i) t[i] = sw[i] * u[i];
+    for (i64 j = 0; j < d.n_dofs; ++j) x[j] = 0.0;
+    for (int q = 0; q < d.n_factors; ++q) {
+      const u32* c = d.codes[q].data();
+      double* xs = x + d.offset[q];
+      for (i64 i = 0; i < d.n_obs; ++i) xs[c[i]] += t[i];
+    }
+  }
+};
+
 SolveOut Solver::solve(const double* y) const {
   const Design& d = design;
   i64 m = d.n_dofs, n = d.n_obs;
+  if (params.krylov == 2) {  // solver.rs:162-207
+    WeightedDesignOp rect(d);
+    std::vector<double> b(y, y + n);
+    if (!rect.sw.empty())
+      for (i64 i = 0; i < n; ++i) b[i] = rect.sw[i] * y[i];
+    SchwarzOp sop(schwarz);
+    LsmrResult r = lsmr(rect, b.data(), has_precond ? &sop : nullptr, params.tol, params.maxiter,
+                        params.local_size);
+    SolveOut out;
+    out.x = std::move(r.x);
+    out.converged = r.converged;
+    out.iterations = r.iterations;
+    out.demeaned.assign(n, 0.0);
+    matvec_d(d, out.x.data(), out.demeaned.data());
+    for (i64 i = 0; i < n; ++i) out.demeaned[i] = y[i] - out.demeaned[i];
+    std::vector<double> rhs(m), res(m);
+    rmatvec_wdt(d, y, rhs.data());
+    rmatvec_wdt(d, out.demeaned.data(), res.data());
+    out.residual = vnorm(res.data(), m) / std::max(vnorm(rhs.data(), m), 1e-15);
+    return out;
+  }
   ImplicitOp iop(d);
   CsrOp cop(gramian);
   const LinOp& A = has_gramian && params.explicit_op ? (const LinOp&)cop : (const LinOp&)iop;

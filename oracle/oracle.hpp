@@ -22,6 +22,9 @@
 //                                                   exact factors, PoU invariants
 //   CG / GMRES / refinement                      -- pinned: cg.rs SpdMatrix3 case, closed forms,
 //                                                   dense lstsq projection of small problems
+//   LSMR (plain and preconditioned, windowed)    -- pinned: the reference's lsmr/tests.rs
+//                                                   operators and answers, numpy lstsq, the
+//                                                   CG solve of the same design
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -184,14 +187,30 @@ KrylovResult pcg(const LinOp& A, const double* b, i64 n, const LinOp* M, double 
 KrylovResult pgmres(const LinOp& A, const double* b, i64 n, const LinOp* M, double tol,
                     i64 maxiter, i64 restart);
 
+// LSMR (schwarz-precond/src/solve/lsmr.rs:122-316): min ||b - A x|| with an
+// optional preconditioner M ~ A'A (modified Golub-Kahan, one M^-1 per step),
+// windowed reorthogonalisation over the last `local_size` v vectors.
+struct RectOp {
+  virtual ~RectOp() {}
+  virtual i64 rows() const = 0;
+  virtual i64 cols() const = 0;
+  virtual void apply(const double* x, double* y) const = 0;    // y = A x
+  virtual void adjoint(const double* u, double* x) const = 0;  // x = A' u
+};
+enum LsmrStop { LSMR_ZERO_RHS = 0, LSMR_ATB_ZERO = 1, LSMR_RESIDUAL_TOL = 2, LSMR_NORMAL_TOL = 3,
+                LSMR_BREAKDOWN = 4, LSMR_MAXITER = 5 };
+struct LsmrResult { std::vector<double> x; bool converged; i64 iterations; double residual_norm; int stop; };
+LsmrResult lsmr(const RectOp& A, const double* b, const LinOp* M, double tol, i64 maxiter, i64 local_size);
+
 struct Params {
-  int krylov = 0;  // 0 = CG, 1 = GMRES
+  int krylov = 0;  // 0 = CG, 1 = GMRES, 2 = LSMR
   i64 restart = 30;
   int explicit_op = 0;
   double tol = 1e-8;
   i64 maxiter = 1000;
   i64 max_refinements = 2;
   int precond = 1;  // 0 = none, 1 = additive, 2 = multiplicative
+  i64 local_size = 0;  // LSMR reorthogonalisation window (0 = None)
   LocalSolverConfig local;
 };
 

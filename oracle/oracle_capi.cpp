@@ -37,9 +37,56 @@ struct OracleParams {
   int schur_split;
   int dense_threshold;
   int64_t dense_core;
+  int64_t local_size;    // LSMR window (0 = None)
 };
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
+
+// LSMR on a dense row-major rows x cols matrix, optionally with the diagonal
+// preconditioner M^-1 = diag(minv) (the operators of lsmr/tests.rs).
+// info = [converged, iterations, residual_norm, stop].
+int oracle_lsmr_dense(int64_t rows, int64_t cols, const double* a, const double* b, const double* minv,
+                      double tol, int64_t maxiter, int64_t local_size, double* x, double* info) {
+  struct Dense : RectOp {
+    int64_t r, c;
+    const double* a;
+    i64 rows() const override { return r; }
+    i64 cols() const override { return c; }
+    void apply(const double* x, double* y) const override {
+      for (int64_t i = 0; i < r; ++i) {
+        double s = 0.0;
+        for (int64_t j = 0; j < c; ++j) s += a[i * c + j] * x[j];
+        y[i] = s;
+      }
+    }
+    void adjoint(const double* u, double* x) const override {
+      for (int64_t j = 0; j < c; ++j) {
+        double s = 0.0;
+        for (int64_t i = 0; i < r; ++i) s += a[i * c + j] * u[i];
+        x[j] = s;
+      }
+    }
+  };
+  struct Diag : LinOp {
+    int64_t n;
+    const double* d;
+    void apply(const double* x, double* y) const override {
+      for (int64_t i = 0; i < n; ++i) y[i] = d[i] * x[i];
+    }
+  };
+  try {
+    Dense A;
+    A.r = rows; A.c = cols; A.a = a;
+    Diag M;
+    M.n = cols; M.d = minv;
+    LsmrResult r = lsmr(A, b, minv ? &M : nullptr, tol, maxiter, local_size);
+    std::copy(r.x.begin(), r.x.end(), x);
+    info[0] = r.converged; info[1] = (double)r.iterations; info[2] = r.residual_norm; info[3] = r.stop;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
 
 int oracle_solver_new(const uint32_t* codes, int64_t n_obs, int n_factors, int64_t ld,
                       const double* weights, const OracleParams* p, void** out) {
@@ -59,6 +106,7 @@ int oracle_solver_new(const uint32_t* codes, int64_t n_obs, int n_factors, int64
     pr.local.schur_split = p->schur_split;
     pr.local.dense_threshold = p->dense_threshold;
     pr.local.dense_core = p->dense_core;
+    pr.local_size = p->local_size;
     Design d = make_design(codes, n_obs, n_factors, ld, weights);
     *out = new Solver(std::move(d), pr);
     return 0;

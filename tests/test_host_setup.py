@@ -103,8 +103,14 @@ def test_config_validation_mirrors_within_py():
     # CG / LSMR with multiplicative Schwarz (within-py/src/lib.rs:639-656)
     with pytest.raises(ValueError, match="symmetric"):
         host_solver(codes, config=W.CG(), preconditioner=W.MultiplicativeSchwarz())
+    with pytest.raises(ValueError, match="LSMR requires a symmetric"):
+        host_solver(codes, config=W.LSMR(), preconditioner=W.MultiplicativeSchwarz())
     with pytest.raises(ValueError):
-        host_solver(codes, config=W.LSMR())
+        W.LSMR(local_size=-1)
+    s = host_solver(codes, config=W.LSMR(local_size=5))
+    p = N.Params()
+    N.check(N.load().wa_solver_params(s._h, C.byref(p)))
+    assert (p.krylov, p.local_size, p.operator_repr, p.max_refinements) == (2, 5, 0, 0)
     with pytest.raises(TypeError):
         host_solver(codes, config="cg")
     with pytest.raises(TypeError):

@@ -43,6 +43,7 @@ class Params(C.Structure):
         ("precision", C.c_int32),
         ("block", C.c_int32),
         ("host_only", C.c_int32),
+        ("local_size", C.c_int32),
         ("local", LocalConfig),
     ]
 

@@ -87,13 +87,20 @@ class GMRES:
 
 
 class LSMR:
-    """Accepted for API compatibility; solving with it raises ValueError
-    (modified LSMR is not part of the MI355X path)."""
+    """Modified LSMR on W^(1/2) D with the Schwarz preconditioner as M ~ D'WD
+    (crates/within-py/src/lib.rs:398-421; schwarz-precond/src/solve/lsmr.rs).
+    ``local_size``: windowed reorthogonalisation over that many past vectors
+    (None = off).  Runs in FP64 on the GPU with no refinement step."""
 
     def __init__(self, tol=1e-8, maxiter=1000, local_size=None):
         self.tol = float(tol)
         self.maxiter = int(maxiter)
-        self.local_size = local_size
+        if local_size is not None and int(local_size) < 0:
+            raise ValueError("local_size must be >= 0")
+        self.local_size = None if local_size is None else int(local_size)
+
+    def __repr__(self):
+        return f"LSMR(tol={self.tol}, maxiter={self.maxiter}, local_size={self.local_size})"
 
 
 class ApproxCholConfig:
@@ -217,22 +224,27 @@ def _params(config, preconditioner):
     L.wa_params_default(C.byref(p))
     if config is None:
         config = CG()
-    if isinstance(config, LSMR):
-        raise ValueError("LSMR is not provided by within_amd (CG and GMRES are)")
-    if not isinstance(config, (CG, GMRES)):
+    if not isinstance(config, (CG, GMRES, LSMR)):
         raise TypeError("config must be CG, GMRES, or LSMR")
     kind, loc, reduction = _precond_config(preconditioner)
-    if isinstance(config, CG) and kind == 2:
-        raise ValueError("CG requires a symmetric preconditioner; use 'additive' or switch to GMRES")
-    p.krylov = 1 if isinstance(config, GMRES) else 0
-    p.restart = config.restart if isinstance(config, GMRES) else 30
-    p.operator_repr = int(config.operator)
+    if isinstance(config, (CG, LSMR)) and kind == 2:
+        method = "CG" if isinstance(config, CG) else "LSMR"
+        raise ValueError(f"{method} requires a symmetric preconditioner; use 'additive' or switch to GMRES")
     p.preconditioner = kind
     p.tol = config.tol
     p.maxiter = config.maxiter
-    p.max_refinements = config.max_refinements
-    p.precision = _PRECISIONS[config.precision]
-    p.block = 0 if config.block is None else int(config.block)
+    if isinstance(config, LSMR):  # lib.rs:621-631: implicit operator, no refinement
+        p.krylov = 2
+        p.local_size = config.local_size or 0
+        p.operator_repr = 0
+        p.max_refinements = 0
+    else:
+        p.krylov = 1 if isinstance(config, GMRES) else 0
+        p.restart = config.restart if isinstance(config, GMRES) else 30
+        p.operator_repr = int(config.operator)
+        p.max_refinements = config.max_refinements
+        p.precision = _PRECISIONS[config.precision]
+        p.block = 0 if config.block is None else int(config.block)
     (p.local.ac_seed, p.local.ac_split, p.local.approx_schur, p.local.schur_seed, p.local.schur_split,
      p.local.dense_threshold) = loc
     return p, reduction

@@ -522,6 +522,366 @@ void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, dou
   k_cg_beta<<<1, kScalarBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, itn, kp);
 }
 
+// ===========================================================================
+// LSMR (schwarz-precond/src/solve/lsmr.rs:122-316) over W^(1/2) D: the
+// observation-space half of the modified Golub-Kahan step, the per-column
+// scalar recurrences (one thread per column, FP64) and the fused DOF-space
+// vector updates.  Columns advance in lockstep; a column that has stopped is
+// left untouched by every vector kernel.
+// ===========================================================================
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_lsmr_rhs(int64_t n, const double* __restrict__ sw,
+                                                     const double* __restrict__ Y, double* __restrict__ U, int kp,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kBlock * NV];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < n; i += (int64_t)gridDim.x * gpb) {
+    VecT<double, NV> u = vload<double, NV>(Y + i * kp + pt * NV);
+    const double w = sw ? sw[i] : 1.0;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      u.v[c] = sw ? w * u.v[c] : u.v[c];
+      acc[c] += u.v[c] * u.v[c];
+    }
+    vstore<double, NV>(U + i * kp + pt * NV, u);
+  }
+  block_col_sums<NV>(acc, sh, kp, gpb, part);
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_lsmr_fwd(int64_t n, int Q, const uint32_t* __restrict__ codes,
+                                                     const double* __restrict__ sw, const double* __restrict__ X,
+                                                     double* __restrict__ U, const LsmrCol* __restrict__ cs, int kp,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kBlock * NV];
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  double acc[NV], sc[NV];
+  bool act[NV], any = false;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const LsmrCol& q = cs[pt * NV + c];
+    act[c] = q.active != 0;
+    sc[c] = -(q.alpha * q.beta_prev_inv);
+    acc[c] = 0.0;
+    any = any || act[c];
+  }
+  if (any) {
+    for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < n; i += (int64_t)gridDim.x * gpb) {
+      double s[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) s[c] = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const VecT<double, NV> x = vload<double, NV>(X + (int64_t)codes[(int64_t)q * n + i] * kp + pt * NV);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) s[c] += x.v[c];
+      }
+      const double w = sw ? sw[i] : 1.0;
+      VecT<double, NV> u = vload<double, NV>(U + i * kp + pt * NV);
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        if (act[c]) {
+          const double av = sw ? s[c] * w : s[c];
+          const double val = av + sc[c] * u.v[c];
+          u.v[c] = val;
+          acc[c] += val * val;
+        }
+      vstore<double, NV>(U + i * kp + pt * NV, u);
+    }
+  }
+  block_col_sums<NV>(acc, sh, kp, gpb, part);
+}
+
+static inline unsigned lsmr_blocks(int64_t n, int kp) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(n, kp), kLsmrBlocks));
+}
+
+int launch_lsmr_rhs(hipStream_t st, int64_t n, const double* sw, const double* Y, double* U, int kp, double* part) {
+  const unsigned nb = lsmr_blocks(n, kp);
+  WA_VEC_SWITCH(double, kp, (k_lsmr_rhs<NV><<<nb, kBlock, 0, st>>>(n, sw, Y, U, kp, part)));
+  return (int)nb;
+}
+
+int launch_lsmr_fwd(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const double* sw, const double* V,
+                    double* U, const LsmrCol* cs, int kp, double* part) {
+  const unsigned nb = lsmr_blocks(n, kp);
+  WA_VEC_SWITCH(double, kp, (k_lsmr_fwd<NV><<<nb, kBlock, 0, st>>>(n, Q, codes, sw, V, U, cs, kp, part)));
+  return (int)nb;
+}
+
+// beta_1 = ||b|| (lsmr.rs:188-201, bidiag.rs:551-560): a zero right-hand side
+// stops at once (x = 0, converged); a non-finite one is flagged
+__global__ void __launch_bounds__(kScalarBlock) k_lsmr_start(LsmrCol* __restrict__ cs, const double* __restrict__ part,
+                                                       int nb, double tol, int k, int kp) {
+  __shared__ double sh[kScalarBlock];
+  const double v = col_total(part, nb, kp, sh);
+  const int c = threadIdx.x;
+  if (c >= kp) return;
+  LsmrCol q = {};
+  const double beta = sqrt(v);
+  q.beta = beta;
+  q.bnorm = beta;
+  q.rel_tol = tol;
+  q.abs_tol = tol * beta;
+  q.c_bar = 1.0;
+  q.prev_rho = q.prev_rho_bar = 1.0;
+  if (c < k && !isfinite(v)) q.bad = 1;
+  if (c < k && isfinite(v) && beta > 0.0) {
+    q.active = 1;
+    q.beta_inv = 1.0 / beta;
+    q.beta_prev_inv = q.beta_inv;
+  } else {
+    q.converged = 1;  // zero right-hand side (or padding, or flagged)
+  }
+  cs[c] = q;
+}
+
+// alpha_1 = sqrt(<v, p~>) and the recurrence seeds (bidiag.rs:562-589,
+// recurrence.rs:78-86, 240-246); A'b = 0 stops with x = 0 and |r| = ||b||
+__global__ void __launch_bounds__(kScalarBlock) k_lsmr_alpha1(LsmrCol* __restrict__ cs,
+                                                        const double* __restrict__ part, int nb, int kp,
+                                                        int* __restrict__ n_active) {
+  __shared__ double sh[kScalarBlock];
+  const double v = col_total(part, nb, kp, sh);
+  const int c = threadIdx.x;
+  if (c < kp) {
+    LsmrCol& q = cs[c];
+    q.inv_alpha = 0.0;
+    if (q.active) {
+      const double alpha = v > 0.0 ? sqrt(v) : 0.0;
+      q.alpha = alpha;
+      q.inv_alpha = alpha > 0.0 ? 1.0 / alpha : 0.0;
+      q.alpha_bar = alpha;
+      q.phi_bar = q.bnorm;
+      q.zeta_bar = alpha * q.bnorm;
+      q.a_norm_sq = alpha * alpha;
+      if (alpha == 0.0) {
+        q.active = 0;
+        q.converged = 1;
+        q.stop = 1;
+        q.resid = q.bnorm;
+      }
+    }
+  }
+  __syncthreads();
+  if (c == 0) {
+    int a = 0;
+    for (int j = 0; j < kp; ++j) a += cs[j].active;
+    *n_active = a;
+  }
+}
+
+// beta_{k+1} = ||u|| (bidiag.rs:348-366); a zero beta zeroes p~ and so v
+__global__ void __launch_bounds__(kScalarBlock) k_lsmr_beta(LsmrCol* __restrict__ cs, const double* __restrict__ part,
+                                                      int nb, int kp) {
+  __shared__ double sh[kScalarBlock];
+  const double v = col_total(part, nb, kp, sh);
+  const int c = threadIdx.x;
+  if (c >= kp) return;
+  LsmrCol& q = cs[c];
+  if (!q.active) return;
+  const double beta = sqrt(v);
+  q.beta = beta;
+  if (beta == 0.0) {
+    q.beta_inv = 0.0;
+    q.p_coeff = 0.0;
+  } else {
+    q.beta_inv = 1.0 / beta;
+    q.p_coeff = beta / q.alpha;
+  }
+}
+
+struct DevGivens {  // recurrence.rs:20-30
+  double c, s, r;
+  __device__ DevGivens(double a, double b) {
+    r = hypot(a, b);
+    if (r > 0.0) {
+      c = a / r;
+      s = b / r;
+    } else {
+      c = 1.0;
+      s = 0.0;
+    }
+  }
+};
+
+// alpha_{k+1} = sqrt(<v, p~>), both rotations, the x / h / hbar coefficients
+// and the stopping tests (recurrence.rs:89-138, 181-229, 263-275; lsmr.rs:225-251)
+__global__ void __launch_bounds__(kScalarBlock) k_lsmr_step(LsmrCol* __restrict__ cs, const double* __restrict__ part,
+                                                      int nb, int itn, int kp, int* __restrict__ n_active) {
+  __shared__ double sh[kScalarBlock];
+  const double v = col_total(part, nb, kp, sh);
+  const int c = threadIdx.x;
+  if (c < kp) {
+    LsmrCol& q = cs[c];
+    q.upd = q.active;
+    if (q.active) {
+      const double eps = 2.220446049250313e-16, tiny = 2.2250738585072014e-308;
+      const double alpha = v > 0.0 ? sqrt(v) : 0.0, beta = q.beta;
+      q.a_norm_sq += alpha * alpha + beta * beta;
+      const DevGivens ph(q.alpha_bar, beta);
+      const double theta_new = ph.s * alpha;
+      const double alpha_bar_new = -ph.c * alpha, phi_bar_new = ph.s * q.phi_bar;
+      const double theta_bar = q.s_bar * ph.r;
+      const DevGivens pb(q.c_bar * ph.r, theta_new);
+      const double zeta = pb.c * q.zeta_bar, zeta_bar_new = -pb.s * q.zeta_bar;
+      q.alpha_bar = alpha_bar_new;
+      q.phi_bar = phi_bar_new;
+      q.c_bar = pb.c;
+      q.s_bar = pb.s;
+      q.zeta_bar = zeta_bar_new;
+      const double rho = ph.r, rho_bar = pb.r;
+      const double dx = rho * rho_bar, dh = q.prev_rho * q.prev_rho_bar;
+      q.t_x = fabs(dx) > eps ? zeta / dx : 0.0;
+      q.t_hbar = fabs(dh) > eps ? theta_bar * rho / dh : 0.0;
+      q.t_h = fabs(rho) > eps ? theta_new / rho : 0.0;
+      q.prev_rho = rho;
+      q.prev_rho_bar = rho_bar;
+      q.inv_alpha = alpha > 0.0 ? 1.0 / alpha : 0.0;
+      q.alpha = alpha;
+      q.beta_prev_inv = q.beta_inv;
+      q.iters = itn;
+      const double resid = fabs(q.phi_bar);
+      q.resid = resid;
+      int stop = -1;
+      if (resid <= q.abs_tol) {
+        stop = 2;
+      } else {
+        const double a_norm = fmax(sqrt(q.a_norm_sq), tiny);
+        if (fabs(q.zeta_bar) / (a_norm * fmax(resid, tiny)) <= q.rel_tol) stop = 3;
+      }
+      if (stop < 0 && alpha == 0.0) stop = 4;
+      if (stop >= 0) {
+        q.active = 0;
+        q.converged = 1;
+        q.stop = stop;
+      }
+    }
+  }
+  __syncthreads();
+  if (c == 0) {
+    int a = 0;
+    for (int j = 0; j < kp; ++j) a += cs[j].active;
+    *n_active = a;
+  }
+}
+
+void launch_lsmr_start(hipStream_t st, LsmrCol* cs, const double* part, int nb, double tol, int k, int kp) {
+  k_lsmr_start<<<1, kScalarBlock, 0, st>>>(cs, part, nb, tol, k, kp);
+}
+void launch_lsmr_alpha1(hipStream_t st, LsmrCol* cs, const double* part, int nb, int kp, int* n_active) {
+  k_lsmr_alpha1<<<1, kScalarBlock, 0, st>>>(cs, part, nb, kp, n_active);
+}
+void launch_lsmr_beta(hipStream_t st, LsmrCol* cs, const double* part, int nb, int kp) {
+  k_lsmr_beta<<<1, kScalarBlock, 0, st>>>(cs, part, nb, kp);
+}
+void launch_lsmr_step(hipStream_t st, LsmrCol* cs, const double* part, int nb, int itn, int kp, int* n_active) {
+  k_lsmr_step<<<1, kScalarBlock, 0, st>>>(cs, part, nb, itn, kp, n_active);
+}
+
+// DOF-space vector kernels: one lane per (row, column) value
+__global__ void __launch_bounds__(kBlock) k_lsmr_ptilde(double* __restrict__ P, const double* __restrict__ ATU,
+                                                        const LsmrCol* __restrict__ cs, int64_t m, int kp) {
+  const int lane = threadIdx.x % kp;
+  const LsmrCol& q = cs[lane];
+  if (!q.active) return;
+  const double bi = q.beta_inv, pc = -q.p_coeff;
+  const int gpb = kBlock / kp;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + lane;
+    P[id] = bi * ATU[id] + pc * P[id];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_lsmr_init_vec(double* __restrict__ V, double* __restrict__ H,
+                                                          double* __restrict__ HB, double* __restrict__ X,
+                                                          const LsmrCol* __restrict__ cs, int64_t m, int kp) {
+  const int lane = threadIdx.x % kp;
+  const double ia = cs[lane].inv_alpha;
+  const int gpb = kBlock / kp;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + lane;
+    const double v = V[id] * ia;
+    V[id] = v;
+    H[id] = v;
+    HB[id] = 0.0;
+    X[id] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_lsmr_update(double* __restrict__ V, double* __restrict__ H,
+                                                        double* __restrict__ HB, double* __restrict__ X,
+                                                        const LsmrCol* __restrict__ cs, int64_t m, int kp) {
+  const int lane = threadIdx.x % kp;
+  const LsmrCol& q = cs[lane];
+  if (!q.upd) return;
+  const double ia = q.inv_alpha, tx = q.t_x, thb = q.t_hbar, th = q.t_h;
+  const bool scale = ia > 0.0;
+  const int gpb = kBlock / kp;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + lane;
+    const double v = scale ? V[id] * ia : V[id];
+    const double h_old = H[id];
+    const double hb = h_old - thb * HB[id];
+    HB[id] = hb;
+    X[id] += tx * hb;
+    H[id] = v - th * h_old;
+    V[id] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_lsmr_reorth(double* __restrict__ V, double* __restrict__ P,
+                                                        const double* __restrict__ Vj, const double* __restrict__ Pj,
+                                                        const double* __restrict__ coef,
+                                                        const LsmrCol* __restrict__ cs, int64_t m, int kp) {
+  const int lane = threadIdx.x % kp;
+  if (!cs[lane].active) return;
+  const double c = -coef[lane];
+  const int gpb = kBlock / kp;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + lane;
+    V[id] = c * Vj[id] + V[id];
+    P[id] = c * Pj[id] + P[id];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_lsmr_push(const double* __restrict__ V, const double* __restrict__ P,
+                                                      double* __restrict__ Vs, double* __restrict__ Ps,
+                                                      const LsmrCol* __restrict__ cs, int64_t m, int kp) {
+  const int lane = threadIdx.x % kp;
+  const double ia = cs[lane].inv_alpha;
+  const int gpb = kBlock / kp;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / kp; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + lane;
+    Vs[id] = V[id];
+    Ps[id] = P[id] * ia;
+  }
+}
+
+static inline unsigned vec_blocks(int64_t m, int kp) { return grid_stride_blocks(m, kp); }
+
+void launch_lsmr_ptilde(hipStream_t st, double* P, const double* ATU, const LsmrCol* cs, int64_t m, int kp) {
+  if (m > 0) k_lsmr_ptilde<<<vec_blocks(m, kp), kBlock, 0, st>>>(P, ATU, cs, m, kp);
+}
+void launch_lsmr_init_vec(hipStream_t st, double* V, double* H, double* HB, double* X, const LsmrCol* cs, int64_t m,
+                          int kp) {
+  if (m > 0) k_lsmr_init_vec<<<vec_blocks(m, kp), kBlock, 0, st>>>(V, H, HB, X, cs, m, kp);
+}
+void launch_lsmr_update(hipStream_t st, double* V, double* H, double* HB, double* X, const LsmrCol* cs, int64_t m,
+                        int kp) {
+  if (m > 0) k_lsmr_update<<<vec_blocks(m, kp), kBlock, 0, st>>>(V, H, HB, X, cs, m, kp);
+}
+void launch_lsmr_reorth(hipStream_t st, double* V, double* P, const double* Vj, const double* Pj, const double* c,
+                        const LsmrCol* cs, int64_t m, int kp) {
+  if (m > 0) k_lsmr_reorth<<<vec_blocks(m, kp), kBlock, 0, st>>>(V, P, Vj, Pj, c, cs, m, kp);
+}
+void launch_lsmr_push(hipStream_t st, const double* V, const double* P, double* Vs, double* Ps, const LsmrCol* cs,
+                      int64_t m, int kp) {
+  if (m > 0) k_lsmr_push<<<vec_blocks(m, kp), kBlock, 0, st>>>(V, P, Vs, Ps, cs, m, kp);
+}
+
 // Preconditioner elementwise kernels (same lane mapping as the vector ops).
 
 // Rr[j] -= sum2[s] / n_red(s)

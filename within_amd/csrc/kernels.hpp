@@ -100,6 +100,42 @@ struct ColState {
   int active, converged, iters, pad2;
 };
 
+// per-column LSMR state (schwarz-precond/src/solve/lsmr/bidiag.rs,
+// recurrence.rs): bidiagonalisation scalars, the two Givens chains, the
+// x / h / hbar coefficients of the current step and the stopping state
+struct LsmrCol {
+  double alpha, beta, beta_inv, beta_prev_inv, p_coeff;
+  double alpha_bar, phi_bar, c_bar, s_bar, zeta_bar, a_norm_sq, prev_rho, prev_rho_bar;
+  double abs_tol, rel_tol, bnorm, resid;
+  double inv_alpha, t_x, t_hbar, t_h;
+  int active, upd, converged, iters, stop, bad;
+};
+constexpr int kLsmrBlocks = 2048;  // fixed grid of the observation-space LSMR reductions
+
+// U = sw o Y (sw == nullptr: unit), part = column sums of U^2 (kLsmrBlocks blocks)
+int launch_lsmr_rhs(hipStream_t st, int64_t n, const double* sw, const double* Y, double* U, int kp, double* part);
+// U = sw o (D V) + s_c U with s_c = -alpha_c / beta_c (active columns), part = sums of U^2
+int launch_lsmr_fwd(hipStream_t st, int64_t n, int Q, const uint32_t* codes, const double* sw, const double* V,
+                    double* U, const LsmrCol* cs, int kp, double* part);
+void launch_lsmr_start(hipStream_t st, LsmrCol* cs, const double* part, int nb, double tol, int k, int kp);
+void launch_lsmr_alpha1(hipStream_t st, LsmrCol* cs, const double* part, int nb, int kp, int* n_active);
+void launch_lsmr_beta(hipStream_t st, LsmrCol* cs, const double* part, int nb, int kp);
+void launch_lsmr_step(hipStream_t st, LsmrCol* cs, const double* part, int nb, int itn, int kp, int* n_active);
+// P = beta_inv ATU - p_coeff P (active columns)
+void launch_lsmr_ptilde(hipStream_t st, double* P, const double* ATU, const LsmrCol* cs, int64_t m, int kp);
+// V *= inv_alpha; H = V; HB = 0; X = 0 (every column)
+void launch_lsmr_init_vec(hipStream_t st, double* V, double* H, double* HB, double* X, const LsmrCol* cs, int64_t m,
+                          int kp);
+// V *= inv_alpha; HB = H - t_hbar HB; X += t_x HB; H = V - t_h H (columns updated this step)
+void launch_lsmr_update(hipStream_t st, double* V, double* H, double* HB, double* X, const LsmrCol* cs, int64_t m,
+                        int kp);
+// windowed Gram-Schmidt: V -= c Vj, P -= c Pj (active columns); ring push:
+// Vs = V, Ps = inv_alpha P
+void launch_lsmr_reorth(hipStream_t st, double* V, double* P, const double* Vj, const double* Pj, const double* c,
+                        const LsmrCol* cs, int64_t m, int kp);
+void launch_lsmr_push(hipStream_t st, const double* V, const double* P, double* Vs, double* Ps, const LsmrCol* cs,
+                      int64_t m, int kp);
+
 template <typename T>
 void launch_pull(hipStream_t st, const DevPass<T>& p, const T* X, T* Y, T* part, int kp);
 void launch_scatter_obs(hipStream_t st, const DevPass<double>& p, const double* w, const double* V, double* out,

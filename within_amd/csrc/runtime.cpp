@@ -126,6 +126,14 @@ struct wa_solver {
   double* sc = nullptr;      // small per-column scalars
   int* imask = nullptr;
   ColState* cs = nullptr;
+  // LSMR (allocated on its first solve): sqrt weights per observation,
+  // observation-space partials, per-column state, reorthogonalisation ring
+  double* swobs = nullptr;
+  double* lpart = nullptr;
+  double* lcoef = nullptr;
+  LsmrCol* lcs = nullptr;
+  double *ring_v = nullptr, *ring_p = nullptr;
+  int64_t ring_cap = 0;
   double* hbuf = nullptr;    // pinned host scratch
   Timer timer;
   // GMRES bases, allocated on first use
@@ -234,6 +242,7 @@ struct wa_solver {
     if (params.krylov == 1) gmres<T>(k, kp, rhs, x, tol, conv, its);
     else cg<T>(k, kp, rhs, x, tol, conv, its);
   }
+  void lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>& its);
   void solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
                    int64_t ldd, wa_rhs_stats* stats);
   std::vector<double> col_norms(const double* V, int64_t rows, int kp);
@@ -871,6 +880,121 @@ void wa_solver::gmres(int k, int kp, const T* b, T* x, const std::vector<double>
 
 // One block of up to 32 right-hand sides (Solver::solve per column,
 // solver.rs:158-239 + iterative_refinement :296-347).
+// LSMR over W^(1/2) D with the Schwarz preconditioner as M ~ D'WD
+// (solver.rs:162-207, lsmr.rs:122-316): the modified Golub-Kahan process for
+// every column of the block at once, FP64 throughout.  Without a
+// preconditioner M = I, which gives the plain Golub-Kahan recurrences in exact
+// arithmetic (lsmr/tests.rs:270-309).  Reads Y from Yr; leaves x in X64.
+void wa_solver::lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>& its) {
+  const Design& d = plan.d;
+  const int64_t vbytes = sizeof(double) * m * kp;
+  if (!lpart) {
+    lpart = dalloc<double>((int64_t)kLsmrBlocks * KMAX);
+    lcoef = dalloc<double>(KMAX);
+    lcs = dalloc<LsmrCol>(KMAX);
+    if (!d.weights.empty()) {
+      std::vector<double> sw(n);
+      for (int64_t i = 0; i < n; ++i) sw[i] = std::sqrt(d.weights[i]);
+      swobs = upload(sw);
+    }
+  }
+  const int64_t cap = std::min<int64_t>(params.local_size, std::min<int64_t>(n, mu));
+  if (cap > ring_cap) {
+    ring_v = dalloc<double>(cap * m * KMAX);
+    ring_p = dalloc<double>(cap * m * KMAX);
+    ring_cap = cap;
+  }
+  double *X = X64, *V = cx64, *P = cr64, *H = cz64, *HB = cp64, *ATU = cap64, *U = E;
+  int* nact = imask + KMAX;
+  const bool pre = has_pre && params.preconditioner != 0;
+  hipEvent_t ev = nullptr;
+  auto adjoint = [&]() {  // ATU = D' (W^(1/2) U)
+    tic(3, ev);
+    for (int q = 0; q < Q; ++q) launch_scatter_obs(st, p64[q], swobs, U, ATU, part64, kp);
+    toc(3, ev);
+  };
+  auto solve_m = [&]() {  // V = M^-1 P
+    if (pre) pre_apply<double>(P, V, kp);
+    else HIPCHK(hipMemcpyAsync(V, P, vbytes, hipMemcpyDeviceToDevice, st));
+  };
+  int64_t count = 0, next = 0;
+  auto push = [&]() {
+    if (cap == 0) return;
+    launch_lsmr_push(st, V, P, ring_v + next * m * KMAX, ring_p + next * m * KMAX, lcs, m, kp);
+    next = (next + 1) % cap;
+    if (count < cap) ++count;
+  };
+  // u = W^(1/2) y (unnormalised; 1/beta_1 is carried as beta_prev_inv),
+  // p~ = A'u / beta_1, v = M^-1 p~, alpha_1 = sqrt(<v, p~>)
+  tic(3, ev);
+  int nb = launch_lsmr_rhs(st, n, swobs, Yr, U, kp, lpart);
+  toc(3, ev);
+  tic(2, ev);
+  launch_lsmr_start(st, lcs, lpart, nb, params.tol, k, kp);
+  HIPCHK(hipMemsetAsync(P, 0, vbytes, st));
+  toc(2, ev);
+  adjoint();
+  tic(2, ev);
+  launch_lsmr_ptilde(st, P, ATU, lcs, m, kp);
+  toc(2, ev);
+  solve_m();
+  tic(2, ev);
+  nb = launch_dot_part<double>(st, V, P, m, kp, redp);
+  launch_lsmr_alpha1(st, lcs, redp, nb, kp, nact);
+  launch_lsmr_init_vec(st, V, H, HB, X, lcs, m, kp);
+  toc(2, ev);
+  push();
+  std::vector<LsmrCol> h(kp);
+  HIPCHK(hipMemcpyAsync(h.data(), lcs, sizeof(LsmrCol) * kp, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  bool any = false;
+  for (int c = 0; c < k; ++c) {
+    if (h[c].bad) throw Error(WA_ERR_ITERATIVE_SOLVE, "lsmr: right-hand side entries must be finite");
+    any = any || h[c].active;
+  }
+  int64_t itn = 0;
+  while (any && itn < params.maxiter) {
+    ++itn;
+    tic(3, ev);
+    nb = launch_lsmr_fwd(st, n, Q, codes, swobs, V, U, lcs, kp, lpart);
+    toc(3, ev);
+    tic(2, ev);
+    launch_lsmr_beta(st, lcs, lpart, nb, kp);
+    toc(2, ev);
+    adjoint();
+    tic(2, ev);
+    launch_lsmr_ptilde(st, P, ATU, lcs, m, kp);
+    toc(2, ev);
+    solve_m();
+    tic(2, ev);
+    const int64_t start = count < cap ? 0 : next;
+    for (int64_t i = 0; i < count; ++i) {  // oldest first (bidiag.rs:206-217)
+      const int64_t j = (start + i) % cap;
+      launch_dot<double>(st, V, ring_p + j * m * KMAX, m, kp, redp, lcoef);
+      launch_lsmr_reorth(st, V, P, ring_v + j * m * KMAX, ring_p + j * m * KMAX, lcoef, lcs, m, kp);
+    }
+    nb = launch_dot_part<double>(st, V, P, m, kp, redp);
+    launch_lsmr_step(st, lcs, redp, nb, (int)itn, kp, nact);
+    launch_lsmr_update(st, V, H, HB, X, lcs, m, kp);
+    toc(2, ev);
+    push();
+    HIPCHK(hipMemcpyAsync(hbuf, nact, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    any = *(int*)hbuf > 0;
+  }
+  HIPCHK(hipMemcpyAsync(h.data(), lcs, sizeof(LsmrCol) * kp, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int c = 0; c < k; ++c) {
+    if (h[c].active) {  // maxiter reached
+      conv[c] = 0;
+      its[c] = params.maxiter;
+    } else {
+      conv[c] = h[c].converged;
+      its[c] = h[c].iters;
+    }
+  }
+}
+
 void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
                             int64_t ldd, wa_rhs_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
@@ -889,10 +1013,12 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
   std::vector<int> conv(kb, 0), cconv(kb, 0);
   std::vector<int64_t> its(kb, 0), cits(kb, 0);
   const int64_t vbytes = sizeof(double) * m * kp;
-  const bool mixed = params.precision == 1;
+  const bool mixed = params.precision == 1 && params.krylov != 2;
   bool fresh = false;      // E and R64 hold y - Dx and D'W(y - Dx) for the current X64
   bool dof_final = false;  // R64 holds b - Gx for the current X64 (E is stale)
-  if (!mixed) {
+  if (params.krylov == 2) {
+    lsmr(kb, kp, conv, its);
+  } else if (!mixed) {
     std::vector<double> tol(kb, params.tol);
     krylov<double>(kb, kp, B64, cx64, tol, conv, its);
     HIPCHK(hipMemcpyAsync(X64, cx64, vbytes, hipMemcpyDeviceToDevice, st));
@@ -1128,8 +1254,10 @@ static wa_status create_impl(const wa_design* d, const wa_params* p, const uint8
   try {
     if (!d || !p || !out) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (p->krylov == 2) throw Error(WA_ERR_UNSUPPORTED, "LSMR is not provided by within_amd");
-    if (p->krylov != 0 && p->krylov != 1) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
+    if (p->krylov < 0 || p->krylov > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
+    if (p->krylov == 2 && p->preconditioner == 2)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR requires a symmetric preconditioner; use additive or GMRES");
+    if (p->local_size < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR local_size must be >= 0");
     if (p->preconditioner < 0 || p->preconditioner > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown preconditioner");
     if (p->preconditioner == 2 && p->krylov == 0)
       throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
@@ -1155,8 +1283,8 @@ static wa_status create_impl(const wa_design* d, const wa_params* p, const uint8
       const int64_t nd = deserialize_precond(pre, pre_len, pl);
       if (nd != pl.d.n_dofs)
         throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner was built for a design with a different number of DOFs");
-      if (pl.precond == 2 && p->krylov == 0)
-        throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
+      if (pl.precond == 2 && p->krylov != 1)
+        throw Error(WA_ERR_INVALID_ARGUMENT, "CG and LSMR require a symmetric preconditioner; use additive or GMRES");
       std::vector<char> seen(pl.d.n_dofs, 0);
       for (int q = 0; q < pl.d.Q; ++q)
         for (u32 c : pl.d.codes[q]) seen[pl.d.offset[q] + c] = 1;
@@ -1246,9 +1374,10 @@ wa_status wa_solver_set_params(wa_solver* s, const wa_params* p) {
       throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner settings are fixed when the solver is built");
     if (p->operator_repr == 1 && !s->has_gram)
       throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no explicit Gramian");
-    if (p->krylov == 2) throw Error(WA_ERR_UNSUPPORTED, "LSMR is not provided by within_amd");
-    if (p->krylov == 0 && p->preconditioner == 2)
-      throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner");
+    if (p->krylov < 0 || p->krylov > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
+    if (p->krylov != 1 && p->preconditioner == 2)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "CG and LSMR require a symmetric preconditioner");
+    if (p->local_size < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR local_size must be >= 0");
     if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
     s->params = *p;
     return WA_OK;
