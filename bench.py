@@ -232,6 +232,16 @@ def main():
     errf = (torch.linalg.vector_norm(Df - Dref, dim=1) / torch.linalg.vector_norm(Dref, dim=1)).cpu().numpy()
     fp64_block_rate = sum(bool(stf[j][0]) and errf[j] <= 1e-8 for j in range(k)) / tf
     log(f"bench: FP64 block {k} RHS in {tf * 1e3:.1f} ms ({fp64_block_rate:.2f} RHS/s)")
+
+    # --- LSMR (FP64, all k RHS in one block) on the same preconditioner --------
+    log("bench: LSMR block solve ...")
+    base.s.set_config(W.LSMR(tol=args.tol))
+    base.solve(Y, k, torch, Xf, Df)  # first LSMR call allocates its buffers
+    tl, stl = base.solve(Y, k, torch, Xf, Df)
+    errl = (torch.linalg.vector_norm(Df - Dref, dim=1) / torch.linalg.vector_norm(Dref, dim=1)).cpu().numpy()
+    lsmr_rate = sum(bool(stl[j][0]) and errl[j] <= 1e-8 for j in range(k)) / tl
+    log(f"bench: LSMR block {k} RHS in {tl * 1e3:.1f} ms ({lsmr_rate:.2f} RHS/s), iters={[s[1] for s in stl]}, "
+        f"max rel err {errl.max():.2e}")
     del Xf, Df
 
     value = sum(solved) / t_step
@@ -272,6 +282,8 @@ def main():
                    "scale": args.scale},
         "plain_fp64_k1_rhs_per_s": round(base_rate, 4),
         "fp64_block_rhs_per_s": round(fp64_block_rate, 3),
+        "lsmr_block_rhs_per_s": round(lsmr_rate, 3),
+        "lsmr_iterations": int(max(s[1] for s in stl)),
         "speedup_vs_plain_fp64": round(value / base_rate, 3) if base_rate > 0 else None,
         "iterations": [s[1] for s in st],
         "max_rel_err_vs_fp64_ref": float(err.max()),

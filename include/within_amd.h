@@ -152,6 +152,15 @@ wa_status wa_precond_create(const uint8_t* bytes, int64_t len, int32_t host_only
  * local config come from the bytes when it was built from them). */
 wa_status wa_solver_params(const wa_solver* s, wa_params* out);
 
+/* Scheduling metrics of the built Schwarz preconditioner
+ * (schwarz/additive/planning.rs:140-212, operator/preconditioner.rs:64-79;
+ * AdditiveSchwarzDiagnostics / subdomain_inner_parallel_work in
+ * within-py/src/lib.rs:197-258, 872-876): work[i] (n_subdomains entries, may
+ * be NULL) = 2 nnz(C) + n_local when the larger block of subdomain i exceeds
+ * 10000 rows, else 0 (local_solver.rs:348-356); out4 = total and max of work,
+ * total scatter entries (sum of local sizes), n_dofs. */
+wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4);
+
 /* ---- setup introspection (host data; no GPU needed) ------------------------
  * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
  * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,

@@ -63,7 +63,7 @@ EXPORTS = [
     "wa_solve_batch", "wa_solve_batch_device", "wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply",
     "wa_matvec_d", "wa_rmatvec_wdt", "wa_solver_info", "wa_solver_subdomain", "wa_solver_subdomain_factor",
     "wa_timing_enable", "wa_timing_reset", "wa_timing_get", "wa_solver_precond_bytes",
-    "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params",
+    "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
 ]
 
 _lib = None
@@ -116,6 +116,7 @@ def load():
     L.wa_solver_create_with_precond.argtypes = [P, C.POINTER(Params), P, i64, C.POINTER(P)]
     L.wa_precond_create.argtypes = [P, i64, i32, C.POINTER(P)]
     L.wa_solver_params.argtypes = [P, C.POINTER(Params)]
+    L.wa_precond_work.argtypes = [P, P, P]
     _lib = L
     return L
 

@@ -170,17 +170,29 @@ class BatchSolveResult:
 
 
 class AdditiveSchwarzDiagnostics:
-    def __init__(self, info, reduction):
+    """Build-time scheduling metrics (schwarz/additive/planning.rs:140-212;
+    within-py/src/lib.rs:197-258).  The device combine is a fixed-order
+    per-DOF gather (no atomics), so the resolved strategy is always
+    ``ParallelReduction``."""
+
+    def __init__(self, work4, reduction):
+        total, mx, scatter, n_dofs = (int(v) for v in work4)
         self.reduction_strategy = reduction
         self.resolved_reduction_strategy = ReductionStrategy.ParallelReduction
-        self.total_inner_parallel_work = int(info[7] * 2 + info[1])
-        self.max_inner_parallel_work = int(info[7] * 2 + info[1])
-        self.total_scatter_dofs = int(info[1])
-        self.outer_parallel_capacity = float(max(info[0], 1))
-        self.scatter_overlap = float(info[1]) / max(float(info[9]), 1.0)
+        self.total_inner_parallel_work = total
+        self.max_inner_parallel_work = mx
+        self.total_scatter_dofs = scatter
+        self.outer_parallel_capacity = 0.0 if mx == 0 else total / mx
+        self.scatter_overlap = scatter / max(n_dofs, 1)
 
     def __repr__(self):
-        return (f"AdditiveSchwarzDiagnostics(total_scatter_dofs={self.total_scatter_dofs}, "
+        return ("AdditiveSchwarzDiagnostics("
+                f"reduction_strategy={self.reduction_strategy.name}, "
+                f"resolved_reduction_strategy={self.resolved_reduction_strategy.name}, "
+                f"total_inner_parallel_work={self.total_inner_parallel_work}, "
+                f"max_inner_parallel_work={self.max_inner_parallel_work}, "
+                f"total_scatter_dofs={self.total_scatter_dofs}, "
+                f"outer_parallel_capacity={self.outer_parallel_capacity:.3f}, "
                 f"scatter_overlap={self.scatter_overlap:.3f})")
 
 
@@ -358,15 +370,22 @@ class FePreconditioner:
     def n_subdomains(self):
         return int(self._info()[0])
 
+    def _work(self):
+        w = np.zeros(max(self.n_subdomains, 1), dtype=np.int64)
+        out = np.zeros(4, dtype=np.int64)
+        N.check(N.load().wa_precond_work(self._h, N.ptr(w), N.ptr(out)))
+        return w[: self.n_subdomains], out
+
     @property
     def subdomain_inner_parallel_work(self):
-        return [0] * self.n_subdomains
+        """Estimated nested-parallel work per subdomain (preconditioner.rs:64-79)."""
+        return [int(v) for v in self._work()[0]]
 
     def additive_schwarz_diagnostics(self):
         if self._kind() != 1:
             return None
         red = self._solver._reduction if self._solver is not None else ReductionStrategy.Auto
-        return AdditiveSchwarzDiagnostics(self._info(), red)
+        return AdditiveSchwarzDiagnostics(self._work()[1], red)
 
     def __reduce__(self):
         return (FePreconditioner, (self.to_bytes(),))

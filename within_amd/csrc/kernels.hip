@@ -432,7 +432,8 @@ __device__ inline double col_total(const double* __restrict__ part, int nb, int 
   return sh[threadIdx.x];
 }
 
-// after A p: pap <= 0 -> stop (not converged); else alpha = rz / pap
+// after A p: pap <= 0 -> stop (not converged); else alpha = rz / pap (a NaN
+// pap or rz gives a NaN alpha, which carries into x as in cg.rs:77-86)
 __global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ part,
                                                      int nb, double* __restrict__ pap, double* __restrict__ alpha,
                                                      int itn, int kp) {
@@ -444,7 +445,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict_
   ColState& q = s[c];
   alpha[c] = 0.0;
   if (!q.active) return;
-  if (!(v > 0.0)) {
+  if (v <= 0.0) {
     q.active = 0;
     q.converged = 0;
     q.iters = itn;
@@ -469,6 +470,11 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict_
   if (q.rnorm / q.bnorm <= q.tol) {
     q.active = 0;
     q.converged = 1;
+  } else if (isnan(q.rnorm)) {
+    // a NaN residual has reached x and no later step can remove it: stop
+    // (not converged) instead of running out the iteration budget
+    q.active = 0;
+    q.converged = 0;
   }
 }
 

@@ -1197,7 +1197,8 @@ void build_plan(Plan& p) {
     flatten(p);
     p.meta.clear();
     for (const Subdomain& sd : p.subs)
-      p.meta.push_back({sd.q, sd.r, sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r(), (i64)sd.f.core.size()});
+      p.meta.push_back({sd.q, sd.r, sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r(), (i64)sd.f.core.size(), sd.ct.n_q(),
+                        sd.ct.n_r(), (i64)sd.ct.c.indices.size()});
   }
 }
 
@@ -1368,8 +1369,11 @@ i64 deserialize_precond(const uint8_t* buf, i64 len, Plan& p) {
   }
   {
     i64 se = 0, sc = 0;
-    for (const SubMeta& m : p.meta) {
-      ok = ok && m.q >= 0 && m.r >= 0 && m.n_elim >= 0 && m.n_core >= 0;
+    for (size_t s = 0; s < p.meta.size(); ++s) {
+      const SubMeta& m = p.meta[s];
+      ok = ok && m.q >= 0 && m.r >= 0 && m.n_elim >= 0 && m.n_core >= 0 && m.n_q >= 0 && m.n_r >= 0 &&
+           m.c_nnz >= 0 && (m.n_elim == m.n_q || m.n_elim == m.n_r) &&
+           (f.loc_ptr.size() != (size_t)f.n_sub + 1 || m.n_q + m.n_r == (i64)f.loc_ptr[s + 1] - f.loc_ptr[s]);
       se += m.n_elim;
       sc += m.n_core;
     }

@@ -165,10 +165,11 @@ struct PrecondFlat {
   i64 max_core = 0, ac_nnz = 0, c_nnz = 0, dense_subs = 0;
 };
 
-// What the device setup needs of each subdomain (kept also when the
-// preconditioner comes from serialized bytes and the subdomain objects are
-// not rebuilt): its factor pair, eliminated count and dense-core size.
-struct SubMeta { int32_t q = 0, r = 0; i64 n_elim = 0, n_core = 0; };
+// What the device setup and the scheduling metrics need of each subdomain
+// (kept also when the preconditioner comes from serialized bytes and the
+// subdomain objects are not rebuilt): its factor pair, eliminated count,
+// dense-core size, block sizes and cross-tab nonzeros.
+struct SubMeta { int32_t q = 0, r = 0; i64 n_elim = 0, n_core = 0, n_q = 0, n_r = 0, c_nnz = 0; };
 
 struct Plan {
   Design d;

@@ -1504,6 +1504,25 @@ wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, 
   });
 }
 
+wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4) {
+  if (!s || !out4) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
+  if (s->plan.meta.empty()) { g_err = "solver has no preconditioner"; return WA_ERR_INVALID_ARGUMENT; }
+  int64_t total = 0, mx = 0, scatter = 0;
+  for (size_t i = 0; i < s->plan.meta.size(); ++i) {
+    const SubMeta& m = s->plan.meta[i];
+    const int64_t w = std::max(m.n_q, m.n_r) > 10000 ? 2 * m.c_nnz + m.n_q + m.n_r : 0;
+    if (work) work[i] = w;
+    total += w;
+    mx = std::max(mx, w);
+    scatter += m.n_q + m.n_r;
+  }
+  out4[0] = total;
+  out4[1] = mx;
+  out4[2] = scatter;
+  out4[3] = s->plan.d.n_dofs;
+  return WA_OK;
+}
+
 wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
   if (!s || !info) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
   const PrecondFlat& f = s->plan.pre;
