@@ -192,3 +192,33 @@ def test_large_dense_core_is_bitwise_the_oracles(dense_core, oracle_mod):
             assert np.array_equal(a, np.ravel(oa[k])), (i, k)
         big = max(big, len(oa["core"]))
     assert big >= 512
+
+
+@pytest.mark.parametrize("dense_core", [1, 1500])
+def test_parallel_elimination_rounds_are_bitwise_the_oracles(dense_core, oracle_mod):
+    # a reduced system large enough that the rounds eliminate and merge in
+    # parallel (>= 1024 picks per round): the same per-entry sums in the same
+    # order as the oracle's sequential map-based rounds
+    codes = synth.akm_panel(150000, 15000, 6000, 8, seed=21)["codes"]
+    o = oracle_mod.OracleSolver(codes, None, ac_split=1, dense_core=dense_core)
+    prec = W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+    s = W.Solver(codes, preconditioner=prec, _host_only=True, _dense_core=dense_core)
+    L = N.load()
+    most = 0
+    for i in range(o.n_subdomains):
+        if o.subdomain(i)["dense"]:
+            continue
+        oa = o.subdomain_ac(i)
+        n = len(oa["pivot"])
+        nnz = C.c_int64()
+        arrs = dict(pivot=np.zeros(n), col_ptr=np.zeros(n + 1, np.uint32), col_idx=np.zeros(len(oa["col_idx"]), np.uint32),
+                    col_w=np.zeros(len(oa["col_w"])), round_ptr=np.zeros(len(oa["round_ptr"]), np.uint32),
+                    order=np.zeros(len(oa["order"]), np.uint32), core=np.zeros(len(oa["core"]), np.uint32),
+                    core_inv=np.zeros(oa["core_inv"].size))
+        N.check(L.wa_solver_subdomain_factor(s._h, i, C.byref(nnz), *[N.ptr(a) for a in arrs.values()]))
+        for k, a in arrs.items():
+            assert np.array_equal(a, np.ravel(oa[k])), (i, k)
+        rp = np.ravel(oa["round_ptr"]).astype(np.int64)
+        if len(rp) > 1:
+            most = max(most, int(np.diff(rp).max()))
+    assert most >= 1024

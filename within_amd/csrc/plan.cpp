@@ -544,24 +544,41 @@ static inline uint64_t round_key(uint64_t seed, uint64_t round, uint64_t v) {
 // + v), merged after the round in (v ascending, emission) order.  Once at most
 // `dense_core` vertices remain they are factored exactly (anchored at the
 // last one); a core whose anchored minor is singular falls back to rounds.
-static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 dense_core) {
+static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 dense_core, bool report = false) {
+  const auto t_start = std::chrono::steady_clock::now();
+  auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
+  using Nbr = std::pair<u32, double>;
   Factor f;
   i64 n = lap.nrows;
   f.n = n;
-  std::vector<std::map<u32, double>> adj(n);
-  for (i64 i = 0; i < n; ++i)
+  // adjacency lists sorted by neighbour (a repeated column keeps its last value)
+  std::vector<std::vector<Nbr>> adj(n);
+#pragma omp parallel for schedule(dynamic, 256) if (n >= 4096)
+  for (i64 i = 0; i < n; ++i) {
+    std::vector<Nbr>& row = adj[i];
     for (u32 p = lap.indptr[i]; p < lap.indptr[i + 1]; ++p) {
       double w = -lap.data[p];
-      if ((i64)lap.indices[p] != i && w > 0.0) adj[i][lap.indices[p]] = w;
+      if ((i64)lap.indices[p] != i && w > 0.0) row.push_back({lap.indices[p], w});
     }
+    std::stable_sort(row.begin(), row.end(), [](const Nbr& x, const Nbr& y) { return x.first < y.first; });
+    size_t o = 0;
+    for (size_t k = 0; k < row.size(); ++k) {
+      if (o > 0 && row[o - 1].first == row[k].first) row[o - 1] = row[k];
+      else row[o++] = row[k];
+    }
+    row.resize(o);
+  }
   f.pivot.assign(n, 0.0);
-  std::vector<std::vector<std::pair<u32, double>>> cols(n);
+  std::vector<std::vector<Nbr>> cols(n);
   std::vector<char> live(n, 1);
-  std::vector<uint64_t> key(n), taken(n, 0);
+  std::vector<uint64_t> key(n), taken(n, 0), dirty(n, 0);
   i64 left = n, cut = dense_core > 1 ? dense_core : 0;
   uint64_t round = 0;
   f.round_ptr.push_back(0);
-  std::vector<u32> pick;
+  std::vector<u32> pick, order, touched;
+  std::vector<std::vector<Edge>> fills;
+  std::vector<i64> acnt(n, 0), abeg(n, 0);  // per touched vertex: its fill additions
+  std::vector<Nbr> adds;
   while (left > 0) {
     if (left <= cut) {
       std::vector<u32> core;
@@ -573,13 +590,17 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       std::vector<double> a(c * c, 0.0);
       for (i64 i = 0; i < c; ++i) {
         double sum = 0.0;
-        for (const auto& kv : adj[core[i]]) {
+        for (const Nbr& kv : adj[core[i]]) {
           sum += kv.second;
           a[i * c + at[kv.first]] = -kv.second;
         }
         a[i * c + i] = sum;
       }
+      const double t_rounds = secs();
       std::vector<double> inv = anchored_inverse(a, c);
+      if (report)
+        std::fprintf(stderr, "[within_amd]     approximate Cholesky: %llu rounds %.3f s, dense core %lld: %.3f s\n",
+                     (unsigned long long)round, t_rounds, (long long)c, secs() - t_rounds);
       if (!inv.empty()) {
         f.core = std::move(core);
         f.core_inv = std::move(inv);
@@ -589,7 +610,7 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
     }
     // greedy maximal independent set in (degree, key, id) order
     pick.clear();
-    std::vector<u32> order;
+    order.clear();
     for (i64 v = 0; v < n; ++v)
       if (live[v]) { key[v] = round_key(seed, round, (uint64_t)v); order.push_back((u32)v); }
     std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
@@ -600,28 +621,80 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       if (taken[v] == round + 1) continue;
       pick.push_back(v);
       taken[v] = round + 1;
-      for (const auto& kv : adj[v]) taken[kv.first] = round + 1;
+      for (const Nbr& kv : adj[v]) taken[kv.first] = round + 1;
     }
     std::sort(pick.begin(), pick.end());
-    std::vector<Edge> fill;
-    std::vector<std::pair<u32, double>> nb;
-    for (u32 v : pick) {
-      nb.assign(adj[v].begin(), adj[v].end());
+    // eliminate the set: the picked vertices share no edge, so each sees the
+    // round's starting adjacency; pivots and clique trees are independent
+    const i64 np = (i64)pick.size();
+    if ((i64)fills.size() < np) fills.resize(np);
+#pragma omp parallel for schedule(dynamic, 64) if (np >= 1024)
+    for (i64 t = 0; t < np; ++t) {
+      const u32 v = pick[t];
+      const std::vector<Nbr>& nb = adj[v];
       double dv = 0.0;
-      for (const auto& p : nb) dv += p.second;
+      for (const Nbr& q : nb) dv += q.second;
       f.pivot[v] = dv;
-      if (nb.size() >= 2) sample_clique_tree(nb, dv, seed + (uint64_t)v, split, fill);
-      for (const auto& p : nb) adj[p.first].erase(v);
-      adj[v].clear();
-      cols[v] = nb;
+      fills[t].clear();
+      if (nb.size() >= 2) sample_clique_tree(nb, dv, seed + (uint64_t)v, split, fills[t]);
+    }
+    // neighbours of the set lose their edge to it and gain the fill, added
+    // per entry in (v ascending, emission) order
+    touched.clear();
+    for (i64 t = 0; t < np; ++t) {
+      const u32 v = pick[t];
+      for (const Nbr& q : adj[v])
+        if (dirty[q.first] != round + 1) { dirty[q.first] = round + 1; touched.push_back(q.first); }
+    }
+    for (u32 u : touched) acnt[u] = 0;
+    for (i64 t = 0; t < np; ++t)
+      for (const Edge& e : fills[t]) { acnt[e.lo]++; acnt[e.hi]++; }
+    i64 run = 0;
+    for (u32 u : touched) {  // acnt becomes the write cursor of u's range
+      abeg[u] = run;
+      run += acnt[u];
+      acnt[u] = abeg[u];
+    }
+    adds.resize(run);
+    for (i64 t = 0; t < np; ++t)
+      for (const Edge& e : fills[t]) {
+        adds[acnt[e.lo]++] = {e.hi, e.w};
+        adds[acnt[e.hi]++] = {e.lo, e.w};
+      }
+    for (i64 t = 0; t < np; ++t) {
+      const u32 v = pick[t];
+      cols[v] = std::move(adj[v]);
+      adj[v] = std::vector<Nbr>();
       live[v] = 0;
       f.order.push_back(v);
     }
-    for (const Edge& e : fill) {
-      adj[e.lo][e.hi] += e.w;
-      adj[e.hi][e.lo] += e.w;
+    const i64 nt = (i64)touched.size();
+#pragma omp parallel for schedule(dynamic, 64) if (nt >= 1024)
+    for (i64 k = 0; k < nt; ++k) {
+      const u32 u = touched[k];
+      Nbr* a0 = adds.data() + abeg[u];
+      Nbr* a1 = adds.data() + acnt[u];
+      std::stable_sort(a0, a1, [](const Nbr& x, const Nbr& y) { return x.first < y.first; });
+      const std::vector<Nbr>& old = adj[u];
+      std::vector<Nbr> merged;
+      merged.reserve(old.size() + (size_t)(a1 - a0));
+      size_t i = 0;
+      const Nbr* p = a0;
+      while (i < old.size() || p < a1) {
+        if (i < old.size() && !live[old[i].first]) { ++i; continue; }
+        if (p == a1 || (i < old.size() && old[i].first < p->first)) {
+          merged.push_back(old[i++]);
+          continue;
+        }
+        const u32 j = p->first;
+        double acc = 0.0;
+        if (i < old.size() && old[i].first == j) acc = old[i++].second;
+        for (; p < a1 && p->first == j; ++p) acc += p->second;
+        merged.push_back({j, acc});
+      }
+      adj[u] = std::move(merged);
     }
-    left -= (i64)pick.size();
+    left -= np;
     f.round_ptr.push_back((u32)f.order.size());
     ++round;
   }
@@ -774,8 +847,13 @@ void build_local_solver(Subdomain& s, const LocalConfig& cfg) {
       return;
     }
   }
+  const bool big = e.n_keep >= 50000;
+  auto t0 = std::chrono::steady_clock::now();
   Csr S = cfg.approx_schur ? schur_sampled(e, cfg.schur_seed, cfg.schur_split) : schur_exact(e, s.inv_diag_elim);
-  s.f = factor_laplacian(S, cfg.ac_seed, cfg.ac_split, cfg.dense_core);
+  if (big && verbose())
+    std::fprintf(stderr, "[within_amd]     schur complement (n=%lld nnz=%zu): %.3f s\n", (long long)e.n_keep,
+                 S.indices.size(), std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  s.f = factor_laplacian(S, cfg.ac_seed, cfg.ac_split, cfg.dense_core, big && verbose());
 }
 
 // ---------------------------------------------------------------------------
