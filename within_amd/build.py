@@ -1,8 +1,9 @@
 """Build the in-tree native library ``within_amd/_lib/libwithin_amd.so``.
 
 * ``csrc/plan.cpp`` (host setup, OpenMP) with g++;
-* ``csrc/kernels.hip`` and ``csrc/runtime.cpp`` (device code + C ABI) with
-  hipcc for gfx950, packed FP32 instructions disabled;
+* ``csrc/kernels.hip``, ``csrc/dense.hip`` and ``csrc/runtime.cpp`` (device
+  code + C ABI) with hipcc for gfx950, packed FP32 instructions disabled
+  (``dense.hip`` also without FP contraction);
 * linked into one shared library exporting the symbols of
   ``include/within_amd.h``.
 """
@@ -50,10 +51,12 @@ def build(verbose: bool = False) -> str:
     _run(["g++", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
           os.path.join(CSRC, "plan.cpp"), "-o", plan_o], verbose)
     objs.append(plan_o)
-    for src in ("kernels.hip", "runtime.cpp"):
+    for src in ("kernels.hip", "runtime.cpp", "dense.hip"):
         o = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        _run([hipcc, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        # the dense-core inverse must round like the host factorisation
+        exact = ["-ffp-contract=off"] if src == "dense.hip" else []
+        _run([hipcc, *HIP_FLAGS, *exact, *lang, "-c", os.path.join(CSRC, src), "-o", o], verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"], verbose)

@@ -1,0 +1,284 @@
+// dense.hip -- the anchored dense-core inverse of the approximate Cholesky
+// factor on the GPU (operator/local_solver.rs:207-300 semantics), used at
+// setup for large cores.
+//
+// The arithmetic of every entry is the one fixed by anchored_inverse in
+// plan.cpp (and by the oracle's plain loops):
+//   l_ij = (a_ij - sum_{k<j} l_ik l_jk) / l_jj,  l_jj = sqrt(a_jj - sum_{k<j} l_jk^2)
+//   U_ij = (delta_ij - sum_{k=j}^{i-1} l_ik U_kj) / l_ii         (U = L^-1)
+//   inv_ij = sum_{k>=i} U_ki U_kj                                  (j <= i)
+// each sum term by term, k ascending, one rounded multiply and one rounded
+// add or subtract per term (this file is built without FP contraction).  The
+// kernels only choose which thread evaluates which entry and where operands
+// are staged, so the result is the host factorisation bit for bit.  Terms
+// whose U factor is an exact zero (k below the diagonal of U) are added where
+// that keeps the loops uniform; they leave a +0 or nonzero partial sum
+// unchanged.
+//
+// Stage 1 is left-looking by 32-column tiles: the tile's columns receive the
+// contributions of every earlier column (one thread per entry), then the
+// diagonal tile is finished column by column and the rows below it are
+// substituted against it (one thread per row).  Stage 2 runs row tiles in
+// order, each column tile of the row tile in its own workgroup (one thread
+// per entry, then the tile's own rows in order).  Stage 3 evaluates the whole
+// lower triangle at once.  Partial sums written to memory and read back are
+// exact, so splitting a sum across kernels keeps its rounding.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace wa {
+namespace {
+
+constexpr int TB = 32;  // column tile
+constexpr int RB = 64;  // rows per workgroup (one thread each)
+constexpr int KC = 32;  // k values staged in LDS per step
+
+struct HipFail : Error {
+  explicit HipFail(const std::string& m) : Error(10, m) {}
+};
+#define DCHK(x)                                                                 \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) throw HipFail(std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Left-looking update of the column tile [c0, c0 + w) for rows >= c0:
+// L_ij -= sum_{k<c0} L_ik L_jk, k ascending, one thread per entry (the partial
+// sums are stored and the tile kernels continue them from k = c0).
+__global__ void __launch_bounds__(TB * TB) k_chol_update(double* __restrict__ L, int m, int c0, int w) {
+  __shared__ double As[TB][KC + 1];
+  __shared__ double Bs[TB][KC + 1];
+  const int tx = threadIdx.x % TB, ty = threadIdx.x / TB;
+  const int64_t ib = c0 + (int64_t)blockIdx.x * TB, i = ib + ty;
+  const int64_t j = c0 + tx;
+  const bool live = i < m && tx < w && j <= i;
+  double acc = live ? L[i * m + j] : 0.0;
+  for (int kc = 0; kc < c0; kc += KC) {
+    const int kw = min(KC, c0 - kc);
+    {
+      const int r = threadIdx.x / KC, kk = threadIdx.x % KC;
+      As[r][kk] = (ib + r < m && kk < kw) ? L[(ib + r) * m + kc + kk] : 0.0;
+      Bs[r][kk] = (r < w && kk < kw) ? L[(int64_t)(c0 + r) * m + kc + kk] : 0.0;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kw; ++kk) acc = acc - As[ty][kk] * Bs[tx][kk];
+    __syncthreads();
+  }
+  if (live) L[i * m + j] = acc;
+}
+
+// Diagonal tile [c0, c0 + w) after its update: rows c0 + t, t < w, finished
+// column by column (k from c0 on).
+__global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ L, int m, int c0, int w,
+                                                  int* __restrict__ fail) {
+  __shared__ double R[TB][TB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < TB * TB; e += 64) {
+    const int r = e / TB, k = e % TB;
+    R[r][k] = (r < w && k <= r) ? L[(int64_t)(c0 + r) * m + c0 + k] : 0.0;
+  }
+  __syncthreads();
+  for (int s = 0; s < w; ++s) {
+    if (t == s) {
+      double d = R[s][s];
+      for (int k = 0; k < s; ++k) d = d - R[s][k] * R[s][k];
+      if (!(d > 0.0)) *fail = 1;
+      R[s][s] = __dsqrt_rn(d);
+    }
+    __syncthreads();
+    if (t > s && t < w) {
+      double x = R[t][s];
+      for (int k = 0; k < s; ++k) x = x - R[t][k] * R[s][k];
+      R[t][s] = x / R[s][s];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < TB * TB; e += 64) {
+    const int r = e / TB, k = e % TB;
+    if (r < w && k <= r) L[(int64_t)(c0 + r) * m + c0 + k] = R[r][k];
+  }
+}
+
+// Rows r0 + blockIdx.x * RB + t (< m) of the column tile [c0, c0 + w) after
+// its update: forward substitution against the finished diagonal tile.
+__global__ void __launch_bounds__(RB) k_chol_below(double* __restrict__ L, int m, int c0, int w, int r0) {
+  __shared__ double As[RB][TB + 1];
+  __shared__ double D[TB][TB + 1];
+  const int t = threadIdx.x;
+  const int64_t ib = r0 + (int64_t)blockIdx.x * RB, i = ib + t;
+  for (int e = t; e < RB * TB; e += RB) {
+    const int r = e / TB, k = e % TB;
+    As[r][k] = (ib + r < m && k < w) ? L[(ib + r) * m + c0 + k] : 0.0;
+  }
+  for (int e = t; e < TB * TB; e += RB) {
+    const int r = e / TB, k = e % TB;
+    D[r][k] = (r < w && k <= r) ? L[(int64_t)(c0 + r) * m + c0 + k] : 1.0;
+  }
+  __syncthreads();
+  double* r_ = As[t];
+  for (int jj = 0; jj < w; ++jj) {
+    double x = r_[jj];
+    for (int k = 0; k < jj; ++k) x = x - r_[k] * D[jj][k];
+    r_[jj] = x / D[jj][jj];
+  }
+  if (i < m)
+    for (int jj = 0; jj < w; ++jj) L[i * m + c0 + jj] = r_[jj];
+}
+
+// U rows [r0, r0 + TB) of column tile j0 = blockIdx.x * TB (<= r0): one
+// thread per entry for k in [j0, r0), then the tile's own rows in order.
+__global__ void __launch_bounds__(TB * TB) k_inv_lower(const double* __restrict__ L, double* __restrict__ U, int m,
+                                                       int r0) {
+  __shared__ double Ls[TB][KC + 1];
+  __shared__ double Us[KC][TB + 1];
+  __shared__ double S[TB][TB + 1];
+  __shared__ double Lt[TB][TB + 1];
+  const int tx = threadIdx.x % TB, ty = threadIdx.x / TB;
+  const int j0 = blockIdx.x * TB;
+  const int64_t i = r0 + ty;
+  const int j = j0 + tx;
+  double acc = (i == j) ? 1.0 : 0.0;
+  for (int kc = j0; kc < r0; kc += KC) {
+    const int kw = min(KC, r0 - kc);
+    {
+      const int r = threadIdx.x / KC, kk = threadIdx.x % KC;
+      Ls[r][kk] = (r0 + r < m && kk < kw) ? L[(int64_t)(r0 + r) * m + kc + kk] : 0.0;
+      const int k2 = threadIdx.x / TB, jj = threadIdx.x % TB;
+      Us[k2][jj] = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kw; ++kk) acc = acc - Ls[ty][kk] * Us[kk][tx];
+    __syncthreads();
+  }
+  S[ty][tx] = acc;
+  Lt[ty][tx] = (r0 + ty < m && tx <= ty) ? L[(int64_t)(r0 + ty) * m + r0 + tx] : 1.0;
+  __syncthreads();
+  // k in [r0, i): one thread per column, the column kept in S
+  if (ty == 0) {
+    for (int s = 0; s < TB; ++s) {
+      double x = S[s][tx];
+      for (int k = 0; k < s; ++k) x = x - Lt[s][k] * S[k][tx];
+      S[s][tx] = (r0 + s >= j) ? x / Lt[s][s] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (i < m && j < m) U[i * m + j] = S[ty][tx];
+}
+
+// inv rows [i0, i0 + RB) x columns [j0, j0 + TB), lower triangle, mirrored.
+__global__ void __launch_bounds__(RB) k_inv_sym(const double* __restrict__ U, double* __restrict__ inv, int m,
+                                                int n) {
+  __shared__ double Ua[KC][RB + 1];
+  __shared__ double Ub[KC][TB + 1];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * RB, j0 = blockIdx.y * TB;
+  if (j0 >= min(i0 + RB, m)) return;
+  const int64_t i = i0 + t;
+  double acc[TB];
+#pragma unroll
+  for (int jj = 0; jj < TB; ++jj) acc[jj] = 0.0;
+  for (int kc = i0; kc < m; kc += KC) {
+    const int kw = min(KC, m - kc);
+    for (int e = t; e < KC * RB; e += RB) {
+      const int kk = e / RB, r = e % RB;
+      Ua[kk][r] = (kk < kw && i0 + r < m) ? U[(int64_t)(kc + kk) * m + i0 + r] : 0.0;
+    }
+    for (int e = t; e < KC * TB; e += RB) {
+      const int kk = e / TB, jj = e % TB;
+      Ub[kk][jj] = (kk < kw && j0 + jj < m) ? U[(int64_t)(kc + kk) * m + j0 + jj] : 0.0;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kw; ++kk) {
+      const double uki = Ua[kk][t];
+#pragma unroll
+      for (int jj = 0; jj < TB; ++jj) acc[jj] = acc[jj] + uki * Ub[kk][jj];
+    }
+    __syncthreads();
+  }
+  if (i < m)
+#pragma unroll
+    for (int jj = 0; jj < TB; ++jj) {
+      const int64_t j = j0 + jj;
+      if (j <= i) {
+        inv[i * n + j] = acc[jj];
+        inv[j * n + i] = acc[jj];
+      }
+    }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) { DCHK(hipMalloc(&p, bytes)); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+}  // namespace
+
+bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<double>& inv) {
+  const bool report = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (!report) return;
+    (void)hipDeviceSynchronize();
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[within_amd]       dense core %s: %.3f s\n", what,
+                 std::chrono::duration<double>(t1 - t0).count());
+    t0 = t1;
+  };
+  inv.assign((size_t)n * n, 0.0);
+  const i64 m = n - 1;
+  if (m <= 0) return true;
+  int dev = 0;
+  DCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  DCHK(hipGetDeviceProperties(&prop, dev));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    throw HipFail(std::string("within_amd requires a gfx950 (MI355X) device, found ") + prop.gcnArchName);
+  std::vector<double> l((size_t)m * m, 0.0);
+  for (i64 i = 0; i < m; ++i)
+    for (i64 j = 0; j <= i; ++j) l[i * m + j] = a[i * n + j];
+  const size_t mm = (size_t)m * m * sizeof(double), nn = (size_t)n * n * sizeof(double);
+  DevBuf dL(mm), dU(mm), dI(nn), dF(sizeof(int));
+  DCHK(hipMemcpy(dL.p, l.data(), mm, hipMemcpyHostToDevice));
+  DCHK(hipMemset(dU.p, 0, mm));
+  DCHK(hipMemset(dI.p, 0, nn));
+  DCHK(hipMemset(dF.p, 0, sizeof(int)));
+  lap("upload");
+  const int mi = (int)m, ni = (int)n;
+  for (int c0 = 0; c0 < mi; c0 += TB) {
+    const int w = std::min(TB, mi - c0), c1 = c0 + w;
+    if (c0 > 0) k_chol_update<<<(unsigned)((mi - c0 + TB - 1) / TB), TB * TB>>>(dL.as<double>(), mi, c0, w);
+    k_chol_diag<<<1, 64>>>(dL.as<double>(), mi, c0, w, dF.as<int>());
+    if (c1 < mi) k_chol_below<<<(unsigned)((mi - c1 + RB - 1) / RB), RB>>>(dL.as<double>(), mi, c0, w, c1);
+  }
+  DCHK(hipGetLastError());
+  int failed = 0;
+  DCHK(hipMemcpy(&failed, dF.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (failed) return false;
+  lap("cholesky");
+  for (int r0 = 0; r0 < mi; r0 += TB)
+    k_inv_lower<<<(unsigned)(r0 / TB + 1), TB * TB>>>(dL.as<double>(), dU.as<double>(), mi, r0);
+  DCHK(hipGetLastError());
+  lap("triangular inverse");
+  k_inv_sym<<<dim3((unsigned)((mi + RB - 1) / RB), (unsigned)((mi + TB - 1) / TB)), RB>>>(dU.as<double>(),
+                                                                                        dI.as<double>(), mi, ni);
+  DCHK(hipGetLastError());
+  lap("product");
+  DCHK(hipMemcpy(inv.data(), dI.p, nn, hipMemcpyDeviceToHost));
+  lap("download");
+  return true;
+}
+
+}  // namespace wa

@@ -544,7 +544,8 @@ static inline uint64_t round_key(uint64_t seed, uint64_t round, uint64_t v) {
 // + v), merged after the round in (v ascending, emission) order.  Once at most
 // `dense_core` vertices remain they are factored exactly (anchored at the
 // last one); a core whose anchored minor is singular falls back to rounds.
-static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 dense_core, bool report = false) {
+static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 dense_core, bool device,
+                               bool report = false) {
   const auto t_start = std::chrono::steady_clock::now();
   auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
   using Nbr = std::pair<u32, double>;
@@ -597,7 +598,12 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
         a[i * c + i] = sum;
       }
       const double t_rounds = secs();
-      std::vector<double> inv = anchored_inverse(a, c);
+      std::vector<double> inv;
+      if (device && c >= kDeviceCore) {
+        if (!device_anchored_inverse(a, c, inv)) inv.clear();
+      } else {
+        inv = anchored_inverse(a, c);
+      }
       if (report)
         std::fprintf(stderr, "[within_amd]     approximate Cholesky: %llu rounds %.3f s, dense core %lld: %.3f s\n",
                      (unsigned long long)round, t_rounds, (long long)c, secs() - t_rounds);
@@ -853,7 +859,7 @@ void build_local_solver(Subdomain& s, const LocalConfig& cfg) {
   if (big && verbose())
     std::fprintf(stderr, "[within_amd]     schur complement (n=%lld nnz=%zu): %.3f s\n", (long long)e.n_keep,
                  S.indices.size(), std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-  s.f = factor_laplacian(S, cfg.ac_seed, cfg.ac_split, cfg.dense_core, big && verbose());
+  s.f = factor_laplacian(S, cfg.ac_seed, cfg.ac_split, cfg.dense_core, cfg.device, big && verbose());
 }
 
 // ---------------------------------------------------------------------------

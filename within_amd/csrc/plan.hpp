@@ -35,7 +35,9 @@ struct LocalConfig {
   int schur_split = 1;
   int dense_threshold = 24;
   i64 dense_core = 8192;
+  bool device = false;  // factor dense cores of at least kDeviceCore on the GPU
 };
+constexpr i64 kDeviceCore = 2048;
 
 struct Design {
   i64 n_obs = 0;
@@ -223,5 +225,9 @@ std::vector<Subdomain> build_subdomains(const Design& d);
 void build_local_solver(Subdomain& s, const LocalConfig& cfg);
 Csr build_gramian(const Design& d);
 std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n);
+// The same inverse evaluated on the GPU (dense.hip), bit for bit: false when
+// the anchored minor is not positive definite; throws wa::Error on a HIP
+// failure.
+bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<double>& inv);
 
 }  // namespace wa

@@ -1277,6 +1277,7 @@ static wa_status create_impl(const wa_design* d, const wa_params* p, const uint8
     pl.cfg.schur_split = p->local.schur_split;
     pl.cfg.dense_threshold = p->local.dense_threshold;
     pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
+    pl.cfg.device = !s->host_only;
     pl.precond = p->preconditioner;
     pl.explicit_op = p->operator_repr == 1;
     if (pre) {
