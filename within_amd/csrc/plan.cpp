@@ -21,13 +21,16 @@ static bool verbose() {
 }
 struct PhaseTimer {
   const char* name;
+  bool open = true;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   explicit PhaseTimer(const char* n) : name(n) {}
-  ~PhaseTimer() {
-    if (verbose())
+  void done() {
+    if (open && verbose())
       std::fprintf(stderr, "[within_amd] %s: %.3f s\n", name,
                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    open = false;
   }
+  ~PhaseTimer() { done(); }
 };
 
 enum { E_EMPTY = 1, E_OVERFLOW = 4, E_SINGULAR = 5, E_LOCAL = 6, E_INVALID = 9 };
@@ -779,24 +782,53 @@ static Csr schur_exact(const ElimView& e, const std::vector<double>& inv) {
 }
 
 static Csr schur_sampled(const ElimView& e, uint64_t seed, int split) {
-  std::vector<Edge> edges;
-  std::vector<std::pair<u32, double>> nb;
-  for (i64 k = 0; k < e.n_elim; ++k) {
-    u32 b = e.e2k->indptr[k], en = e.e2k->indptr[k + 1];
-    if (en - b <= 1) continue;
-    nb.clear();
-    for (u32 p = b; p < en; ++p) nb.push_back({e.e2k->indices[p], e.e2k->data[p]});
-    sample_clique_tree(nb, (*e.delim)[k], seed + (uint64_t)k, split, edges);
-  }
-  std::stable_sort(edges.begin(), edges.end(), [](const Edge& a, const Edge& b) {
-    return a.lo != b.lo ? a.lo < b.lo : a.hi < b.hi;
-  });
-  std::vector<Edge> m;
-  for (const Edge& x : edges) {
-    if (!m.empty() && m.back().lo == x.lo && m.back().hi == x.hi) m.back().w += x.w;
-    else m.push_back(x);
-  }
   i64 n = e.n_keep;
+  // clique trees of every eliminated star, in star order (blocks of stars
+  // sampled in parallel and concatenated in block order)
+  const i64 nblk = std::max<i64>(1, std::min<i64>(256, e.n_elim / 4096));
+  std::vector<std::vector<Edge>> part(nblk);
+#pragma omp parallel for schedule(dynamic, 1) if (nblk > 1)
+  for (i64 bk = 0; bk < nblk; ++bk) {
+    std::vector<std::pair<u32, double>> nb;
+    const i64 k0 = e.n_elim * bk / nblk, k1 = e.n_elim * (bk + 1) / nblk;
+    for (i64 k = k0; k < k1; ++k) {
+      u32 b = e.e2k->indptr[k], en = e.e2k->indptr[k + 1];
+      if (en - b <= 1) continue;
+      nb.clear();
+      for (u32 p = b; p < en; ++p) nb.push_back({e.e2k->indices[p], e.e2k->data[p]});
+      sample_clique_tree(nb, (*e.delim)[k], seed + (uint64_t)k, split, part[bk]);
+    }
+  }
+  // a stable sort by (lo, hi): a stable counting sort by lo, then each lo
+  // bucket stably by hi; duplicates then summed in emission order
+  std::vector<i64> lo_ptr(n + 1, 0);
+  for (const auto& pe : part)
+    for (const Edge& x : pe) lo_ptr[x.lo + 1]++;
+  for (i64 i = 0; i < n; ++i) lo_ptr[i + 1] += lo_ptr[i];
+  std::vector<Edge> edges(lo_ptr[n]);
+  {
+    std::vector<i64> cur(lo_ptr.begin(), lo_ptr.end() - 1);
+    for (const auto& pe : part)
+      for (const Edge& x : pe) edges[cur[x.lo]++] = x;
+  }
+  part.clear();
+  std::vector<i64> m_cnt(n, 0);
+#pragma omp parallel for schedule(dynamic, 64) if (n >= 256)
+  for (i64 i = 0; i < n; ++i) {
+    Edge* a = edges.data() + lo_ptr[i];
+    Edge* b = edges.data() + lo_ptr[i + 1];
+    std::stable_sort(a, b, [](const Edge& x, const Edge& y) { return x.hi < y.hi; });
+    i64 o = 0;
+    for (Edge* x = a; x < b; ++x) {
+      if (o > 0 && a[o - 1].hi == x->hi) a[o - 1].w += x->w;
+      else a[o++] = *x;
+    }
+    m_cnt[i] = o;
+  }
+  std::vector<Edge> m;
+  m.reserve(edges.size());
+  for (i64 i = 0; i < n; ++i) m.insert(m.end(), edges.begin() + lo_ptr[i], edges.begin() + lo_ptr[i] + m_cnt[i]);
+  edges.clear();
   std::vector<u32> lo_cnt(n, 0), up_cnt(n, 0);
   std::vector<double> dg(n, 0.0);
   for (const Edge& x : m) {
@@ -948,51 +980,76 @@ static void flatten(Plan& p) {
   f.loc_sub.resize(f.n_loc);
   f.red_sub.resize(f.n_red);
   f.red_loc.resize(f.n_red);
-  f.elim_ptr.assign(1, 0);
-  f.keep_ptr.assign(1, 0);
-  f.keep_ptr.reserve(f.n_red + 1);
   f.inv_pivot.assign(f.n_red, 0.0);
   i64 max_rounds = 0;
   for (const auto& s : subs) max_rounds = std::max<i64>(max_rounds, (i64)s.f.round_ptr.size() - 1);
-  // keep rows and elimination rows, subdomain by subdomain
+  // keep rows and elimination rows, subdomain by subdomain: row and nonzero
+  // offsets first, then every subdomain's rows filled in parallel
+  std::vector<i64> er0(f.n_sub + 1, 0), ez0(f.n_sub + 1, 0), kz0(f.n_sub + 1, 0);
   for (i64 si = 0; si < f.n_sub; ++si) {
     const Subdomain& s = subs[si];
-    u32 L0 = f.loc_ptr[si], R0 = f.red_ptr[si];
-    i64 nq = s.ct.n_q(), nl = s.n_local();
+    const Csr& e2k = s.eliminate_q ? s.ct.c : s.ct.ct;
+    const Csr& k2e = s.eliminate_q ? s.ct.ct : s.ct.c;
+    er0[si + 1] = er0[si] + e2k.nrows;
+    ez0[si + 1] = ez0[si] + (i64)e2k.indices.size();
+    kz0[si + 1] = kz0[si] + (i64)k2e.indices.size();
+    f.c_nnz += (i64)s.ct.c.indices.size();
+    if (s.dense) f.dense_subs++;
+  }
+  f.elim_loc.resize(er0[f.n_sub]);
+  f.elim_inv.resize(er0[f.n_sub]);
+  f.elim_ptr.resize(er0[f.n_sub] + 1);
+  f.elim_ptr[0] = 0;
+  f.elim_idx.resize(ez0[f.n_sub]);
+  f.elim_val.resize(ez0[f.n_sub]);
+  f.keep_ptr.resize(f.n_red + 1);
+  f.keep_ptr[0] = 0;
+  f.keep_idx.resize(kz0[f.n_sub]);
+  f.keep_val.resize(kz0[f.n_sub]);
+  f.keep_rowsum.resize(f.n_red);
+#pragma omp parallel
+  for (i64 si = 0; si < f.n_sub; ++si) {
+    const Subdomain& s = subs[si];
+    const u32 L0 = f.loc_ptr[si], R0 = f.red_ptr[si];
+    const i64 nq = s.ct.n_q(), nl = s.n_local();
+    const Csr& e2k = s.eliminate_q ? s.ct.c : s.ct.ct;   // elim rows -> keep cols
+    const Csr& k2e = s.eliminate_q ? s.ct.ct : s.ct.c;   // keep rows -> elim cols
+    const u32 elim_base = s.eliminate_q ? L0 : L0 + (u32)nq;
+    const u32 keep_base = s.eliminate_q ? L0 + (u32)nq : L0;
+#pragma omp for schedule(static)
     for (i64 k = 0; k < nl; ++k) {
       f.loc_gidx[L0 + k] = s.gidx[k];
       double w = s.pou.empty() ? 1.0 : s.pou[k];
       f.loc_scale[L0 + k] = k >= nq ? -w : w;
       f.loc_sub[L0 + k] = (u32)si;
     }
-    const Csr& e2k = s.eliminate_q ? s.ct.c : s.ct.ct;   // elim rows -> keep cols
-    const Csr& k2e = s.eliminate_q ? s.ct.ct : s.ct.c;   // keep rows -> elim cols
-    u32 elim_base = s.eliminate_q ? L0 : L0 + (u32)nq;
-    u32 keep_base = s.eliminate_q ? L0 + (u32)nq : L0;
+#pragma omp for schedule(dynamic, 1024)
     for (i64 e = 0; e < e2k.nrows; ++e) {
-      f.elim_loc.push_back(elim_base + (u32)e);
-      f.elim_inv.push_back(s.inv_diag_elim[e]);
-      for (u32 q = e2k.indptr[e]; q < e2k.indptr[e + 1]; ++q) {
-        f.elim_idx.push_back(R0 + e2k.indices[q]);
-        f.elim_val.push_back(e2k.data[q]);
+      const i64 row = er0[si] + e;
+      f.elim_loc[row] = elim_base + (u32)e;
+      f.elim_inv[row] = s.inv_diag_elim[e];
+      i64 o = ez0[si] + e2k.indptr[e];
+      for (u32 q = e2k.indptr[e]; q < e2k.indptr[e + 1]; ++q, ++o) {
+        f.elim_idx[o] = R0 + e2k.indices[q];
+        f.elim_val[o] = e2k.data[q];
       }
-      f.elim_ptr.push_back((u32)f.elim_idx.size());
+      f.elim_ptr[row + 1] = (u32)(ez0[si] + e2k.indptr[e + 1]);
     }
+#pragma omp for schedule(dynamic, 1024)
     for (i64 j = 0; j < k2e.nrows; ++j) {
       f.red_sub[R0 + j] = (u32)si;
       f.red_loc[R0 + j] = keep_base + (u32)j;
       double rs = 0.0;
-      for (u32 q = k2e.indptr[j]; q < k2e.indptr[j + 1]; ++q) {
+      i64 o = kz0[si] + k2e.indptr[j];
+      for (u32 q = k2e.indptr[j]; q < k2e.indptr[j + 1]; ++q, ++o) {
         double v = k2e.data[q] * s.inv_diag_elim[k2e.indices[q]];
-        f.keep_idx.push_back(elim_base + k2e.indices[q]);
-        f.keep_val.push_back(v);
+        f.keep_idx[o] = elim_base + k2e.indices[q];
+        f.keep_val[o] = v;
         rs += v;
       }
-      f.keep_ptr.push_back((u32)f.keep_idx.size());
-      f.keep_rowsum.push_back(rs);
+      f.keep_ptr[R0 + j + 1] = (u32)(kz0[si] + k2e.indptr[j + 1]);
+      f.keep_rowsum[R0 + j] = rs;
     }
-    f.c_nnz += (i64)s.ct.c.indices.size();
-    if (s.dense) f.dense_subs++;
   }
   // factor levels: round t of every subdomain, then the core gather level
   std::vector<std::vector<std::pair<u32, double>>> in_edges(f.n_red);
@@ -1079,13 +1136,8 @@ DeviceLayout make_device_layout(const Plan& p) {
     L.pre = p.pre;
     return L;
   }
-  // active levels
-  std::vector<char> act(d.n_dofs, 0);
-  for (int q = 0; q < Q; ++q) {
-    const u32* c = d.codes[q].data();
-    const i64 o = d.offset[q];
-    for (i64 i = 0; i < n; ++i) act[o + c[i]] = 1;
-  }
+  PhaseTimer t_levels("layout: levels");
+  // active levels: the non-empty rows of each factor's pass
   // factor with the most levels, ordered by the factor with the next-most
   int big = 0, ref = -1;
   for (int q = 1; q < Q; ++q)
@@ -1094,14 +1146,20 @@ DeviceLayout make_device_layout(const Plan& p) {
     if (q != big && (ref < 0 || d.levels[q] > d.levels[ref])) ref = q;
   L.int_of.assign(d.n_dofs, UINT32_MAX);
   for (int q = 0; q < Q; ++q) {
+    const std::vector<u32>& rp = p.passes[q].row_ptr;
     std::vector<u32> lv;
     for (i64 l = 0; l < d.levels[q]; ++l)
-      if (act[d.offset[q] + l]) lv.push_back((u32)l);
+      if (rp[l + 1] > rp[l]) lv.push_back((u32)l);
     if (q == big && ref >= 0) {
+      // the smallest `ref` code a level meets is its first element's: a
+      // pass's rows are sorted by the code of the factor with the most
+      // levels among the others, which is `ref` for `big`
+      const FactorPass& fp = p.passes[q];
+      const int ts = ref < q ? ref : ref - 1;
       std::vector<u32> key(d.levels[q], UINT32_MAX);
-      const u32* cb = d.codes[q].data();
-      const u32* cr = d.codes[ref].data();
-      for (i64 i = 0; i < n; ++i) key[cb[i]] = std::min(key[cb[i]], cr[i]);
+#pragma omp parallel for schedule(static)
+      for (i64 l = 0; l < d.levels[q]; ++l)
+        if (rp[l + 1] > rp[l]) key[l] = fp.other[(size_t)rp[l] * (Q - 1) + ts];
       std::stable_sort(lv.begin(), lv.end(), [&](u32 a, u32 b) { return key[a] < key[b]; });
     }
     L.off.push_back((i64)L.ext_of.size());
@@ -1117,13 +1175,16 @@ DeviceLayout make_device_layout(const Plan& p) {
   for (int q = 0; q < Q; ++q) {
     const u32* c = d.codes[q].data();
     const i64 o = d.offset[q];
-    for (i64 i = 0; i < n; ++i) L.dof_of_obs[(size_t)q * n + i] = L.int_of[o + c[i]];
+    u32* out = L.dof_of_obs.data() + (size_t)q * n;
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < n; ++i) out[i] = L.int_of[o + c[i]];
   }
+  t_levels.done();
+  PhaseTimer t_passes("layout: passes");
   // factor passes: active rows in internal order; elements re-sorted by the
   // internal index of the largest other factor (gather locality)
   const int Qm = Q - 1;
   L.passes.resize(Q);
-#pragma omp parallel for schedule(dynamic, 1)
   for (int q = 0; q < Q; ++q) {
     const FactorPass& fp = p.passes[q];
     FactorPass& np = L.passes[q];
@@ -1132,49 +1193,73 @@ DeviceLayout make_device_layout(const Plan& p) {
     for (int r = 0; r < Q; ++r)
       if (r != q && (sec < 0 || d.levels[r] > d.levels[sec])) sec = r;
     const int ts = sec < 0 ? 0 : (sec < q ? sec : sec - 1);
-    np.row_ptr.assign(1, 0);
-    np.obs.reserve(fp.obs.size());
-    np.other.reserve(fp.other.size());
-    if (!fp.w.empty()) np.w.reserve(fp.w.size());
-    std::vector<u32> idx;
-    for (i64 r = 0; r < L.levels[q]; ++r) {
+    const i64 nr = L.levels[q];
+    np.row_ptr.assign(nr + 1, 0);
+    np.diag.resize(nr);
+    for (i64 r = 0; r < nr; ++r) {
       const u32 l = L.ext_of[L.off[q] + r] - (u32)d.offset[q];
-      idx.clear();
-      for (u32 e = fp.row_ptr[l]; e < fp.row_ptr[l + 1]; ++e) idx.push_back(e);
-      if (Qm > 0)
-        std::stable_sort(idx.begin(), idx.end(), [&](u32 a, u32 b) {
-          return L.int_of[fp.other[(size_t)a * Qm + ts]] < L.int_of[fp.other[(size_t)b * Qm + ts]];
-        });
-      for (u32 e : idx) {
-        np.obs.push_back(fp.obs[e]);
-        for (int t = 0; t < Qm; ++t) np.other.push_back(L.int_of[fp.other[(size_t)e * Qm + t]]);
-        if (!fp.w.empty()) np.w.push_back(fp.w[e]);
+      np.row_ptr[r + 1] = np.row_ptr[r] + (fp.row_ptr[l + 1] - fp.row_ptr[l]);
+      np.diag[r] = fp.diag[l];
+    }
+    np.obs.resize(fp.obs.size());
+    np.other.resize(fp.other.size());
+    if (!fp.w.empty()) np.w.resize(fp.w.size());
+    // rows in parallel, each a stable sort by (internal index of `sec`, element)
+    const int chunk = (int)std::max<i64>(1, std::min<i64>(256, nr / 512));
+#pragma omp parallel
+    {
+      std::vector<std::pair<u32, u32>> idx;
+#pragma omp for schedule(dynamic, chunk)
+      for (i64 r = 0; r < nr; ++r) {
+        const u32 l = L.ext_of[L.off[q] + r] - (u32)d.offset[q];
+        idx.clear();
+        for (u32 e = fp.row_ptr[l]; e < fp.row_ptr[l + 1]; ++e)
+          idx.push_back({Qm > 0 ? L.int_of[fp.other[(size_t)e * Qm + ts]] : 0u, e});
+        if (Qm > 0) std::sort(idx.begin(), idx.end());
+        size_t o = np.row_ptr[r];
+        for (const auto& ke : idx) {
+          const u32 e = ke.second;
+          np.obs[o] = fp.obs[e];
+          for (int t = 0; t < Qm; ++t) np.other[o * Qm + t] = L.int_of[fp.other[(size_t)e * Qm + t]];
+          if (!fp.w.empty()) np.w[o] = fp.w[e];
+          ++o;
+        }
       }
-      np.row_ptr.push_back((u32)np.obs.size());
-      np.diag.push_back(fp.diag[l]);
     }
   }
+  t_passes.done();
+  PhaseTimer t_gram("layout: gramian");
   // explicit Gramian: rows and columns renumbered, columns ascending
   if (p.has_gram) {
     const Csr& g = p.gram;
     Csr& ng = L.gram;
     ng.nrows = ng.ncols = L.m;
-    ng.indptr.assign(1, 0);
-    ng.indices.reserve(g.indices.size());
-    ng.data.reserve(g.data.size());
-    std::vector<std::pair<u32, double>> row;
+    ng.indptr.assign(L.m + 1, 0);
     for (i64 r = 0; r < L.m; ++r) {
       const u32 u = L.ext_of[r];
-      row.clear();
-      for (u32 k = g.indptr[u]; k < g.indptr[u + 1]; ++k) row.push_back({L.int_of[g.indices[k]], g.data[k]});
-      std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-      for (const auto& e : row) {
-        ng.indices.push_back(e.first);
-        ng.data.push_back(e.second);
+      ng.indptr[r + 1] = ng.indptr[r] + (g.indptr[u + 1] - g.indptr[u]);
+    }
+    ng.indices.resize(g.indices.size());
+    ng.data.resize(g.data.size());
+#pragma omp parallel
+    {
+      std::vector<std::pair<u32, double>> row;
+#pragma omp for schedule(dynamic, 1024)
+      for (i64 r = 0; r < L.m; ++r) {
+        const u32 u = L.ext_of[r];
+        row.clear();
+        for (u32 k = g.indptr[u]; k < g.indptr[u + 1]; ++k) row.push_back({L.int_of[g.indices[k]], g.data[k]});
+        std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        size_t o = ng.indptr[r];
+        for (const auto& e : row) {
+          ng.indices[o] = e.first;
+          ng.data[o++] = e.second;
+        }
       }
-      ng.indptr.push_back((u32)ng.indices.size());
     }
   }
+  t_gram.done();
+  PhaseTimer t_pre("layout: preconditioner");
   // preconditioner: DOF indices renumbered; each subdomain's eliminated block
   // (its elimination rows) re-laid out in internal DOF order
   if (!p.meta.empty()) {
@@ -1190,37 +1275,56 @@ DeviceLayout make_device_layout(const Plan& p) {
     std::iota(lp.begin(), lp.end(), 0u);
     std::vector<u32> order(ne);
     std::iota(order.begin(), order.end(), 0u);
-    i64 a = 0;
-    while (a < ne) {  // runs of elimination rows of one subdomain
-      const u32 sub = f.loc_sub[f.elim_loc[a]];
-      i64 b = a;
-      u32 base = f.elim_loc[a];
-      while (b < ne && f.loc_sub[f.elim_loc[b]] == sub) base = std::min(base, f.elim_loc[b++]);
-      std::stable_sort(order.begin() + a, order.begin() + b, [&](u32 x, u32 y) {
-        return L.int_of[f.loc_gidx[f.elim_loc[x]]] < L.int_of[f.loc_gidx[f.elim_loc[y]]];
-      });
-      for (i64 t = a; t < b; ++t) lp[f.elim_loc[order[t]]] = base + (u32)(t - a);
-      a = b;
+    // runs of elimination rows of one subdomain, each ordered by internal DOF
+    std::vector<i64> run(1, 0);
+    for (i64 t = 1; t < ne; ++t)
+      if (f.loc_sub[f.elim_loc[t]] != f.loc_sub[f.elim_loc[t - 1]]) run.push_back(t);
+    if (ne > 0) run.push_back(ne);
+    const i64 nruns = (i64)run.size() - 1;
+#pragma omp parallel
+    {
+      std::vector<std::pair<u32, u32>> kv;
+#pragma omp for schedule(dynamic, 1)
+      for (i64 q = 0; q < nruns; ++q) {
+        const i64 a = run[q], b = run[q + 1];
+        u32 base = f.elim_loc[a];
+        kv.clear();
+        for (i64 t = a; t < b; ++t) {
+          base = std::min(base, f.elim_loc[t]);
+          kv.push_back({L.int_of[f.loc_gidx[f.elim_loc[t]]], (u32)t});
+        }
+        std::sort(kv.begin(), kv.end());  // = a stable sort by internal DOF
+        for (i64 t = a; t < b; ++t) {
+          order[t] = kv[t - a].second;
+          lp[f.elim_loc[order[t]]] = base + (u32)(t - a);
+        }
+      }
     }
+#pragma omp parallel for schedule(static)
     for (i64 l = 0; l < f.n_loc; ++l) {
       nf.loc_gidx[lp[l]] = L.int_of[f.loc_gidx[l]];
       nf.loc_scale[lp[l]] = f.loc_scale[l];
       nf.loc_sub[lp[l]] = f.loc_sub[l];
     }
-    nf.elim_ptr.assign(1, 0);
-    nf.elim_idx.clear();
-    nf.elim_val.clear();
+    nf.elim_ptr.assign(ne + 1, 0);
+    for (i64 t = 0; t < ne; ++t) {
+      const u32 i = order[t];
+      nf.elim_ptr[t + 1] = nf.elim_ptr[t] + (f.elim_ptr[i + 1] - f.elim_ptr[i]);
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
     for (i64 t = 0; t < ne; ++t) {
       const u32 i = order[t];
       nf.elim_loc[t] = lp[f.elim_loc[i]];
       nf.elim_inv[t] = f.elim_inv[i];
-      for (u32 k = f.elim_ptr[i]; k < f.elim_ptr[i + 1]; ++k) {
-        nf.elim_idx.push_back(f.elim_idx[k]);
-        nf.elim_val.push_back(f.elim_val[k]);
+      u32 o = nf.elim_ptr[t];
+      for (u32 k = f.elim_ptr[i]; k < f.elim_ptr[i + 1]; ++k, ++o) {
+        nf.elim_idx[o] = f.elim_idx[k];
+        nf.elim_val[o] = f.elim_val[k];
       }
-      nf.elim_ptr.push_back((u32)nf.elim_idx.size());
     }
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < f.keep_idx.size(); ++k) nf.keep_idx[k] = lp[f.keep_idx[k]];
+#pragma omp parallel for schedule(static)
     for (i64 j = 0; j < f.n_red; ++j) nf.red_loc[j] = lp[f.red_loc[j]];
     nf.dof_ptr.assign(L.m + 1, 0);
     for (u32 g : nf.loc_gidx) nf.dof_ptr[g + 1]++;
@@ -1254,10 +1358,11 @@ void build_plan(Plan& p) {
     PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
-    // subdomains with a large reduced system first, one at a time (their dense
-    // cores use every thread), then the rest in parallel
+    // large subdomains first, one at a time (their Schur sampling, rounds and
+    // dense cores use every thread), then the rest in parallel
     std::vector<size_t> serial, rest;
-    for (size_t i = 0; i < p.subs.size(); ++i) (p.subs[i].n_keep() >= 1024 ? serial : rest).push_back(i);
+    for (size_t i = 0; i < p.subs.size(); ++i)
+      (p.subs[i].n_keep() >= 1024 || p.subs[i].n_local() >= 65536 ? serial : rest).push_back(i);
     auto build_one = [&](size_t i) {
       try {
         auto t0 = std::chrono::steady_clock::now();

@@ -298,8 +298,19 @@ static void check_device() {
 
 void wa_solver::setup_device() {
   const Design& d = plan.d;
+  const bool report = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+  auto t_ph = std::chrono::steady_clock::now();
+  auto phase = [&](const char* what) {
+    if (!report) return;
+    (void)hipDeviceSynchronize();
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[within_amd] device %s: %.3f s\n", what, std::chrono::duration<double>(t1 - t_ph).count());
+    t_ph = t1;
+  };
   check_device();
+  phase("init");
   const DeviceLayout lay = make_device_layout(plan);
+  phase("layout");
   n = d.n_obs;
   m = lay.m;
   mu = d.n_dofs;
@@ -340,6 +351,7 @@ void wa_solver::setup_device() {
     p64.push_back(a);
     p32.push_back(b);
   }
+  phase("operator passes");
   if (plan.has_gram) {
     const Csr& g = lay.gram;
     g64.ptr = g32.ptr = upload(g.indptr);
@@ -356,6 +368,7 @@ void wa_solver::setup_device() {
     max_slots = std::max(max_slots, g64.sched.n_slots);
     has_gram = true;
   }
+  phase("gramian");
   if (!plan.meta.empty()) {
     const PrecondFlat& f = lay.pre;
     // dense cores: the large ones are applied by the dense kernels from
@@ -565,6 +578,7 @@ void wa_solver::setup_device() {
     w32.sum3 = dalloc<float>(nsub * KMAX);
     has_pre = true;
   }
+  phase("preconditioner");
   part64 = dalloc<double>(max_slots * KMAX);
   w64.part = part64;
   w32.part = (float*)part64;
@@ -586,6 +600,7 @@ void wa_solver::setup_device() {
   cp32 = dalloc<float>(m * KMAX); cap32 = dalloc<float>(m * KMAX);
   rhs32 = dalloc<float>(m * KMAX);
   HIPCHK(hipDeviceSynchronize());
+  phase("work vectors");
 }
 
 template <typename T>
