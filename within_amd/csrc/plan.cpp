@@ -898,54 +898,71 @@ void build_local_solver(Subdomain& s, const LocalConfig& cfg) {
 // explicit Gramian -- gramian/explicit.rs:25-108, :307-409 (sorted rows, diag
 // kept, zero off-diagonals dropped)
 // ---------------------------------------------------------------------------
-Csr build_gramian(const Design& d) {
+Csr build_gramian(const Design& d, const std::vector<FactorPass>& passes) {
+  // Row a (level j of factor q) = its diagonal (sum of weights) and, per other
+  // factor r, the weight sums of the levels of r that share observations with
+  // it.  Read off the level-sorted passes: the elements of a row are ordered by
+  // the code of q's largest other factor, so that factor's sums are runs; the
+  // remaining factors' sums go through a dense per-thread accumulator.
   Csr g;
   g.nrows = g.ncols = d.n_dofs;
   g.indptr.assign(d.n_dofs + 1, 0);
-  // per factor: observations grouped by level, in observation order
-  std::vector<std::vector<u32>> ptr(d.Q), ob(d.Q);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int q = 0; q < d.Q; ++q) {
-    ptr[q].assign(d.levels[q] + 1, 0);
-    for (i64 i = 0; i < d.n_obs; ++i) ptr[q][d.codes[q][i] + 1]++;
-    for (i64 j = 0; j < d.levels[q]; ++j) ptr[q][j + 1] += ptr[q][j];
-    ob[q].resize(d.n_obs);
-    std::vector<u32> cur(ptr[q].begin(), ptr[q].end() - 1);
-    for (i64 i = 0; i < d.n_obs; ++i) ob[q][cur[d.codes[q][i]]++] = (u32)i;
-  }
-  // one row per DOF: its weight sums with every DOF it shares an observation
-  // with, accumulated in observation order, columns ascending
+  const int Q = d.Q, Qm = Q - 1;
+  i64 max_levels = 0;
+  for (int q = 0; q < Q; ++q) max_levels = std::max(max_levels, d.levels[q]);
   std::vector<std::vector<std::pair<u32, double>>> rows(d.n_dofs);
+  for (int q = 0; q < Q; ++q) {
+    const FactorPass& fp = passes[q];
+    int sec = -1;
+    for (int r = 0; r < Q; ++r)
+      if (r != q && (sec < 0 || d.levels[r] > d.levels[sec])) sec = r;
+    const int ts = sec < 0 ? -1 : (sec < q ? sec : sec - 1);
+    const i64 nl = d.levels[q];
+    const int chunk = (int)std::max<i64>(1, std::min<i64>(64, nl / 256));
 #pragma omp parallel
-  {
-    std::vector<double> acc(d.n_dofs, 0.0);
-    std::vector<char> hit(d.n_dofs, 0);
-    std::vector<u32> touched;
-    auto add = [&](u32 k, double w) {
-      if (!hit[k]) { hit[k] = 1; touched.push_back(k); }
-      acc[k] += w;
-    };
-#pragma omp for schedule(dynamic, 64)
-    for (i64 a = 0; a < d.n_dofs; ++a) {
-      int q = 0;
-      while (q + 1 < d.Q && d.offset[q + 1] <= a) ++q;
-      const i64 j = a - d.offset[q];
-      if (ptr[q][j] == ptr[q][j + 1]) continue;
-      touched.clear();
-      for (u32 t = ptr[q][j]; t < ptr[q][j + 1]; ++t) {
-        const u32 i = ob[q][t];
-        const double w = d.w(i);
-        add((u32)a, w);
-        for (int r = 0; r < d.Q; ++r)
-          if (r != q) add((u32)(d.offset[r] + d.codes[r][i]), w);
-      }
-      std::sort(touched.begin(), touched.end());
-      auto& row = rows[a];
-      row.reserve(touched.size());
-      for (u32 k : touched) {
-        if (acc[k] != 0.0 || k == (u32)a) row.push_back({k, acc[k]});
-        acc[k] = 0.0;
-        hit[k] = 0;
+    {
+      std::vector<double> acc(Qm > 1 ? max_levels : 0, 0.0);
+      std::vector<char> hit(Qm > 1 ? max_levels : 0, 0);
+      std::vector<u32> touched;
+      std::vector<std::pair<u32, double>> blk;
+#pragma omp for schedule(dynamic, chunk)
+      for (i64 j = 0; j < nl; ++j) {
+        const u32 e0 = fp.row_ptr[j], e1 = fp.row_ptr[j + 1];
+        if (e0 == e1) continue;
+        const u32 a = (u32)(d.offset[q] + j);
+        auto& row = rows[a];
+        // blocks in factor order; the diagonal sits in q's own block
+        for (int r = 0, t = 0; r < Q; ++r) {
+          if (r == q) {
+            row.push_back({a, fp.diag[j]});
+            continue;
+          }
+          const int tt = t++;
+          blk.clear();
+          if (tt == ts) {
+            for (u32 e = e0; e < e1;) {
+              const u32 c = fp.other[(size_t)e * Qm + tt];
+              double sum = 0.0;
+              for (; e < e1 && fp.other[(size_t)e * Qm + tt] == c; ++e) sum += fp.w.empty() ? 1.0 : fp.w[e];
+              if (sum != 0.0) blk.push_back({c, sum});
+            }
+          } else {
+            touched.clear();
+            for (u32 e = e0; e < e1; ++e) {
+              const u32 c = fp.other[(size_t)e * Qm + tt];
+              const u32 l = c - (u32)d.offset[r];
+              if (!hit[l]) { hit[l] = 1; touched.push_back(l); }
+              acc[l] += fp.w.empty() ? 1.0 : fp.w[e];
+            }
+            std::sort(touched.begin(), touched.end());
+            for (u32 l : touched) {
+              if (acc[l] != 0.0) blk.push_back({(u32)(d.offset[r] + l), acc[l]});
+              acc[l] = 0.0;
+              hit[l] = 0;
+            }
+          }
+          row.insert(row.end(), blk.begin(), blk.end());
+        }
       }
     }
   }
@@ -1347,7 +1364,7 @@ void build_plan(Plan& p) {
   }
   if (p.explicit_op || p.precond == 2) {
     PhaseTimer t("explicit gramian");
-    p.gram = build_gramian(d);
+    p.gram = build_gramian(d, p.passes);
     p.has_gram = true;
   }
   if ((p.precond == 1 || p.precond == 2) && !p.pre_given) {

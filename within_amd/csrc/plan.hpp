@@ -223,7 +223,7 @@ DeviceLayout make_device_layout(const Plan& p);
 // Individual steps (exposed for tests through the C ABI).
 std::vector<Subdomain> build_subdomains(const Design& d);
 void build_local_solver(Subdomain& s, const LocalConfig& cfg);
-Csr build_gramian(const Design& d);
+Csr build_gramian(const Design& d, const std::vector<FactorPass>& passes);
 std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n);
 // The same inverse evaluated on the GPU (dense.hip), bit for bit: false when
 // the anchored minor is not positive definite; throws wa::Error on a HIP
