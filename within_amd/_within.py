@@ -290,12 +290,15 @@ class _Design:
         h = C.c_void_p()
         N.check(L.wa_design_create(N.ptr(cats), self.n_obs, self.n_factors, max(self.n_obs, 1), N.ptr(w),
                                    C.byref(h)))
+        self._destroy = L.wa_design_destroy
         self.h = h
         self.n_dofs = L.wa_design_n_dofs(h)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            N.load().wa_design_destroy(self.h)
+        # the destroy function is held by the instance: at interpreter exit
+        # the module's globals may already be gone
+        if getattr(self, "h", None) and getattr(self, "_destroy", None):
+            self._destroy(self.h)
             self.h = None
 
 
@@ -325,14 +328,15 @@ class FePreconditioner:
         rc = N.load().wa_precond_create(_byte_buffer(b), len(b), 1 if _host_only else 0, C.byref(h))
         if rc != 0:
             raise ValueError(f"failed to deserialize preconditioner: {N.last_error()}")
+        self._destroy = N.load().wa_solver_destroy
         self._own = h
         self._h = h
         self._solver = None
         self._bytes = b
 
     def __del__(self):
-        if getattr(self, "_own", None):
-            N.load().wa_solver_destroy(self._own)
+        if getattr(self, "_own", None) and getattr(self, "_destroy", None):
+            self._destroy(self._own)
             self._own = None
 
     def _info(self):
@@ -432,14 +436,15 @@ class Solver:
             N.check(L.wa_solver_create_with_precond(self._design.h, C.byref(p), _byte_buffer(b), len(b),
                                                     C.byref(h)))
             N.check(L.wa_solver_params(h, C.byref(p)))
+        self._destroy = L.wa_solver_destroy
         self._h = h
         self._kind = int(p.preconditioner)
         self._params = p
         self.time_setup = time.perf_counter() - t0
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            N.load().wa_solver_destroy(self._h)
+        if getattr(self, "_h", None) and getattr(self, "_destroy", None):
+            self._destroy(self._h)
             self._h = None
 
     def set_config(self, config):

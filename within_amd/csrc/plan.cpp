@@ -1394,9 +1394,15 @@ void build_plan(Plan& p) {
         code[i] = e.code;
       }
     };
-    for (size_t i : serial) build_one(i);
+    {
+      PhaseTimer ts("local solvers, large subdomains");
+      for (size_t i : serial) build_one(i);
+    }
+    {
+      PhaseTimer tr("local solvers, the rest");
 #pragma omp parallel for schedule(dynamic, 1)
-    for (size_t t = 0; t < rest.size(); ++t) build_one(rest[t]);
+      for (size_t t = 0; t < rest.size(); ++t) build_one(rest[t]);
+    }
     for (size_t i = 0; i < p.subs.size(); ++i)
       if (code[i]) throw Error(code[i], err[i]);
     PhaseTimer tf("flatten");
