@@ -1,0 +1,77 @@
+"""GPU at the full metric size: the 20M-observation, 3-factor AKM panel
+(worker 2M Zipf 1.1, firm 200k with mobility 0.05, year 30), checked through
+size-independent properties instead of the oracle (which would take minutes
+here):
+
+* the mixed-precision block solve (FP32 inner CG, FP64 refinement, explicit
+  Gramian) reaches the FP64 solve's demeaned residuals within 1e-8 relative,
+  RHS by RHS, with converged flags (the metric's own acceptance rule);
+* the residual reported per RHS is ||D'W(y - Dx)|| / ||D'W y|| of the
+  returned demeaned vector (recomputed through the design operators);
+* repeated solves are bitwise identical (fixed-order reductions, no atomics).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import within_amd as W
+from within_amd import _native as N
+from within_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+K = 8
+
+
+@pytest.fixture(scope="module")
+def full():
+    p = synth.akm_panel(20_000_000, 2_000_000, 200_000, 30, zipf_s=1.1, mobility=0.05, seed=20)
+    codes = p["codes"]
+    n = codes.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    Y = torch.randn((K, n), generator=g, device="cuda", dtype=torch.float64)  # column-major n x K
+    Y[0] += torch.from_numpy(p["worker_fe"]).cuda()[torch.from_numpy(codes[:, 0].astype(np.int64)).cuda()]
+    prec = W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+    s = W.Solver(codes, config=W.CG(precision="mixed", tol=1e-9, operator=W.OperatorRepr.Explicit),
+                 preconditioner=prec)
+    return s, Y
+
+
+def solve_device(s, Y):
+    n, m = s.n_obs, s.n_dofs
+    X = torch.zeros((K, m), device="cuda", dtype=torch.float64)
+    D = torch.zeros((K, n), device="cuda", dtype=torch.float64)
+    stats = (N.RhsStats * K)()
+    torch.cuda.synchronize()
+    N.check(N.load().wa_solve_batch_device(s._h, C.c_void_p(Y.data_ptr()), K, n, C.c_void_p(X.data_ptr()), m,
+                                           C.c_void_p(D.data_ptr()), n, stats))
+    torch.cuda.synchronize()
+    return X, D, [(st.converged, st.iterations, st.residual) for st in stats]
+
+
+def test_full_size_mixed_solve_meets_the_metric_bar(full):
+    s, Y = full
+    _, D, st = solve_device(s, Y)
+    assert all(c for c, _, _ in st)
+    _, D2, _ = solve_device(s, Y)
+    assert torch.equal(D, D2)  # bitwise repeatable
+    s.set_config(W.CG(tol=1e-11, max_refinements=4, operator=W.OperatorRepr.Explicit))
+    _, Dref, st_ref = solve_device(s, Y)
+    assert all(c for c, _, _ in st_ref)
+    err = (torch.linalg.vector_norm(D - Dref, dim=1) / torch.linalg.vector_norm(Dref, dim=1)).cpu().numpy()
+    assert err.max() <= 1e-8, err
+    # the reported residual is the observation-space one of the returned vector
+    L = N.load()
+    m = s.n_dofs
+    for j in (0, K - 1):
+        d = np.ascontiguousarray(D[j].cpu().numpy())
+        y = np.ascontiguousarray(Y[j].cpu().numpy())
+        r, b = np.zeros(m), np.zeros(m)
+        N.check(L.wa_rmatvec_wdt(s._h, N.ptr(d), s.n_obs, N.ptr(r), m, 1))
+        N.check(L.wa_rmatvec_wdt(s._h, N.ptr(y), s.n_obs, N.ptr(b), m, 1))
+        res = np.linalg.norm(r) / max(np.linalg.norm(b), 1e-15)
+        assert res <= 1.01e-9
+        assert abs(st[j][2] - res) <= 1e-12 + 1e-3 * res, (st[j][2], res)
