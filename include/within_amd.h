@@ -4,7 +4,9 @@
  *
  * Plain pointers and sizes only.  Host-pointer entry points copy in and out;
  * *_device entry points take pointers to gfx950 device memory.  Every call is
- * synchronous on return.  Status codes mirror the reference's WithinError
+ * synchronous on return.  A solver handle may be shared between host threads:
+ * its calls are serialised (each uses the handle's stream and work buffers).
+ * Status codes mirror the reference's WithinError
  * variants (crates/within/src/error.rs:25-59); non-convergence is reported in
  * the per-RHS stats, never as an error (solver.rs:158-239).
  *

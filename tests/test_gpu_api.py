@@ -234,3 +234,27 @@ def test_properties_on_random_panels():
         for red in (W.ReductionStrategy.AtomicScatter, W.ReductionStrategy.ParallelReduction):
             rr = W.solve(c, y, preconditioner=W.AdditiveSchwarz(reduction=red))
             assert rr.converged and rel(rr.demeaned, r.demeaned) <= 1e-8
+
+
+def test_one_solver_shared_by_threads(problem):
+    """The reference's Solver is shared across threads (solver.rs:242-277);
+    concurrent calls on one handle give the sequential results."""
+    import threading
+
+    c, y = problem
+    s = W.Solver(c, W.CG(tol=1e-10))
+    rng = np.random.default_rng(5)
+    ys = [y + rng.standard_normal(len(y)) for _ in range(6)]
+    seq = [s.solve(v).demeaned for v in ys]
+    out = [None] * len(ys)
+
+    def work(i):
+        out[i] = s.solve(ys[i]).demeaned
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(len(ys))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for a, b in zip(out, seq):
+        assert np.array_equal(a, b)

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -92,6 +93,10 @@ struct Timer {
 };
 
 struct wa_solver {
+  // one call at a time uses the handle's stream and work buffers: calls from
+  // several host threads on one handle are serialised (the reference's
+  // Solver is shared across threads, solver.rs:242-277)
+  mutable std::mutex call_mu;
   Plan plan;
   wa_params params{};
   bool host_only = false;
@@ -1180,6 +1185,7 @@ static wa_status level_a(wa_solver* s, const double* in, int64_t rows_in, int64_
                          int64_t rows_out, int64_t ldout, int32_t k, F&& body) {
   try {
     need_device(s);
+    std::lock_guard<std::mutex> lock(s->call_mu);
     if (k <= 0 || k > KMAX) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be in 1..32");
     if (!in || !out || ldin < rows_in || ldout < rows_out)
       throw Error(WA_ERR_INVALID_ARGUMENT, "bad buffer or leading dimension");
@@ -1357,6 +1363,7 @@ wa_status wa_precond_create(const uint8_t* bytes, int64_t len, int32_t host_only
 wa_status wa_solver_precond_bytes(const wa_solver* s, uint8_t* buf, int64_t* len) {
   try {
     if (!s || !len) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lock(s->call_mu);
     if (s->plan.meta.empty()) throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no preconditioner");
     if (s->precond_bytes.empty()) s->precond_bytes = serialize_precond(s->plan);
     const int64_t need = (int64_t)s->precond_bytes.size();
@@ -1376,6 +1383,7 @@ wa_status wa_solver_params(const wa_solver* s, wa_params* out) {
     g_err = "null argument";
     return WA_ERR_INVALID_ARGUMENT;
   }
+  std::lock_guard<std::mutex> lock(s->call_mu);
   *out = s->params;
   return WA_OK;
 }
@@ -1385,6 +1393,7 @@ void wa_solver_destroy(wa_solver* s) { delete s; }
 wa_status wa_solver_set_params(wa_solver* s, const wa_params* p) {
   try {
     if (!s || !p) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lock(s->call_mu);
     const wa_params& o = s->params;
     if (p->preconditioner != o.preconditioner || std::memcmp(&p->local, &o.local, sizeof(wa_local_config)) != 0)
       throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner settings are fixed when the solver is built");
@@ -1410,6 +1419,7 @@ static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ld
     if (ldy < s->n || ldx < s->mu || (dem && ldd < s->n))
       throw Error(WA_ERR_OBS_COUNT_MISMATCH, "ObservationCountMismatch: leading dimension too small");
     if (k == 0) return WA_OK;
+    std::lock_guard<std::mutex> lock(s->call_mu);
     const double* dY = Y;
     double *dX = X, *dD = dem;
     if (!device) {
@@ -1598,14 +1608,18 @@ wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz
 }
 
 void wa_timing_enable(wa_solver* s, int32_t on) {
-  if (s) s->timer.on = on != 0 && !s->host_only;
+  if (!s) return;
+  std::lock_guard<std::mutex> lock(s->call_mu);
+  s->timer.on = on != 0 && !s->host_only;
 }
 void wa_timing_reset(wa_solver* s) {
   if (!s) return;
+  std::lock_guard<std::mutex> lock(s->call_mu);
   for (int i = 0; i < 5; ++i) { s->timer.launches[i] = 0; s->timer.ms[i] = 0; }
 }
 wa_status wa_timing_get(wa_solver* s, int32_t kclass, int64_t* launches, double* ms) {
   if (!s || kclass < 0 || kclass > 4) { g_err = "bad timing class"; return WA_ERR_INVALID_ARGUMENT; }
+  std::lock_guard<std::mutex> lock(s->call_mu);
   if (launches) *launches = s->timer.launches[kclass];
   if (ms) *ms = s->timer.ms[kclass];
   return WA_OK;
