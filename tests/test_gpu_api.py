@@ -258,3 +258,27 @@ def test_one_solver_shared_by_threads(problem):
         t.join()
     for a, b in zip(out, seq):
         assert np.array_equal(a, b)
+
+
+def test_single_factor_matches_the_reference_semantics(oracle_mod):
+    """With one factor the Schwarz preconditioner has no subdomains and its
+    apply is zero (the additive sum over no subdomains), so CG stops at once,
+    unconverged, with x = 0, as the oracle (and the reference) do; with the
+    preconditioner off the solve converges to the group means."""
+    rng = np.random.default_rng(0)
+    c1 = np.asfortranarray(rng.integers(0, 50, (1000, 1)).astype(np.uint32))
+    y = rng.standard_normal(1000)
+    r = W.solve(c1, y)
+    ro = oracle_mod.OracleSolver(c1, None, ac_split=2).solve_batch(y[:, None])
+    assert r.converged == bool(ro["converged"][0]) is False
+    assert r.iterations == int(ro["iterations"][0])
+    assert np.array_equal(r.demeaned, y) and not np.any(r.x)
+    p = W.Solver(c1).preconditioner()
+    assert p.n_subdomains == 0 and not np.any(p.apply(np.ones(50)))
+    r0 = W.solve(c1, y, preconditioner=W.Preconditioner.Off)
+    ro0 = oracle_mod.OracleSolver(c1, None, precond=0).solve_batch(y[:, None])
+    assert r0.converged and bool(ro0["converged"][0])
+    assert rel(r0.demeaned, ro0["demeaned"][:, 0]) <= 1e-10
+    means = np.bincount(c1[:, 0], weights=y, minlength=50) / np.maximum(np.bincount(c1[:, 0], minlength=50), 1)
+    r1 = W.solve(c1, y, W.CG(tol=1e-13), preconditioner=W.Preconditioner.Off)
+    assert np.abs(r1.demeaned - (y - means[c1[:, 0]])).max() <= 1e-12

@@ -141,3 +141,18 @@ def test_corrupted_bytes_never_crash(akm):
             continue
         assert p.to_bytes() == bytes(a)
     assert rejected > 0
+
+
+def test_single_factor_preconditioner_has_no_subdomains():
+    """One factor: no factor pairs, so the Schwarz preconditioner has no
+    subdomains (factor_pairs.rs:63-304); it still exists, serialises and
+    reports zero scheduling work."""
+    c1 = np.asfortranarray(np.random.default_rng(0).integers(0, 50, (1000, 1)).astype(np.uint32))
+    s = host_solver(c1)
+    p = s.preconditioner()
+    assert p is not None and p.n_subdomains == 0 and p.subdomain_inner_parallel_work == []
+    q = W.FePreconditioner(p.to_bytes(), _host_only=True)
+    assert q.to_bytes() == p.to_bytes() and q.nrows == 50
+    d = q.additive_schwarz_diagnostics()
+    assert (d.total_inner_parallel_work, d.total_scatter_dofs, d.outer_parallel_capacity) == (0, 0, 0.0)
+    assert host_solver(c1, preconditioner=q).info()[0] == 0

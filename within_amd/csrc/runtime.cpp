@@ -234,6 +234,7 @@ struct wa_solver {
   void setup_device();
   template <typename T> void op_apply(const T* X, T* Y, int kp);
   template <typename T> void pre_apply(const T* R, T* Z, int kp);
+  template <typename T> void precondition(const T* R, T* Z, int kp);
   void rhs64(const double* V, double* out, int kp);
   template <typename T>
   void cg(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
@@ -647,6 +648,18 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp) {
   toc(1, a);
 }
 
+// z = M^-1 r.  No preconditioner: z = r.  A Schwarz preconditioner with no
+// subdomains (one factor: no factor pairs) sums over nothing, z = 0, as the
+// reference's additive and multiplicative applies do
+// (schwarz/additive/preconditioner.rs:164-214, multiplicative.rs:128-245).
+template <typename T>
+void wa_solver::precondition(const T* R, T* Z, int kp) {
+  const int64_t bytes = sizeof(T) * m * kp;
+  if (params.preconditioner == 0) HIPCHK(hipMemcpyAsync(Z, R, bytes, hipMemcpyDeviceToDevice, st));
+  else if (has_pre) pre_apply<T>(R, Z, kp);
+  else HIPCHK(hipMemsetAsync(Z, 0, bytes, st));
+}
+
 void wa_solver::rhs64(const double* V, double* out, int kp) {
   hipEvent_t a = nullptr;
   tic(3, a);
@@ -680,8 +693,7 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
   HIPCHK(hipMemsetAsync(x, 0, bytes, st));
   HIPCHK(hipMemcpyAsync(r, rhs, bytes, hipMemcpyDeviceToDevice, st));
   hipEvent_t ev = nullptr;
-  if (has_pre && params.preconditioner != 0) pre_apply<T>(r, z, kp);
-  else HIPCHK(hipMemcpyAsync(z, r, bytes, hipMemcpyDeviceToDevice, st));
+  precondition<T>(r, z, kp);
   HIPCHK(hipMemcpyAsync(p, z, bytes, hipMemcpyDeviceToDevice, st));
   tic(2, ev);
   launch_dot<T>(st, r, z, m, kp, redp, rz);
@@ -707,8 +719,7 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
     nb = launch_cg_update<T>(st, x, p, r, ap, alpha, m, kp, redp);
     launch_cg_check(st, cs, redp, nb, rr, (int)itn, kp);
     toc(2, ev);
-    if (has_pre && params.preconditioner != 0) pre_apply<T>(r, z, kp);
-    else HIPCHK(hipMemcpyAsync(z, r, bytes, hipMemcpyDeviceToDevice, st));
+    precondition<T>(r, z, kp);
     tic(2, ev);
     nb = launch_dot_part<T>(st, r, z, m, kp, redp);
     launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, (int)itn, kp);
@@ -798,8 +809,7 @@ void wa_solver::gmres(int k, int kp, const T* b, T* x, const std::vector<double>
     int j = 0;
     auto any_inc = [&]() { for (int c = 0; c < kp; ++c) if (inc[c]) return true; return false; };
     while (j < cycle && any_inc()) {
-      if (has_pre && params.preconditioner != 0) pre_apply<T>(V(j), Z(j), kp);
-      else HIPCHK(hipMemcpyAsync(Z(j), V(j), sizeof(T) * vb, hipMemcpyDeviceToDevice, st));
+      precondition<T>(V(j), Z(j), kp);
       op_apply<T>(Z(j), w, kp);
       hipEvent_t ev = nullptr;
       tic(2, ev);
@@ -926,17 +936,13 @@ void wa_solver::lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>
   }
   double *X = X64, *V = cx64, *P = cr64, *H = cz64, *HB = cp64, *ATU = cap64, *U = E;
   int* nact = imask + KMAX;
-  const bool pre = has_pre && params.preconditioner != 0;
   hipEvent_t ev = nullptr;
   auto adjoint = [&]() {  // ATU = D' (W^(1/2) U)
     tic(3, ev);
     for (int q = 0; q < Q; ++q) launch_scatter_obs(st, p64[q], swobs, U, ATU, part64, kp);
     toc(3, ev);
   };
-  auto solve_m = [&]() {  // V = M^-1 P
-    if (pre) pre_apply<double>(P, V, kp);
-    else HIPCHK(hipMemcpyAsync(V, P, vbytes, hipMemcpyDeviceToDevice, st));
-  };
+  auto solve_m = [&]() { precondition<double>(P, V, kp); };  // V = M^-1 P
   int64_t count = 0, next = 0;
   auto push = [&]() {
     if (cap == 0) return;
@@ -1364,7 +1370,7 @@ wa_status wa_solver_precond_bytes(const wa_solver* s, uint8_t* buf, int64_t* len
   try {
     if (!s || !len) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lock(s->call_mu);
-    if (s->plan.meta.empty()) throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no preconditioner");
+    if (s->plan.precond == 0) throw Error(WA_ERR_INVALID_ARGUMENT, "solver has no preconditioner");
     if (s->precond_bytes.empty()) s->precond_bytes = serialize_precond(s->plan);
     const int64_t need = (int64_t)s->precond_bytes.size();
     if (buf) {
@@ -1495,13 +1501,13 @@ wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32
 }
 
 wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld) {
-  if (s && !s->has_pre) {
+  if (s && s->params.preconditioner == 0) {
     g_err = "solver has no preconditioner";
     return WA_ERR_INVALID_ARGUMENT;
   }
   return level_a(s, R, s ? s->mu : 0, ld, Z, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
     launch_col_to_row(s->st, din, s->mu, s->m, k, s->cr64, kp, s->ext_of);
-    s->pre_apply<double>(s->cr64, s->cz64, kp);
+    s->precondition<double>(s->cr64, s->cz64, kp);
     launch_row_to_col(s->st, s->cz64, kp, s->mu, k, dout, s->mu, s->int_of);
   });
 }
@@ -1532,7 +1538,7 @@ wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, 
 
 wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4) {
   if (!s || !out4) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
-  if (s->plan.meta.empty()) { g_err = "solver has no preconditioner"; return WA_ERR_INVALID_ARGUMENT; }
+  if (s->plan.precond == 0) { g_err = "solver has no preconditioner"; return WA_ERR_INVALID_ARGUMENT; }
   int64_t total = 0, mx = 0, scatter = 0;
   for (size_t i = 0; i < s->plan.meta.size(); ++i) {
     const SubMeta& m = s->plan.meta[i];
