@@ -117,6 +117,11 @@ def test_edge_columns_and_errors():
     bad[7] = np.nan
     with pytest.raises(N.NativeError):
         s.solve(bad)
-    # the solver keeps working after the error
+    # a negative or NaN tolerance is an input error for LSMR (lsmr.rs:262-266)
+    for bad_tol in (-1.0, float("nan")):
+        s.set_config(W.LSMR(tol=bad_tol))
+        with pytest.raises(N.NativeError):
+            s.solve(Y[:, 0])
+    # the solver keeps working after the errors
     s.set_config(W.LSMR(tol=1e-10))
     assert s.solve(Y[:, 0]).converged

@@ -917,6 +917,8 @@ void wa_solver::gmres(int k, int kp, const T* b, T* x, const std::vector<double>
 // arithmetic (lsmr/tests.rs:270-309).  Reads Y from Yr; leaves x in X64.
 void wa_solver::lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>& its) {
   const Design& d = plan.d;
+  if (!std::isfinite(params.tol) || params.tol < 0.0)  // lsmr.rs:262-266
+    throw Error(WA_ERR_ITERATIVE_SOLVE, "lsmr: tolerance must be finite and nonnegative");
   const int64_t vbytes = sizeof(double) * m * kp;
   if (!lpart) {
     lpart = dalloc<double>((int64_t)kLsmrBlocks * KMAX);
