@@ -138,6 +138,16 @@ def test_input_validation():
         warnings.simplefilter("always")
         host_solver(c)
     assert any("F-contiguous" in str(r.message) for r in rec)
+    # the reference's test is the element stride along the observations
+    # (within-py/src/lib.rs:732-744): F-order and single-factor C-order
+    # arrays pass silently, a column slice of an F-order array too; a column
+    # slice of a C-order array (stride 2) warns
+    with pytest.warns(UserWarning, match="F-contiguous"):
+        host_solver(c[:, :1])
+    for ok in (np.asfortranarray(c), np.ascontiguousarray(c[:, :1]), np.asfortranarray(np.column_stack([c, c]))[:, ::2]):
+        with warnings.catch_warnings():
+            warnings.simplefilter("error")
+            host_solver(ok)
 
 
 def test_empty_observations_error_code():

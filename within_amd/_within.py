@@ -268,7 +268,9 @@ def _categories(categories):
         raise ValueError("categories must be a 2-D array of shape (n_obs, n_factors)")
     if a.dtype != np.uint32:
         raise TypeError("categories must have dtype uint32")
-    if a.shape[0] > 1 and a.shape[1] > 1 and not a.flags.f_contiguous:
+    # within-py/src/lib.rs:732-744: warn unless consecutive observations of a
+    # factor are adjacent (element stride 1 along axis 0; 0 for empty arrays)
+    if a.strides[0] // a.itemsize not in (0, 1):
         warnings.warn("categories array is not F-contiguous (column-major). Use np.asfortranarray(categories) "
                       "for faster solves.", UserWarning, stacklevel=3)
     return np.asfortranarray(a)
