@@ -167,7 +167,9 @@ wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4);
  * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
  * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,
  * operator_passes, dense_subdomains, max_core, device_dofs (active levels
- * the device solves over; 0 for a host-only solver), reserved. */
+ * the device solves over; 0 for a host-only solver), apply_streams (2 when the
+ * additive apply runs the factor pairs' local solves on two streams, else 1;
+ * 0 for a host-only solver). */
 wa_status wa_solver_info(const wa_solver* s, int64_t* info16);
 /* Subdomain i: meta[0..7] = q, r, n_q, n_r, eliminate_q, dense, rounds, core
  * size; gidx (n_q + n_r), pou (n_q + n_r) may be NULL. */

@@ -295,3 +295,31 @@ def test_dense_core_multiplicative_apply_matches_oracle(oracle_mod):
     N.check(N.load().wa_precond_apply(s._h, N.ptr(R), N.ptr(Z), 3, m))
     ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(3)])
     assert rel(Z, ref) <= 1e-10, rel(Z, ref)
+
+
+def test_two_stream_apply_is_bitwise_the_one_stream_apply(oracle_mod, monkeypatch):
+    """With a 64-vertex dense core the worker-firm pair keeps 11 elimination
+    levels, so the additive apply runs the other pairs' local solves on a
+    second stream beside that level chain.  Same kernels on disjoint rows: the
+    result equals the one-stream apply bit for bit, for applies and for whole
+    mixed-precision solves, and the oracle to 1e-10."""
+    o = oracle_mod.OracleSolver(BIG, None, ac_split=1, dense_core=64)
+    cfg = W.CG(tol=1e-10, max_refinements=3, precision="mixed")
+    two = W.Solver(BIG, config=cfg, preconditioner=RUST_DEFAULT, _dense_core=64)
+    monkeypatch.setenv("WITHIN_AMD_PRECOND_STREAMS", "1")
+    one = W.Solver(BIG, config=cfg, preconditioner=RUST_DEFAULT, _dense_core=64)
+    assert two.info()[15] == 2 and one.info()[15] == 1
+    m = two.n_dofs
+    rng = np.random.default_rng(31)
+    for k in (1, 4, 32):
+        R = np.asfortranarray(rng.standard_normal((m, k)))
+        Z1, Z2 = np.zeros((m, k), order="F"), np.zeros((m, k), order="F")
+        N.check(N.load().wa_precond_apply(one._h, N.ptr(R), N.ptr(Z1), k, m))
+        N.check(N.load().wa_precond_apply(two._h, N.ptr(R), N.ptr(Z2), k, m))
+        assert np.array_equal(Z1, Z2), k
+        ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(k)])
+        assert rel(Z2, ref) <= 1e-10, (k, rel(Z2, ref))
+    Y = np.asfortranarray(rng.standard_normal((BIG.shape[0], 8)))
+    a, b = one.solve_batch(Y), two.solve_batch(Y)
+    assert all(a.converged) and list(a.iterations) == list(b.iterations)
+    assert np.array_equal(a.demeaned, b.demeaned) and np.array_equal(a.x, b.x)

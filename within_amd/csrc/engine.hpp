@@ -28,7 +28,7 @@
 namespace wa {
 
 constexpr int kWave = 64;
-constexpr int kFixBlock = 512;
+constexpr int kFixBlock = 1024;
 
 template <typename T, int NV>
 struct alignas(sizeof(T) * NV) VecT {
@@ -210,20 +210,21 @@ __global__ void __launch_bounds__(kBlock) k_rows(K k, uint32_t r0, uint32_t r1, 
 }
 
 // ---- fix-up of one split row by a workgroup of NT threads: threads stride
-// over the slots with four independent sums, then a fixed-order tree -------
+// over the slots with eight independent sums (eight loads in flight: a row
+// of a few thousand slots is latency bound), then a fixed-order tree -------
 template <class K, typename T, int NT>
 __device__ __forceinline__ void fix_row(const K& k, const Fixup& f, const T* __restrict__ part, int kp, T* sh) {
+  constexpr int U = 8;
   const int col = threadIdx.x % kp, str = NT / kp, s0 = threadIdx.x / kp;
-  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  T a[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) a[u] = 0;
   uint32_t s = f.slot0 + s0;
-  for (; s + 3 * str < f.slot1; s += 4 * str) {
-    a0 += part[(int64_t)s * kp + col];
-    a1 += part[(int64_t)(s + str) * kp + col];
-    a2 += part[(int64_t)(s + 2 * str) * kp + col];
-    a3 += part[(int64_t)(s + 3 * str) * kp + col];
-  }
-  for (; s < f.slot1; s += str) a0 += part[(int64_t)s * kp + col];
-  sh[threadIdx.x] = (a0 + a1) + (a2 + a3);
+  for (; s + (U - 1) * str < f.slot1; s += U * str)
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] += part[(int64_t)(s + u * str) * kp + col];
+  for (; s < f.slot1; s += str) a[0] += part[(int64_t)s * kp + col];
+  sh[threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   __syncthreads();
   for (int w = str / 2; w >= 1; w >>= 1) {
     if (s0 < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];

@@ -1640,6 +1640,106 @@ template void launch_rsub<double>(hipStream_t, double*, const double*, int64_t);
 
 namespace wa {
 
+// Local solves of one factor pair's subdomains, restricted to the pair's
+// rows, in three stages: the centred Schur right-hand side, the reduced solve
+// (forward levels, dense cores, backward levels) and the back-substitution of
+// the eliminated block.  `part`, `Zc` and `P` are the split-row partials and
+// dense-core scratch of the stream the stage runs on.
+template <typename T>
+static void pair_schur_rhs(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
+                           int kp) {
+  if (ps.red1 == ps.red0) return;
+  run_rows<KeepKind<T>, T>(st,
+                           KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
+                                       p.red_sub, p.loc_ptr, w.sum1, kp},
+                           ps.s_keep, part, kp);
+  run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, part, kp);
+  launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
+}
+
+template <typename T>
+static void pair_reduced(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
+                         T* Zc, T* P, int kp) {
+  if (ps.red1 == ps.red0) return;
+  FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
+  run_levels<FwdKind<T>, T>(st, fk, ps.s_fwd, 0, (int)ps.s_fwd.segs.size(), 1, part, kp);
+  if (ps.core1 > ps.core0) {
+    run_rows<CoreKind<T>, T>(st,
+                             CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
+                                         p.core_mat_off, p.core_mat, kp},
+                             ps.s_core, part, kp);
+    for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, Zc, P, kp);
+    launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
+  }
+  BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
+  // the last level gathers the dense cores: no backward step
+  run_levels<BwdKind<T>, T>(st, bk, ps.s_bwd, 0, (int)ps.s_bwd.segs.size() - 1, -1, part, kp);
+  launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
+}
+
+template <typename T>
+static void pair_elim(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
+                      int kp) {
+  run_rows<ElimKind<T>, T>(st,
+                           ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
+                                       p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
+                           ps.s_elim, part, kp);
+}
+
+template <typename T>
+static void pair_solve(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
+                       T* Zc, T* P, int kp) {
+  pair_schur_rhs<T>(st, p, ps, w, part, kp);
+  pair_reduced<T>(st, p, ps, w, part, Zc, P, kp);
+  pair_elim<T>(st, p, ps, w, part, kp);
+}
+
+// Additive Schwarz with the factor pairs on two streams: the pair with the
+// longest level chain (`lead`, latency bound: one short launch per
+// elimination level) on `st`, the other pairs beside it on `st2`.  Each
+// stream restricts, solves and sums its own pairs' subdomains; the combine
+// waits for both.  Every row is computed by the same kernel code as in
+// launch_precond and the pairs touch disjoint local rows, so the result is
+// bitwise that of launch_precond.  The lead's first stage is enqueued before
+// the other pairs' work so that its chain starts at once.
+template <typename T>
+void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
+                            const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
+                            T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m) {
+  auto restrict_pair = [&](hipStream_t s, const PairSched& ps, T* part) {
+    run_rows<RestrictSegKind<T>, T>(s, RestrictSegKind<T>{R, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp},
+                                    ps.s_loc, part, kp);
+  };
+  auto centre_pair = [&](hipStream_t s, const PairSched& ps, T* part) {
+    run_rows<SegKind<T>, T>(s, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, ps.s_loc, part, kp);
+  };
+  const PairSched& a = pairs[lead];
+  (void)hipEventRecord(fork, st);
+  (void)hipStreamWaitEvent(st2, fork, 0);
+  restrict_pair(st, a, w.part);
+  pair_schur_rhs<T>(st, p, a, w, w.part, kp);
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    if (i == lead) continue;
+    restrict_pair(st2, pairs[i], part2);
+    pair_solve<T>(st2, p, pairs[i], w, part2, Zc2, P2, kp);
+    centre_pair(st2, pairs[i], part2);
+  }
+  (void)hipEventRecord(join, st2);
+  pair_reduced<T>(st, p, a, w, w.part, w.Zc, w.P, kp);
+  pair_elim<T>(st, p, a, w, w.part, kp);
+  centre_pair(st, a, w.part);
+  (void)hipStreamWaitEvent(st, join, 0);
+  launch_combine<T>(st, m, p.dof_ptr, p.dof_loc, p.loc_scale, p.loc_sub, p.loc_ptr, w.S, w.sum3, Z, kp);
+}
+template void launch_precond_overlap<float>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
+                                            const DevPrecond<float>&, const std::vector<PairSched>&, size_t,
+                                            const float*, float*, PrecondWork<float>&, float*, float*, float*, int,
+                                            int64_t);
+template void launch_precond_overlap<double>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
+                                             const DevPrecond<double>&, const std::vector<PairSched>&, size_t,
+                                             const double*, double*, PrecondWork<double>&, double*, double*, double*,
+                                             int, int64_t);
+
 // prolongation of one pair (its subdomains cover distinct DOFs):
 //   d = scale_l (S[l] - mu3); Z[g_l] += d; delta[g_l] = d
 template <typename T>
@@ -1677,32 +1777,7 @@ void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vect
     const int64_t nl = ps.loc1 - ps.loc0;
     run_rows<RestrictSegKind<T>, T>(
         st, RestrictSegKind<T>{rwork, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp}, ps.s_loc, w.part, kp);
-    const int64_t nr = ps.red1 - ps.red0;
-    if (nr > 0) {
-      run_rows<KeepKind<T>, T>(st,
-                               KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
-                                           p.red_sub, p.loc_ptr, w.sum1, kp},
-                               ps.s_keep, w.part, kp);
-      run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, w.part, kp);
-      launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
-      FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-      run_levels<FwdKind<T>, T>(st, fk, ps.s_fwd, 0, (int)ps.s_fwd.segs.size(), 1, w.part, kp);
-      if (ps.core1 > ps.core0) {
-        run_rows<CoreKind<T>, T>(st,
-                                 CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
-                                             p.core_mat_off, p.core_mat, kp},
-                                 ps.s_core, w.part, kp);
-        for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, w.Zc, w.P, kp);
-        launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
-      }
-      BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
-      run_levels<BwdKind<T>, T>(st, bk, ps.s_bwd, 0, (int)ps.s_bwd.segs.size() - 1, -1, w.part, kp);
-      launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
-    }
-    run_rows<ElimKind<T>, T>(st,
-                             ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
-                                         p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
-                             ps.s_elim, w.part, kp);
+    pair_solve<T>(st, p, ps, w, w.part, w.Zc, w.P, kp);
     run_rows<SegKind<T>, T>(st, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, ps.s_loc, w.part, kp);
     (void)hipMemsetAsync(delta, 0, vbytes, st);
     k_prolong_pair<T><<<grid_stride_blocks(nl, kp), kBlock, 0, st>>>(ps.loc0, ps.loc1, p.loc_gidx, p.loc_scale,

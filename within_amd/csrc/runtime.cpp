@@ -148,7 +148,16 @@ struct wa_solver {
     T *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
     double* hd = nullptr;
   };
-  std::vector<PairSched> pairs;            // multiplicative sweep subranges
+  std::vector<PairSched> pairs;            // per-factor-pair subranges (sweep / overlapped apply)
+  // additive apply with the pairs' local solves on two streams: the lead
+  // pair (longest level chain) on st, the others on st2
+  bool overlap = false;
+  size_t lead = 0;
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  double* part2 = nullptr;                 // st2's split-row partials
+  double *zc2_64 = nullptr, *p2_64 = nullptr;  // st2's dense-core scratch
+  float *zc2_32 = nullptr, *p2_32 = nullptr;
   double* mw64 = nullptr;                  // r_work, delta, G delta (multiplicative)
   float* mw32 = nullptr;
   double* ones = nullptr;
@@ -212,7 +221,11 @@ struct wa_solver {
   ~wa_solver() {
     if (st) (void)hipStreamSynchronize(st);
     for (void* p : allocs) (void)hipFree(p);
+    if (st2) (void)hipStreamSynchronize(st2);
     if (hbuf) (void)hipHostFree(hbuf);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -232,6 +245,7 @@ struct wa_solver {
   }
 
   void setup_device();
+  void setup_overlap();
   template <typename T> void op_apply(const T* X, T* Y, int kp);
   template <typename T> void pre_apply(const T* R, T* Z, int kp);
   template <typename T> void precondition(const T* R, T* Z, int kp);
@@ -505,7 +519,7 @@ void wa_solver::setup_device() {
     ints(pre32);
     scheds(pre64);
     scheds(pre32);
-    if (plan.precond == 2) {
+    {
       // per-pair subranges (subdomains are ordered by factor pair)
       std::vector<int64_t> elim_ptr_sub(1, 0), core_row_sub(1, 0);
       for (const SubMeta& sd : plan.meta) {
@@ -554,10 +568,12 @@ void wa_solver::setup_device() {
         pairs.push_back(std::move(ps));
         s0 = s1;
       }
-      mw64 = dalloc<double>(3 * m * KMAX);
-      mw32 = dalloc<float>(3 * m * KMAX);
-      std::vector<double> one(KMAX, 1.0);
-      ones = upload(one);
+      if (plan.precond == 2) {
+        mw64 = dalloc<double>(3 * m * KMAX);
+        mw32 = dalloc<float>(3 * m * KMAX);
+        std::vector<double> one(KMAX, 1.0);
+        ones = upload(one);
+      }
     }
     int64_t nsub = std::max<int64_t>(f.n_sub, 1);
     w64.B = dalloc<double>(f.n_loc * KMAX);
@@ -583,6 +599,7 @@ void wa_solver::setup_device() {
     w32.sum2 = dalloc<float>(nsub * KMAX);
     w32.sum3 = dalloc<float>(nsub * KMAX);
     has_pre = true;
+    setup_overlap();
   }
   phase("preconditioner");
   part64 = dalloc<double>(max_slots * KMAX);
@@ -607,6 +624,48 @@ void wa_solver::setup_device() {
   rhs32 = dalloc<float>(m * KMAX);
   HIPCHK(hipDeviceSynchronize());
   phase("work vectors");
+}
+
+// Two-stream additive apply.  The elimination levels of a large reduced
+// system are one short launch each, bound by latency rather than bandwidth;
+// the other factor pairs' local solves (long Schur-block rows, eliminated
+// rows) run beside them on a second stream.  Only when one pair has a level
+// chain worth hiding (at least kOverlapLevels forward levels) and another
+// pair has work; WITHIN_AMD_PRECOND_STREAMS=1 keeps every apply on one stream.
+void wa_solver::setup_overlap() {
+  constexpr int kOverlapLevels = 8;
+  const char* e = std::getenv("WITHIN_AMD_PRECOND_STREAMS");
+  if (plan.precond != 1 || pairs.size() < 2 || (e && std::atoi(e) == 1)) return;
+  auto depth = [](const PairSched& ps) {
+    int d = 0;
+    for (const Segment& g : ps.s_fwd.segs) d += (g.r1 > g.r0) ? 1 : 0;
+    return d;
+  };
+  lead = 0;
+  for (size_t i = 1; i < pairs.size(); ++i)
+    if (depth(pairs[i]) > depth(pairs[lead]) ||
+        (depth(pairs[i]) == depth(pairs[lead]) && pairs[i].red1 - pairs[i].red0 > pairs[lead].red1 - pairs[lead].red0))
+      lead = i;
+  if (depth(pairs[lead]) < kOverlapLevels) return;
+  int64_t slots = 1, ld = 0;
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    if (i == lead) continue;
+    const PairSched& ps = pairs[i];
+    for (const DevSched* d : {&ps.s_loc, &ps.s_keep, &ps.s_red, &ps.s_elim, &ps.s_fwd, &ps.s_bwd, &ps.s_core})
+      slots = std::max(slots, d->n_slots);
+    for (size_t bi : ps.big) ld = std::max<int64_t>(ld, pre64.big[bi].ld);
+  }
+  part2 = dalloc<double>(slots * KMAX);
+  if (ld > 0) {
+    zc2_64 = dalloc<double>(ld * KMAX);
+    p2_64 = dalloc<double>(kCoreSplitMax * ld * KMAX);
+    zc2_32 = dalloc<float>(ld * KMAX);
+    p2_32 = dalloc<float>(kCoreSplitMax * ld * KMAX);
+  }
+  HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  overlap = true;
 }
 
 template <typename T>
@@ -640,6 +699,13 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp) {
     else
       launch_precond_mult<float>(st, pre32, pairs, g32, R, Z, w32, mw32, mw32 + m * KMAX, mw32 + 2 * m * KMAX,
                                  ones, kp, m);
+  } else if (overlap) {
+    if constexpr (std::is_same<T, double>::value)
+      launch_precond_overlap<double>(st, st2, ev_fork, ev_join, pre64, pairs, lead, R, Z, w64, part2, zc2_64, p2_64,
+                                     kp, m);
+    else
+      launch_precond_overlap<float>(st, st2, ev_fork, ev_join, pre32, pairs, lead, R, Z, w32, (float*)part2, zc2_32,
+                                    p2_32, kp, m);
   } else if constexpr (std::is_same<T, double>::value) {
     launch_precond<double>(st, pre64, R, Z, w64, kp, m);
   } else {
@@ -1576,6 +1642,7 @@ wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
   info[12] = f.dense_subs;
   info[13] = f.max_core;
   info[14] = s->host_only ? 0 : s->m;  // DOFs the device works on (active levels)
+  info[15] = s->host_only ? 0 : (s->overlap ? 2 : 1);  // streams of the additive apply
   return WA_OK;
 }
 
