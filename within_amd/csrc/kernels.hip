@@ -278,7 +278,7 @@ template void launch_dot<float>(hipStream_t, const float*, const float*, int64_t
 template void launch_dot<double>(hipStream_t, const double*, const double*, int64_t, int, double*, double*);
 
 // X += alpha P ; R -= alpha AP ; part = sum R^2   (alpha per column, FP64)
-template <typename T, int NV>
+template <typename T, int NV, bool UPDX>
 __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T* __restrict__ P,
                                                       T* __restrict__ R, const T* __restrict__ AP,
                                                       const double* __restrict__ alpha, int64_t m, int kp,
@@ -295,27 +295,60 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T
 #pragma unroll 2
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < m; i += (int64_t)gridDim.x * gpb) {
     const int64_t id = i * kp + pt * NV;
-    VecT<T, NV> x = vload<T, NV>(X + id), r = vload<T, NV>(R + id);
-    const VecT<T, NV> p = vload<T, NV>(P + id), ap = vload<T, NV>(AP + id);
+    VecT<T, NV> r = vload<T, NV>(R + id);
+    const VecT<T, NV> ap = vload<T, NV>(AP + id);
+    if constexpr (UPDX) {
+      VecT<T, NV> x = vload<T, NV>(X + id);
+      const VecT<T, NV> p = vload<T, NV>(P + id);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) x.v[c] += a[c] * p.v[c];
+      vstore<T, NV>(X + id, x);
+    }
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      x.v[c] += a[c] * p.v[c];
       r.v[c] = r.v[c] - a[c] * ap.v[c];
       acc[c] += (double)r.v[c] * (double)r.v[c];
     }
-    vstore<T, NV>(X + id, x);
     vstore<T, NV>(R + id, r);
   }
   block_col_sums<NV>(acc, sh, kp, gpb, part);
 }
 
+// X = nullptr: only R (and the partials); X is then updated by launch_cg_x
 template <typename T>
 int launch_cg_update(hipStream_t st, T* X, const T* P, T* R, const T* AP, const double* alpha, int64_t m, int kp,
                      double* part) {
   unsigned nb = red_blocks(m, kp);
-  WA_VEC_SWITCH(T, kp, (k_cg_update<T, NV><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part)));
+  if (X)
+    WA_VEC_SWITCH(T, kp, (k_cg_update<T, NV, true><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part)));
+  else
+    WA_VEC_SWITCH(T, kp, (k_cg_update<T, NV, false><<<nb, kBlock, 0, st>>>(X, P, R, AP, alpha, m, kp, part)));
   return (int)nb;
 }
+
+// X += alpha P (alpha per column; the same arithmetic as k_cg_update's X)
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock) k_cg_x(T* __restrict__ X, const T* __restrict__ P,
+                                                 const double* __restrict__ alpha, int64_t m, int kp) {
+  const int V = kp / NV, pt = threadIdx.x % V, gpb = kBlock / V;
+  T a[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) a[c] = (T)alpha[pt * NV + c];
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / V; i < m; i += (int64_t)gridDim.x * gpb) {
+    const int64_t id = i * kp + pt * NV;
+    VecT<T, NV> x = vload<T, NV>(X + id);
+    const VecT<T, NV> p = vload<T, NV>(P + id);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) x.v[c] += a[c] * p.v[c];
+    vstore<T, NV>(X + id, x);
+  }
+}
+template <typename T>
+void launch_cg_x(hipStream_t st, T* X, const T* P, const double* alpha, int64_t m, int kp) {
+  WA_VEC_SWITCH(T, kp, (k_cg_x<T, NV><<<red_blocks(m, kp), kBlock, 0, st>>>(X, P, alpha, m, kp)));
+}
+template void launch_cg_x<float>(hipStream_t, float*, const float*, const double*, int64_t, int);
+template void launch_cg_x<double>(hipStream_t, double*, const double*, const double*, int64_t, int);
 template int launch_cg_update<float>(hipStream_t, float*, const float*, float*, const float*, const double*,
                                      int64_t, int, double*);
 template int launch_cg_update<double>(hipStream_t, double*, const double*, double*, const double*, const double*,
@@ -1700,12 +1733,17 @@ static void pair_solve(hipStream_t st, const DevPrecond<T>& p, const PairSched& 
 // stream restricts, solves and sums its own pairs' subdomains; the combine
 // waits for both.  Every row is computed by the same kernel code as in
 // launch_precond and the pairs touch disjoint local rows, so the result is
-// bitwise that of launch_precond.  The lead's first stage is enqueued before
-// the other pairs' work so that its chain starts at once.
+// bitwise that of launch_precond.  The other pairs start once the lead's
+// bandwidth-bound first stage (restriction, Schur right-hand side) is done,
+// so that they run beside its level chain rather than slowing that stage.
+// `side` (may be null) is more work for st2 after those pairs, still before
+// the combine: independent work of the caller that can fill the chain's idle
+// bandwidth.
 template <typename T>
 void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
                             const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
-                            T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m) {
+                            T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m,
+                            const SideWork* side) {
   auto restrict_pair = [&](hipStream_t s, const PairSched& ps, T* part) {
     run_rows<RestrictSegKind<T>, T>(s, RestrictSegKind<T>{R, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp},
                                     ps.s_loc, part, kp);
@@ -1714,16 +1752,17 @@ void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hi
     run_rows<SegKind<T>, T>(s, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, ps.s_loc, part, kp);
   };
   const PairSched& a = pairs[lead];
-  (void)hipEventRecord(fork, st);
-  (void)hipStreamWaitEvent(st2, fork, 0);
   restrict_pair(st, a, w.part);
   pair_schur_rhs<T>(st, p, a, w, w.part, kp);
+  (void)hipEventRecord(fork, st);
+  (void)hipStreamWaitEvent(st2, fork, 0);
   for (size_t i = 0; i < pairs.size(); ++i) {
     if (i == lead) continue;
     restrict_pair(st2, pairs[i], part2);
     pair_solve<T>(st2, p, pairs[i], w, part2, Zc2, P2, kp);
     centre_pair(st2, pairs[i], part2);
   }
+  if (side) side->run(st2, side->ctx);
   (void)hipEventRecord(join, st2);
   pair_reduced<T>(st, p, a, w, w.part, w.Zc, w.P, kp);
   pair_elim<T>(st, p, a, w, w.part, kp);
@@ -1734,11 +1773,11 @@ void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hi
 template void launch_precond_overlap<float>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
                                             const DevPrecond<float>&, const std::vector<PairSched>&, size_t,
                                             const float*, float*, PrecondWork<float>&, float*, float*, float*, int,
-                                            int64_t);
+                                            int64_t, const SideWork*);
 template void launch_precond_overlap<double>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
                                              const DevPrecond<double>&, const std::vector<PairSched>&, size_t,
                                              const double*, double*, PrecondWork<double>&, double*, double*, double*,
-                                             int, int64_t);
+                                             int, int64_t, const SideWork*);
 
 // prolongation of one pair (its subdomains cover distinct DOFs):
 //   d = scale_l (S[l] - mu3); Z[g_l] += d; delta[g_l] = d

@@ -183,11 +183,19 @@ template <typename T>
 void launch_precond_mult(hipStream_t st, const DevPrecond<T>& p, const std::vector<PairSched>& pairs,
                          const DevCsr<T>& g, const T* R, T* Z, PrecondWork<T>& w, T* rwork, T* delta, T* gd,
                          const double* ones, int kp, int64_t m);
+// work enqueued on a given stream by a callback (side work of an apply)
+struct SideWork {
+  void (*run)(hipStream_t, void*) = nullptr;
+  void* ctx = nullptr;
+};
 // the additive apply with the pairs' local solves split over two streams
 template <typename T>
 void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
                             const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
-                            T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m);
+                            T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m,
+                            const SideWork* side);
+template <typename T>
+void launch_cg_x(hipStream_t st, T* X, const T* P, const double* alpha, int64_t m, int kp);
 template <typename T>
 void launch_axpy_col(hipStream_t st, T* Y, const T* X, const double* coef, double sign, int64_t m, int kp);
 template <typename T>

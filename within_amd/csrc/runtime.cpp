@@ -247,8 +247,8 @@ struct wa_solver {
   void setup_device();
   void setup_overlap();
   template <typename T> void op_apply(const T* X, T* Y, int kp);
-  template <typename T> void pre_apply(const T* R, T* Z, int kp);
-  template <typename T> void precondition(const T* R, T* Z, int kp);
+  template <typename T> void pre_apply(const T* R, T* Z, int kp, const SideWork* side = nullptr);
+  template <typename T> void precondition(const T* R, T* Z, int kp, const SideWork* side = nullptr);
   void rhs64(const double* V, double* out, int kp);
   template <typename T>
   void cg(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
@@ -689,7 +689,7 @@ void wa_solver::op_apply(const T* X, T* Y, int kp) {
 }
 
 template <typename T>
-void wa_solver::pre_apply(const T* R, T* Z, int kp) {
+void wa_solver::pre_apply(const T* R, T* Z, int kp, const SideWork* side) {
   hipEvent_t a = nullptr;
   tic(1, a);
   if (params.preconditioner == 2) {
@@ -702,15 +702,17 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp) {
   } else if (overlap) {
     if constexpr (std::is_same<T, double>::value)
       launch_precond_overlap<double>(st, st2, ev_fork, ev_join, pre64, pairs, lead, R, Z, w64, part2, zc2_64, p2_64,
-                                     kp, m);
+                                     kp, m, side);
     else
       launch_precond_overlap<float>(st, st2, ev_fork, ev_join, pre32, pairs, lead, R, Z, w32, (float*)part2, zc2_32,
-                                    p2_32, kp, m);
+                                    p2_32, kp, m, side);
+    side = nullptr;  // done on st2
   } else if constexpr (std::is_same<T, double>::value) {
     launch_precond<double>(st, pre64, R, Z, w64, kp, m);
   } else {
     launch_precond<float>(st, pre32, R, Z, w32, kp, m);
   }
+  if (side) side->run(st, side->ctx);
   toc(1, a);
 }
 
@@ -719,11 +721,14 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp) {
 // reference's additive and multiplicative applies do
 // (schwarz/additive/preconditioner.rs:164-214, multiplicative.rs:128-245).
 template <typename T>
-void wa_solver::precondition(const T* R, T* Z, int kp) {
+void wa_solver::precondition(const T* R, T* Z, int kp, const SideWork* side) {
   const int64_t bytes = sizeof(T) * m * kp;
   if (params.preconditioner == 0) HIPCHK(hipMemcpyAsync(Z, R, bytes, hipMemcpyDeviceToDevice, st));
-  else if (has_pre) pre_apply<T>(R, Z, kp);
-  else HIPCHK(hipMemsetAsync(Z, 0, bytes, st));
+  else if (has_pre) {
+    pre_apply<T>(R, Z, kp, side);
+    side = nullptr;
+  } else HIPCHK(hipMemsetAsync(Z, 0, bytes, st));
+  if (side) side->run(st, side->ctx);
 }
 
 void wa_solver::rhs64(const double* V, double* out, int kp) {
@@ -766,6 +771,22 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
   launch_dot<T>(st, rhs, rhs, m, kp, redp, bn2);
   launch_cg_start(st, cs, bn2, rz, tolv, k, kp);
   toc(2, ev);
+  // with the two-stream additive apply, x += alpha p (not needed until the
+  // end, p unchanged until the next direction update) runs on the second
+  // stream beside the level chain instead of in the residual update
+  const bool side_x = overlap && params.preconditioner == 1 && has_pre;
+  struct XUpd {
+    T* x;
+    const T* p;
+    const double* alpha;
+    int64_t m;
+    int kp;
+  } xu{x, p, alpha, m, kp};
+  const SideWork xside{[](hipStream_t s2, void* c) {
+                         const XUpd* u = static_cast<const XUpd*>(c);
+                         launch_cg_x<T>(s2, u->x, u->p, u->alpha, u->m, u->kp);
+                       },
+                       &xu};
   int64_t itn = 0;
   bool any = true;
   {
@@ -782,10 +803,10 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
     tic(2, ev);
     int nb = launch_dot_part<T>(st, p, ap, m, kp, redp);
     launch_cg_alpha(st, cs, redp, nb, pap, alpha, (int)itn, kp);
-    nb = launch_cg_update<T>(st, x, p, r, ap, alpha, m, kp, redp);
+    nb = launch_cg_update<T>(st, side_x ? nullptr : x, p, r, ap, alpha, m, kp, redp);
     launch_cg_check(st, cs, redp, nb, rr, (int)itn, kp);
     toc(2, ev);
-    precondition<T>(r, z, kp);
+    precondition<T>(r, z, kp, side_x ? &xside : nullptr);
     tic(2, ev);
     nb = launch_dot_part<T>(st, r, z, m, kp, redp);
     launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, (int)itn, kp);
