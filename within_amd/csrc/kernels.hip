@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "engine.hpp"
 #include "kernels.hpp"
 
@@ -1677,54 +1679,74 @@ namespace wa {
 // rows, in three stages: the centred Schur right-hand side, the reduced solve
 // (forward levels, dense cores, backward levels) and the back-substitution of
 // the eliminated block.  `part`, `Zc` and `P` are the split-row partials and
-// dense-core scratch of the stream the stage runs on.
-template <typename T>
-static void pair_schur_rhs(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
-                           int kp) {
+// dense-core scratch of the stream the stage runs on.  Each launch (or
+// dependent group of launches) is handed to `emit`, which runs it at once or
+// queues it for interleaved submission.
+struct RunNow {
+  template <class F>
+  void operator()(F&& f) const { f(); }
+};
+
+template <typename T, class Emit>
+static void pair_schur_rhs(Emit& emit, hipStream_t st, const DevPrecond<T>& p, const PairSched& ps,
+                           PrecondWork<T>& w, T* part, int kp) {
   if (ps.red1 == ps.red0) return;
-  run_rows<KeepKind<T>, T>(st,
-                           KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
-                                       p.red_sub, p.loc_ptr, w.sum1, kp},
-                           ps.s_keep, part, kp);
-  run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, part, kp);
-  launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp);
+  emit([=, &p, &ps, &w] {
+    run_rows<KeepKind<T>, T>(st,
+                             KeepKind<T>{w.B, w.Rr, p.keep_ptr, p.keep_idx, p.keep_val, p.keep_rowsum, p.red_loc,
+                                         p.red_sub, p.loc_ptr, w.sum1, kp},
+                             ps.s_keep, part, kp);
+  });
+  emit([=, &p, &ps, &w] { run_rows<SegKind<T>, T>(st, SegKind<T>{w.Rr, w.sum2, p.red_ptr, kp}, ps.s_red, part, kp); });
+  emit([=, &p, &ps, &w] { launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp); });
 }
 
-template <typename T>
-static void pair_reduced(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
-                         T* Zc, T* P, int kp) {
+template <typename T, class Emit>
+static void pair_reduced(Emit& emit, hipStream_t st, const DevPrecond<T>& p, const PairSched& ps,
+                         PrecondWork<T>& w, T* part, T* Zc, T* P, int kp) {
   if (ps.red1 == ps.red0) return;
-  FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
-  run_levels<FwdKind<T>, T>(st, fk, ps.s_fwd, 0, (int)ps.s_fwd.segs.size(), 1, part, kp);
-  if (ps.core1 > ps.core0) {
-    run_rows<CoreKind<T>, T>(st,
-                             CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
-                                         p.core_mat_off, p.core_mat, kp},
-                             ps.s_core, part, kp);
-    for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, Zc, P, kp);
-    launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
+  const FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
+  for (size_t t = 0; t < ps.s_fwd.segs.size(); ++t) {
+    const Segment& g = ps.s_fwd.segs[t];
+    if (g.r1 > g.r0 || g.i1 > g.i0) emit([=, &ps] { run_seg<FwdKind<T>, T>(st, fk, ps.s_fwd, g, part, kp); });
   }
-  BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
+  if (ps.core1 > ps.core0) {
+    emit([=, &p, &ps, &w] {
+      run_rows<CoreKind<T>, T>(st,
+                               CoreKind<T>{w.Rr, w.C, p.core_row_ptr, p.core_row_core, p.core_ptr, p.core_idx,
+                                           p.core_mat_off, p.core_mat, kp},
+                               ps.s_core, part, kp);
+      for (size_t bi : ps.big) run_dense_core<T>(st, p.big[bi], p.core_idx, w.Rr, w.C, Zc, P, kp);
+      launch_scatter_rows<T>(st, ps.core0, ps.core1, p.core_idx, w.C, w.Rr, kp);
+    });
+  }
+  const BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
   // the last level gathers the dense cores: no backward step
-  run_levels<BwdKind<T>, T>(st, bk, ps.s_bwd, 0, (int)ps.s_bwd.segs.size() - 1, -1, part, kp);
-  launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp);
+  for (size_t step = 0; step + 1 < ps.s_bwd.segs.size(); ++step) {
+    const Segment& g = ps.s_bwd.segs[ps.s_bwd.segs.size() - 2 - step];
+    if (g.r1 > g.r0 || g.i1 > g.i0) emit([=, &ps] { run_seg<BwdKind<T>, T>(st, bk, ps.s_bwd, g, part, kp); });
+  }
+  emit([=, &p, &ps, &w] { launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp); });
 }
 
-template <typename T>
-static void pair_elim(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
-                      int kp) {
-  run_rows<ElimKind<T>, T>(st,
-                           ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
-                                       p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
-                           ps.s_elim, part, kp);
+template <typename T, class Emit>
+static void pair_elim(Emit& emit, hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w,
+                      T* part, int kp) {
+  emit([=, &p, &ps, &w] {
+    run_rows<ElimKind<T>, T>(st,
+                             ElimKind<T>{w.Rr, w.S, p.elim_ptr, p.elim_idx, p.elim_val, p.elim_inv, p.elim_loc,
+                                         p.loc_sub, p.loc_ptr, w.B, w.sum1, kp},
+                             ps.s_elim, part, kp);
+  });
 }
 
 template <typename T>
 static void pair_solve(hipStream_t st, const DevPrecond<T>& p, const PairSched& ps, PrecondWork<T>& w, T* part,
                        T* Zc, T* P, int kp) {
-  pair_schur_rhs<T>(st, p, ps, w, part, kp);
-  pair_reduced<T>(st, p, ps, w, part, Zc, P, kp);
-  pair_elim<T>(st, p, ps, w, part, kp);
+  RunNow now;
+  pair_schur_rhs<T>(now, st, p, ps, w, part, kp);
+  pair_reduced<T>(now, st, p, ps, w, part, Zc, P, kp);
+  pair_elim<T>(now, st, p, ps, w, part, kp);
 }
 
 // Additive Schwarz with the factor pairs on two streams: the pair with the
@@ -1738,12 +1760,17 @@ static void pair_solve(hipStream_t st, const DevPrecond<T>& p, const PairSched& 
 // so that they run beside its level chain rather than slowing that stage.
 // `side` (may be null) is more work for st2 after those pairs, still before
 // the combine: independent work of the caller that can fill the chain's idle
-// bandwidth.
+// bandwidth.  The two streams' launches are submitted interleaved, in
+// proportion to their counts, so that neither chain waits for the host to
+// finish enqueueing the other.
 template <typename T>
 void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
                             const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
                             T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m,
                             const SideWork* side) {
+  std::vector<std::function<void()>> qa, qb;
+  auto to_a = [&](auto&& f) { qa.emplace_back(std::forward<decltype(f)>(f)); };
+  auto to_b = [&](auto&& f) { qb.emplace_back(std::forward<decltype(f)>(f)); };
   auto restrict_pair = [&](hipStream_t s, const PairSched& ps, T* part) {
     run_rows<RestrictSegKind<T>, T>(s, RestrictSegKind<T>{R, w.B, w.sum1, p.loc_ptr, p.loc_gidx, p.loc_scale, kp},
                                     ps.s_loc, part, kp);
@@ -1752,21 +1779,34 @@ void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hi
     run_rows<SegKind<T>, T>(s, SegKind<T>{w.S, w.sum3, p.loc_ptr, kp}, ps.s_loc, part, kp);
   };
   const PairSched& a = pairs[lead];
+  RunNow now;
   restrict_pair(st, a, w.part);
-  pair_schur_rhs<T>(st, p, a, w, w.part, kp);
+  pair_schur_rhs<T>(now, st, p, a, w, w.part, kp);
   (void)hipEventRecord(fork, st);
   (void)hipStreamWaitEvent(st2, fork, 0);
+  pair_reduced<T>(to_a, st, p, a, w, w.part, w.Zc, w.P, kp);
+  pair_elim<T>(to_a, st, p, a, w, w.part, kp);
+  to_a([&] { centre_pair(st, a, w.part); });
   for (size_t i = 0; i < pairs.size(); ++i) {
     if (i == lead) continue;
-    restrict_pair(st2, pairs[i], part2);
-    pair_solve<T>(st2, p, pairs[i], w, part2, Zc2, P2, kp);
-    centre_pair(st2, pairs[i], part2);
+    const PairSched& b = pairs[i];
+    to_b([&, i] { restrict_pair(st2, pairs[i], part2); });
+    pair_schur_rhs<T>(to_b, st2, p, b, w, part2, kp);
+    pair_reduced<T>(to_b, st2, p, b, w, part2, Zc2, P2, kp);
+    pair_elim<T>(to_b, st2, p, b, w, part2, kp);
+    to_b([&, i] { centre_pair(st2, pairs[i], part2); });
   }
-  if (side) side->run(st2, side->ctx);
-  (void)hipEventRecord(join, st2);
-  pair_reduced<T>(st, p, a, w, w.part, w.Zc, w.P, kp);
-  pair_elim<T>(st, p, a, w, w.part, kp);
-  centre_pair(st, a, w.part);
+  if (side) to_b([&] { side->run(st2, side->ctx); });
+  const size_t per = std::max<size_t>(1, qa.size() / std::max<size_t>(1, qb.size()));
+  size_t ia = 0, ib = 0;
+  while (ia < qa.size() || ib < qb.size()) {
+    for (size_t u = 0; u < per && ia < qa.size(); ++u) qa[ia++]();
+    if (ib < qb.size()) {
+      qb[ib++]();
+      if (ib == qb.size()) (void)hipEventRecord(join, st2);
+    }
+  }
+  if (qb.empty()) (void)hipEventRecord(join, st2);
   (void)hipStreamWaitEvent(st, join, 0);
   launch_combine<T>(st, m, p.dof_ptr, p.dof_loc, p.loc_scale, p.loc_sub, p.loc_ptr, w.S, w.sum3, Z, kp);
 }
