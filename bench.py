@@ -135,7 +135,8 @@ def main():
                     help="only copy a 1 GiB FP32 buffer 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the solver runs on one GPU (the RHS-column sharding across GPUs is not "
+                                   "implemented); --gpus must be 1"}))
         sys.exit(2)
 
     import torch
