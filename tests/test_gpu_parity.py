@@ -323,3 +323,20 @@ def test_two_stream_apply_is_bitwise_the_one_stream_apply(oracle_mod, monkeypatc
     a, b = one.solve_batch(Y), two.solve_batch(Y)
     assert all(a.converged) and list(a.iterations) == list(b.iterations)
     assert np.array_equal(a.demeaned, b.demeaned) and np.array_equal(a.x, b.x)
+
+
+@pytest.mark.parametrize("config", [W.CG(tol=1e-10, max_refinements=3), W.GMRES(tol=1e-9),
+                                    W.GMRES(tol=1e-9, precision="mixed"), W.LSMR(tol=1e-9)],
+                         ids=["cg-fp64", "gmres-fp64", "gmres-mixed", "lsmr"])
+def test_two_stream_solves_are_bitwise_the_one_stream_solves(config, monkeypatch):
+    """Every Krylov method that applies the additive preconditioner gives the
+    same bits with the two-stream apply (the x update of CG on the second
+    stream included)."""
+    two = W.Solver(BIG, config=config, preconditioner=RUST_DEFAULT, _dense_core=64)
+    monkeypatch.setenv("WITHIN_AMD_PRECOND_STREAMS", "1")
+    one = W.Solver(BIG, config=config, preconditioner=RUST_DEFAULT, _dense_core=64)
+    assert two.info()[15] == 2 and one.info()[15] == 1
+    Y = np.asfortranarray(np.random.default_rng(41).standard_normal((BIG.shape[0], 5)))
+    a, b = one.solve_batch(Y), two.solve_batch(Y)
+    assert all(a.converged) and list(a.iterations) == list(b.iterations)
+    assert np.array_equal(a.demeaned, b.demeaned) and np.array_equal(a.x, b.x)
