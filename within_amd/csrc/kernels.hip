@@ -1475,6 +1475,62 @@ __global__ void __launch_bounds__(kBlock) k_core_tiled(const T* __restrict__ M, 
   }
 }
 
+// FP32, 32 columns: the same tiles and staging on the matrix cores.  Each
+// wave owns 32 of the tile's 128 rows and all 32 columns, accumulated by
+// v_mfma_f32_32x32x2_f32 over pairs of k.  Operand layout (checked on gfx950):
+// lane l supplies A[row l % 32][k l / 32] and B[k l / 32][column l % 32];
+// result register r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+__global__ void __launch_bounds__(kBlock) k_core_mfma(const float* __restrict__ M, const float* __restrict__ Zc,
+                                                      float* __restrict__ P, uint32_t n, uint32_t ld, uint32_t ks) {
+  constexpr int KP = 32, NV = 4, TK = 32, TR = kCoreRows, VPR = TK / NV, MV = TR * VPR / kBlock;
+  static_assert(TR == 4 * 32 && kBlock == 256, "four waves of 32 rows");
+  __shared__ float Ms[TK][TR + NV];
+  __shared__ __align__(16) float Zs[TK * KP];
+  const int tid = threadIdx.x, wv = tid / kWave, lane = tid & (kWave - 1);
+  const uint32_t r0 = blockIdx.x * TR;
+  const uint32_t ka = blockIdx.y * ks, kb = min(ld, ka + ks);
+  f32x16 acc = {};
+  VecT<float, NV> mr[MV];
+  auto load_m = [&](uint32_t k0) {
+#pragma unroll
+    for (int u = 0; u < MV; ++u) {
+      const int v = tid + u * kBlock;
+      const int row = v / VPR, kv = (v % VPR) * NV;
+      if (r0 + row < n) {
+        mr[u] = vload<float, NV>(M + (size_t)(r0 + row) * ld + k0 + kv);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) mr[u].v[c] = 0;
+      }
+    }
+  };
+  if (ka < kb) load_m(ka);
+  const int ar = wv * 32 + lane % 32, hk = lane / 32, bc = lane % 32;
+  for (uint32_t k0 = ka; k0 < kb; k0 += TK) {
+#pragma unroll
+    for (int u = 0; u < MV; ++u) {
+      const int v = tid + u * kBlock;
+      const int row = v / VPR, kv = (v % VPR) * NV;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) Ms[kv + c][row] = mr[u].v[c];
+    }
+    for (int v = tid; v < TK * KP / NV; v += kBlock)
+      *reinterpret_cast<VecT<float, NV>*>(&Zs[v * NV]) = vload<float, NV>(Zc + (size_t)k0 * KP + v * NV);
+    __syncthreads();
+    if (k0 + TK < kb) load_m(k0 + TK);
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 2)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Ms[kk + hk][ar], Zs[(kk + hk) * KP + bc], acc, 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const uint32_t row = r0 + wv * 32 + 8 * (r / 4) + 4 * hk + (r % 4);
+    if (row < n) P[((size_t)blockIdx.y * ld + row) * KP + bc] = acc[r];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_core_sum(const T* __restrict__ P, T* __restrict__ C, uint32_t n,
                                                      uint32_t ld, int split, int kp) {
@@ -1555,7 +1611,12 @@ static void run_dense_core(hipStream_t st, const DenseCore<T>& d, const uint32_t
     switch (kp) {
       case 8: k_core_tiled<T, 8><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
       case 16: k_core_tiled<T, 16><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
-      default: k_core_tiled<T, 32><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks); break;
+      default:
+        if constexpr (std::is_same<T, float>::value)
+          k_core_mfma<<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks);
+        else
+          k_core_tiled<T, 32><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks);
+        break;
     }
     k_core_sum<T><<<grid_stride_blocks((int64_t)d.n * kp, 1), kBlock, 0, st>>>(P, out, d.n, d.ld, nsplit, kp);
   } else {
