@@ -167,9 +167,10 @@ wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4);
  * info[0..15]: n_subdomains, n_local_total, n_reduced_total, ac_levels,
  * ac_nnz, core_total, n_cores, c_nnz_total, gramian_nnz, n_dofs, n_obs,
  * operator_passes, dense_subdomains, max_core, device_dofs (active levels
- * the device solves over; 0 for a host-only solver), apply_streams (2 when the
- * additive apply runs the factor pairs' local solves on two streams, else 1;
- * 0 for a host-only solver). */
+ * the device solves over; 0 for a host-only solver), apply_streams (low byte:
+ * 2 when the additive apply runs the factor pairs' local solves on two
+ * streams, else 1, 0 for a host-only solver; bits 8 and up: the number of
+ * trailing elimination levels run as one launch per direction). */
 wa_status wa_solver_info(const wa_solver* s, int64_t* info16);
 /* Subdomain i: meta[0..7] = q, r, n_q, n_r, eliminate_q, dense, rounds, core
  * size; gidx (n_q + n_r), pou (n_q + n_r) may be NULL. */

@@ -308,7 +308,7 @@ def test_two_stream_apply_is_bitwise_the_one_stream_apply(oracle_mod, monkeypatc
     two = W.Solver(BIG, config=cfg, preconditioner=RUST_DEFAULT, _dense_core=64)
     monkeypatch.setenv("WITHIN_AMD_PRECOND_STREAMS", "1")
     one = W.Solver(BIG, config=cfg, preconditioner=RUST_DEFAULT, _dense_core=64)
-    assert two.info()[15] == 2 and one.info()[15] == 1
+    assert two.info()[15] & 255 == 2 and one.info()[15] == 1
     m = two.n_dofs
     rng = np.random.default_rng(31)
     for k in (1, 4, 32):
@@ -335,8 +335,44 @@ def test_two_stream_solves_are_bitwise_the_one_stream_solves(config, monkeypatch
     two = W.Solver(BIG, config=config, preconditioner=RUST_DEFAULT, _dense_core=64)
     monkeypatch.setenv("WITHIN_AMD_PRECOND_STREAMS", "1")
     one = W.Solver(BIG, config=config, preconditioner=RUST_DEFAULT, _dense_core=64)
-    assert two.info()[15] == 2 and one.info()[15] == 1
+    assert two.info()[15] & 255 == 2 and one.info()[15] == 1
     Y = np.asfortranarray(np.random.default_rng(41).standard_normal((BIG.shape[0], 5)))
     a, b = one.solve_batch(Y), two.solve_batch(Y)
     assert all(a.converged) and list(a.iterations) == list(b.iterations)
     assert np.array_equal(a.demeaned, b.demeaned) and np.array_equal(a.x, b.x)
+
+
+@pytest.mark.skip(reason="level runs are switched off: both GPU runs of this test stalled in the first run's "
+                         "waits for the previous level's count; the cause is not found")
+def test_level_runs_are_bitwise_the_per_level_launches(oracle_mod, monkeypatch):
+    """The lead pair's trailing elimination levels as one launch per direction
+    (waves ordered by tickets, one XCD) give the same bits as one launch per
+    level, over many repeated applies (a missed wait would show up as a
+    changed or varying result), and the oracle's apply to 1e-10."""
+    flow = W.Solver(BIG, config=W.CG(tol=1e-10, max_refinements=3, precision="mixed"), preconditioner=RUST_DEFAULT,
+                    _dense_core=64)
+    monkeypatch.setenv("WITHIN_AMD_LEVEL_FLOW", "0")
+    plain = W.Solver(BIG, config=W.CG(tol=1e-10, max_refinements=3, precision="mixed"), preconditioner=RUST_DEFAULT,
+                     _dense_core=64)
+    assert flow.info()[15] & 255 == 2 and flow.info()[15] >> 8 >= 4
+    assert plain.info()[15] == 2
+    o = oracle_mod.OracleSolver(BIG, None, ac_split=1, dense_core=64)
+    m = flow.n_dofs
+    rng = np.random.default_rng(51)
+    L = N.load()
+    for k in (1, 4, 32):
+        R = np.asfortranarray(rng.standard_normal((m, k)))
+        Zp = np.zeros((m, k), order="F")
+        N.check(L.wa_precond_apply(plain._h, N.ptr(R), N.ptr(Zp), k, m))
+        for rep in range(10):
+            Zf = np.zeros((m, k), order="F")
+            N.check(L.wa_precond_apply(flow._h, N.ptr(R), N.ptr(Zf), k, m))
+            assert np.array_equal(Zf, Zp), (k, rep)
+        ref = np.column_stack([o.precond_apply(R[:, j]) for j in range(k)])
+        assert rel(Zf, ref) <= 1e-10
+    Y = np.asfortranarray(rng.standard_normal((BIG.shape[0], 32)))
+    a, b = plain.solve_batch(Y), flow.solve_batch(Y)
+    assert all(a.converged) and list(a.iterations) == list(b.iterations)
+    assert np.array_equal(a.demeaned, b.demeaned) and np.array_equal(a.x, b.x)
+    c = flow.solve_batch(Y)
+    assert np.array_equal(c.x, b.x)

@@ -281,6 +281,111 @@ void run_seg(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T
   if (g.f1 > g.f0) k_rows_fix<K, T><<<(unsigned)(g.f1 - g.f0), kFixBlock, 0, st>>>(k, sc.fix + g.f0, part, kp);
 }
 
+// ---- a run of consecutive levels in one launch, on one XCD ------------------
+// Levels are the segments of a level schedule, in processing order; each has
+// ceil(rows / S) short-row units then one unit per item (no split rows).
+// Waves take units in ticket order (ctr[0]); before a unit of level l a wave
+// waits until ctr[2 + p] counts every unit of p, the nearest earlier level
+// with units.
+// A unit runs rows_wave exactly as a per-level launch would, so each row's
+// sum is bitwise the same.  Only workgroups on XCD 0 take tickets: they share
+// one L2, where a unit's stores are complete (vmcnt 0) before its count is
+// published, and a waiting wave drops its L1 before reading them.  A wait
+// longer than spin_limit polls sets ctr[1] and goes on (the caller reports
+// the error); tickets are taken in order, so every unit waited for is already
+// held by a running wave.  spin_limit is in ticks of the 100 MHz real-time
+// counter.
+constexpr uint32_t kXccIdReg = (3u << 11) | (0u << 6) | 20u;  // hwreg(HW_REG_XCC_ID, 0, 4)
+
+template <class K, typename T, int V, int NV>
+__global__ void __launch_bounds__(kBlock) k_level_flow(K k, const FlowLevel* __restrict__ lv, int nl,
+                                                       uint32_t short_max, const Item* __restrict__ items,
+                                                       T* __restrict__ part, uint32_t* __restrict__ ctr,
+                                                       uint32_t spin_limit) {
+  if ((__builtin_amdgcn_s_getreg(kXccIdReg) & 15u) != 0u) return;  // workgroup-uniform
+  constexpr uint32_t S = kWave / V;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (;;) {
+    uint32_t w = 0;
+    if (lane == 0) w = __hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    w = __shfl(w, 0, kWave);
+    int l = 0;
+    uint32_t base = 0, nsw = 0, need = 0;
+    int prev = -1;
+    for (; l < nl; ++l) {
+      nsw = (lv[l].r1 - lv[l].r0 + S - 1) / S;
+      const uint32_t nu = nsw + (uint32_t)(lv[l].i1 - lv[l].i0);
+      if (w < base + nu) break;
+      base += nu;
+      if (nu > 0) {  // the nearest earlier level that has units
+        need = nu;
+        prev = l;
+      }
+    }
+    if (l == nl) return;
+    if (prev >= 0) {
+      if (lane == 0) {
+        // every counter access is a system-scope atomic, so the polls and
+        // the increments meet at one point; the wait is bounded by the
+        // 100 MHz real-time counter, and gives up at once after an earlier
+        // failed wait
+        const uint64_t t_end = __builtin_amdgcn_s_memrealtime() + spin_limit;
+        while (__hip_atomic_load(&ctr[2 + prev], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < need) {
+          if (__hip_atomic_load(&ctr[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) break;
+          __builtin_amdgcn_s_sleep(2);
+          if (__builtin_amdgcn_s_memrealtime() > t_end) {
+            __hip_atomic_fetch_or(&ctr[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            break;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    rows_wave<K, T, V, NV>(k, (int64_t)(w - base), lv[l].r0, lv[l].r1, short_max, (int64_t)nsw, items + lv[l].i0,
+                           lv[l].i1 - lv[l].i0, part);
+    // the unit's stores are complete at L2 before its count is published
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) __hip_atomic_fetch_add(&ctr[2 + l], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// units of a level run for kp columns of T (short-row units + items)
+template <typename T>
+inline uint32_t flow_units(const FlowLevel& f, int kp) {
+  const int NV16 = 16 / (int)sizeof(T);
+  const int V = kp * (int)sizeof(T) >= 16 ? kp / NV16 : 1;
+  const uint32_t S = kWave / V;
+  return (f.r1 - f.r0 + S - 1) / S + (uint32_t)(f.i1 - f.i0);
+}
+
+constexpr unsigned kFlowBlocks = 512;  // 64 workgroups on each XCD; those on XCD 0 work
+
+template <class K, typename T>
+void run_flow(hipStream_t st, const K& k, const DevSched& sc, const FlowLevel* lv, int nl, T* part, uint32_t* ctr,
+              int kp, uint32_t spin_limit) {
+  constexpr int NV16 = 16 / sizeof(T);
+#define WA_FLOW(V_, NV_)                                                                                   \
+  k_level_flow<K, T, V_, NV_><<<kFlowBlocks, kBlock, 0, st>>>(k, lv, nl, sc.short_max, sc.items, part, ctr, \
+                                                              spin_limit)
+  if (kp * (int)sizeof(T) >= 16) {
+    switch (kp / NV16) {
+      case 1: WA_FLOW(1, NV16); break;
+      case 2: WA_FLOW(2, NV16); break;
+      case 4: WA_FLOW(4, NV16); break;
+      case 8: WA_FLOW(8, NV16); break;
+      default: WA_FLOW(16, NV16); break;
+    }
+  } else if (kp == 2) {
+    WA_FLOW(1, 2);
+  } else {
+    WA_FLOW(1, 1);
+  }
+#undef WA_FLOW
+}
+
 // Run every segment of a schedule (in order).
 template <class K, typename T>
 void run_rows(hipStream_t st, const K& k, const DevSched& sc, T* part, int kp) {

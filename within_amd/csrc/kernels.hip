@@ -1762,12 +1762,54 @@ static void pair_schur_rhs(Emit& emit, hipStream_t st, const DevPrecond<T>& p, c
   emit([=, &p, &ps, &w] { launch_sub_mean<T>(st, ps.red0, ps.red1, p.red_sub, p.red_ptr, w.sum2, w.Rr, kp); });
 }
 
+// clears the level-run counters of one apply; first checks that the previous
+// apply's runs handed out every unit (a run whose workgroups all missed XCD 0
+// would leave its rows untouched)
+// (system-scope atomics throughout, like the runs that use the counters)
+__global__ void k_flow_reset(uint32_t* ctr_f, int nf, uint32_t* ctr_b, int nb, uint32_t prev_f, uint32_t prev_b,
+                             int check) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  auto get = [](uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
+  auto put = [](uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
+  if (check && (get(&ctr_f[0]) < prev_f || get(&ctr_b[0]) < prev_b))
+    __hip_atomic_fetch_or(&ctr_f[1], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  put(&ctr_f[0], 0);
+  for (int l = 0; l < nf; ++l) put(&ctr_f[2 + l], 0);
+  put(&ctr_b[0], 0);
+  for (int l = 0; l < nb; ++l) put(&ctr_b[2 + l], 0);
+}
+
+template <typename T>
+void flow_totals(const PairSched& ps, int kp, uint32_t& tot_f, uint32_t& tot_b) {
+  tot_f = tot_b = 0;
+  for (int t = ps.flow_t0; t < ps.flow_t1; ++t) {
+    const Segment& gf = ps.s_fwd.segs[t];
+    const Segment& gb = ps.s_bwd.segs[t];
+    tot_f += flow_units<T>(FlowLevel{gf.r0, gf.r1, gf.i0, gf.i1}, kp);
+    tot_b += flow_units<T>(FlowLevel{gb.r0, gb.r1, gb.i0, gb.i1}, kp);
+  }
+}
+template void flow_totals<float>(const PairSched&, int, uint32_t&, uint32_t&);
+template void flow_totals<double>(const PairSched&, int, uint32_t&, uint32_t&);
+
 template <typename T, class Emit>
 static void pair_reduced(Emit& emit, hipStream_t st, const DevPrecond<T>& p, const PairSched& ps,
-                         PrecondWork<T>& w, T* part, T* Zc, T* P, int kp) {
+                         PrecondWork<T>& w, T* part, T* Zc, T* P, int kp, const FlowCall* flow = nullptr) {
   if (ps.red1 == ps.red0) return;
+  const bool fl = flow && ps.flow_t0 >= 0;
+  const int nfl = fl ? ps.flow_t1 - ps.flow_t0 : 0;
   const FwdKind<T> fk{w.Rr, p.lvl_rows, p.fin_ptr, p.fin_idx, p.fin_val, kp};
   for (size_t t = 0; t < ps.s_fwd.segs.size(); ++t) {
+    if (fl && (int)t >= ps.flow_t0 && (int)t < ps.flow_t1) {
+      if ((int)t == ps.flow_t0) {
+        const FlowCall fc = *flow;
+        emit([=, &ps] {
+          k_flow_reset<<<1, 64, 0, st>>>(fc.ctr_f, nfl, fc.ctr_b, nfl, fc.prev_f, fc.prev_b, fc.check);
+          run_flow<FwdKind<T>, T>(st, fk, ps.s_fwd, ps.flow_fwd, nfl, part, fc.ctr_f, kp, fc.spin);
+        });
+      }
+      continue;
+    }
     const Segment& g = ps.s_fwd.segs[t];
     if (g.r1 > g.r0 || g.i1 > g.i0) emit([=, &ps] { run_seg<FwdKind<T>, T>(st, fk, ps.s_fwd, g, part, kp); });
   }
@@ -1784,7 +1826,15 @@ static void pair_reduced(Emit& emit, hipStream_t st, const DevPrecond<T>& p, con
   const BwdKind<T> bk{w.Rr, p.lvl_rows, p.bout_ptr, p.bout_idx, p.bout_val, p.inv_pivot, kp};
   // the last level gathers the dense cores: no backward step
   for (size_t step = 0; step + 1 < ps.s_bwd.segs.size(); ++step) {
-    const Segment& g = ps.s_bwd.segs[ps.s_bwd.segs.size() - 2 - step];
+    const int t = (int)ps.s_bwd.segs.size() - 2 - (int)step;
+    if (fl && t >= ps.flow_t0 && t < ps.flow_t1) {
+      if (t == ps.flow_t1 - 1) {
+        const FlowCall fc = *flow;
+        emit([=, &ps] { run_flow<BwdKind<T>, T>(st, bk, ps.s_bwd, ps.flow_bwd, nfl, part, fc.ctr_b, kp, fc.spin); });
+      }
+      continue;
+    }
+    const Segment& g = ps.s_bwd.segs[t];
     if (g.r1 > g.r0 || g.i1 > g.i0) emit([=, &ps] { run_seg<BwdKind<T>, T>(st, bk, ps.s_bwd, g, part, kp); });
   }
   emit([=, &p, &ps, &w] { launch_scatter_rows<T>(st, ps.red0, ps.red1, p.red_loc, w.Rr, w.S, kp); });
@@ -1828,7 +1878,7 @@ template <typename T>
 void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
                             const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
                             T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m,
-                            const SideWork* side) {
+                            const SideWork* side, const FlowCall* flow) {
   std::vector<std::function<void()>> qa, qb;
   auto to_a = [&](auto&& f) { qa.emplace_back(std::forward<decltype(f)>(f)); };
   auto to_b = [&](auto&& f) { qb.emplace_back(std::forward<decltype(f)>(f)); };
@@ -1845,7 +1895,7 @@ void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hi
   pair_schur_rhs<T>(now, st, p, a, w, w.part, kp);
   (void)hipEventRecord(fork, st);
   (void)hipStreamWaitEvent(st2, fork, 0);
-  pair_reduced<T>(to_a, st, p, a, w, w.part, w.Zc, w.P, kp);
+  pair_reduced<T>(to_a, st, p, a, w, w.part, w.Zc, w.P, kp, flow);
   pair_elim<T>(to_a, st, p, a, w, w.part, kp);
   to_a([&] { centre_pair(st, a, w.part); });
   for (size_t i = 0; i < pairs.size(); ++i) {
@@ -1874,11 +1924,11 @@ void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hi
 template void launch_precond_overlap<float>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
                                             const DevPrecond<float>&, const std::vector<PairSched>&, size_t,
                                             const float*, float*, PrecondWork<float>&, float*, float*, float*, int,
-                                            int64_t, const SideWork*);
+                                            int64_t, const SideWork*, const FlowCall*);
 template void launch_precond_overlap<double>(hipStream_t, hipStream_t, hipEvent_t, hipEvent_t,
                                              const DevPrecond<double>&, const std::vector<PairSched>&, size_t,
                                              const double*, double*, PrecondWork<double>&, double*, double*, double*,
-                                             int, int64_t, const SideWork*);
+                                             int, int64_t, const SideWork*, const FlowCall*);
 
 // prolongation of one pair (its subdomains cover distinct DOFs):
 //   d = scale_l (S[l] - mu3); Z[g_l] += d; delta[g_l] = d

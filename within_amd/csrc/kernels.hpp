@@ -79,11 +79,28 @@ struct DevPrecond {
   std::vector<DenseCore<T>> big;  // cores applied by the dense kernels (not in s_core)
 };
 
-// subranges of one factor pair, for the multiplicative sweep
+// one level of a level run: its rows and items (a segment of the schedule)
+struct FlowLevel {
+  uint32_t r0, r1;
+  int64_t i0, i1;
+};
+// subranges of one factor pair (multiplicative sweep, two-stream apply)
 struct PairSched {
   int64_t loc0 = 0, loc1 = 0, red0 = 0, red1 = 0, core0 = 0, core1 = 0;
   DevSched s_loc, s_red, s_keep, s_elim, s_fwd, s_bwd, s_core;
   std::vector<size_t> big;  // indices into DevPrecond::big
+  // levels [flow_t0, flow_t1) run as one launch per direction (flow_t0 < 0: none);
+  // flow_fwd / flow_bwd list their segments in processing order (device)
+  int flow_t0 = -1, flow_t1 = -1;
+  const FlowLevel *flow_fwd = nullptr, *flow_bwd = nullptr;
+};
+// one apply's use of the level runs: counters of the forward and backward
+// runs (error bits in ctr_f[1]); the previous apply's ticket totals, checked
+// when `check` is set before the counters are cleared
+struct FlowCall {
+  uint32_t *ctr_f = nullptr, *ctr_b = nullptr;
+  uint32_t spin = 0, prev_f = 0, prev_b = 0;
+  int check = 0;
 };
 
 // scratch for one preconditioner apply (kp columns)
@@ -193,7 +210,10 @@ template <typename T>
 void launch_precond_overlap(hipStream_t st, hipStream_t st2, hipEvent_t fork, hipEvent_t join,
                             const DevPrecond<T>& p, const std::vector<PairSched>& pairs, size_t lead, const T* R,
                             T* Z, PrecondWork<T>& w, T* part2, T* Zc2, T* P2, int kp, int64_t m,
-                            const SideWork* side);
+                            const SideWork* side, const FlowCall* flow);
+// ticket totals of the lead pair's level runs for kp columns of T
+template <typename T>
+void flow_totals(const PairSched& ps, int kp, uint32_t& tot_f, uint32_t& tot_b);
 template <typename T>
 void launch_cg_x(hipStream_t st, T* X, const T* P, const double* alpha, int64_t m, int kp);
 template <typename T>

@@ -156,6 +156,13 @@ struct wa_solver {
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* part2 = nullptr;                 // st2's split-row partials
+  // the lead pair's trailing levels as one launch per direction (engine.hpp
+  // k_level_flow): counters of the forward and backward runs, the previous
+  // apply's ticket totals
+  uint32_t* flow_ctr = nullptr;
+  int flow_nl = 0;
+  bool flow_used = false;
+  uint32_t flow_prev_f = 0, flow_prev_b = 0;
   double *zc2_64 = nullptr, *p2_64 = nullptr;  // st2's dense-core scratch
   float *zc2_32 = nullptr, *p2_32 = nullptr;
   double* mw64 = nullptr;                  // r_work, delta, G delta (multiplicative)
@@ -246,6 +253,9 @@ struct wa_solver {
 
   void setup_device();
   void setup_overlap();
+  void setup_flow();
+  void check_flow();
+  template <typename T> FlowCall flow_call(int kp);
   template <typename T> void op_apply(const T* X, T* Y, int kp);
   template <typename T> void pre_apply(const T* R, T* Z, int kp, const SideWork* side = nullptr);
   template <typename T> void precondition(const T* R, T* Z, int kp, const SideWork* side = nullptr);
@@ -600,6 +610,7 @@ void wa_solver::setup_device() {
     w32.sum3 = dalloc<float>(nsub * KMAX);
     has_pre = true;
     setup_overlap();
+    setup_flow();
   }
   phase("preconditioner");
   part64 = dalloc<double>(max_slots * KMAX);
@@ -668,6 +679,77 @@ void wa_solver::setup_overlap() {
   overlap = true;
 }
 
+// Trailing levels of the lead pair as one launch per direction: the longest
+// suffix of its level chain (the dense-core level excluded) whose levels have
+// at most kFlowRows rows and no rows split into chunks, in both directions,
+// when it has at least kFlowMin levels.  WITHIN_AMD_LEVEL_FLOW=0 keeps one
+// launch per level (bitwise the same results).
+// Switched off: both GPU runs of the bitwise test stalled in the first level
+// run (every wait for a previous level's count went unanswered, with the
+// counters first at agent, then at system scope); the cause is not found, so
+// no solver takes this path.
+constexpr bool kLevelRuns = false;
+void wa_solver::setup_flow() {
+  constexpr uint32_t kFlowRows = 4096;
+  constexpr int kFlowMin = 4;
+  const char* e = std::getenv("WITHIN_AMD_LEVEL_FLOW");
+  if (!kLevelRuns || !overlap || (e && std::atoi(e) == 0)) return;
+  PairSched& ps = pairs[lead];
+  const std::vector<Segment>& sf = ps.s_fwd.segs;
+  const std::vector<Segment>& sb = ps.s_bwd.segs;
+  if (sf.size() != sb.size() || sf.size() < 2) return;
+  const int t1 = (int)sf.size() - 1;
+  auto ok = [&](const Segment& g) { return g.r1 - g.r0 <= kFlowRows && g.f1 == g.f0; };
+  int t0 = t1;
+  while (t0 > 0 && ok(sf[t0 - 1]) && ok(sb[t0 - 1])) --t0;
+  if (t1 - t0 < kFlowMin) return;
+  std::vector<FlowLevel> hf, hb;
+  for (int t = t0; t < t1; ++t) hf.push_back({sf[t].r0, sf[t].r1, sf[t].i0, sf[t].i1});
+  for (int t = t1 - 1; t >= t0; --t) hb.push_back({sb[t].r0, sb[t].r1, sb[t].i0, sb[t].i1});
+  ps.flow_fwd = upload(hf);
+  ps.flow_bwd = upload(hb);
+  ps.flow_t0 = t0;
+  ps.flow_t1 = t1;
+  flow_nl = t1 - t0;
+  flow_ctr = dalloc<uint32_t>(2 * (2 + flow_nl));
+  HIPCHK(hipMemset(flow_ctr, 0, sizeof(uint32_t) * 2 * (2 + flow_nl)));
+}
+
+template <typename T>
+FlowCall wa_solver::flow_call(int kp) {
+  FlowCall fc;
+  if (!flow_ctr) return fc;
+  fc.ctr_f = flow_ctr;
+  fc.ctr_b = flow_ctr + 2 + flow_nl;
+  fc.spin = 5000000u;  // 50 ms of the 100 MHz real-time counter per wait
+  fc.prev_f = flow_prev_f;
+  fc.prev_b = flow_prev_b;
+  fc.check = flow_used ? 1 : 0;
+  flow_totals<T>(pairs[lead], kp, flow_prev_f, flow_prev_b);
+  flow_used = true;
+  return fc;
+}
+
+// After a synchronised solve or apply: the level runs finished without a
+// timed-out wait and handed out every unit; otherwise the results are not
+// trusted, the error is reported and the solver goes back to one launch per
+// level.
+void wa_solver::check_flow() {
+  if (!flow_ctr || !flow_used) return;
+  uint32_t* h = reinterpret_cast<uint32_t*>(hbuf);
+  HIPCHK(hipMemcpyAsync(h, flow_ctr, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h + 2, flow_ctr + 2 + flow_nl, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const uint32_t tick_f = h[0], err = h[1] | h[3], tick_b = h[2];
+  if (err == 0 && tick_f >= flow_prev_f && tick_b >= flow_prev_b) return;
+  pairs[lead].flow_t0 = pairs[lead].flow_t1 = -1;
+  flow_used = false;
+  char msg[160];
+  std::snprintf(msg, sizeof msg, "preconditioner level run failed (error bits %u, units %u/%u and %u/%u)", err,
+                tick_f, flow_prev_f, tick_b, flow_prev_b);
+  throw Error(WA_ERR_DEVICE, msg);
+}
+
 template <typename T>
 static const DevPass<T>& pass_of(wa_solver* s, int q);
 template <> const DevPass<double>& pass_of<double>(wa_solver* s, int q) { return s->p64[q]; }
@@ -700,12 +782,14 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp, const SideWork* side) {
       launch_precond_mult<float>(st, pre32, pairs, g32, R, Z, w32, mw32, mw32 + m * KMAX, mw32 + 2 * m * KMAX,
                                  ones, kp, m);
   } else if (overlap) {
+    const FlowCall fc = flow_call<T>(kp);
+    const FlowCall* fl = fc.ctr_f && pairs[lead].flow_t0 >= 0 ? &fc : nullptr;
     if constexpr (std::is_same<T, double>::value)
       launch_precond_overlap<double>(st, st2, ev_fork, ev_join, pre64, pairs, lead, R, Z, w64, part2, zc2_64, p2_64,
-                                     kp, m, side);
+                                     kp, m, side, fl);
     else
       launch_precond_overlap<float>(st, st2, ev_fork, ev_join, pre32, pairs, lead, R, Z, w32, (float*)part2, zc2_32,
-                                    p2_32, kp, m, side);
+                                    p2_32, kp, m, side, fl);
     side = nullptr;  // done on st2
   } else if constexpr (std::is_same<T, double>::value) {
     launch_precond<double>(st, pre64, R, Z, w64, kp, m);
@@ -1251,6 +1335,7 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
   toc(3, ev);
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  check_flow();
   double tsolve = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int c = 0; c < kb; ++c) {
     wa_rhs_stats& s = stats[c0 + c];
@@ -1299,6 +1384,7 @@ static wa_status level_a(wa_solver* s, const double* in, int64_t rows_in, int64_
                             hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     HIPCHK(hipGetLastError());
+    s->check_flow();
     if (s->timer.on) s->timer.flush();
     return WA_OK;
   } catch (const std::exception& e) {
@@ -1663,7 +1749,8 @@ wa_status wa_solver_info(const wa_solver* s, int64_t* info) {
   info[12] = f.dense_subs;
   info[13] = f.max_core;
   info[14] = s->host_only ? 0 : s->m;  // DOFs the device works on (active levels)
-  info[15] = s->host_only ? 0 : (s->overlap ? 2 : 1);  // streams of the additive apply
+  // streams of the additive apply; trailing levels run as one launch per direction
+  info[15] = s->host_only ? 0 : (s->overlap ? 2 : 1) + ((int64_t)s->flow_nl << 8);
   return WA_OK;
 }
 
