@@ -641,8 +641,9 @@ void wa_solver::setup_device() {
 // system are one short launch each, bound by latency rather than bandwidth;
 // the other factor pairs' local solves (long Schur-block rows, eliminated
 // rows) run beside them on a second stream.  Only when one pair has a level
-// chain worth hiding (at least kOverlapLevels forward levels) and another
-// pair has work; WITHIN_AMD_PRECOND_STREAMS=1 keeps every apply on one stream.
+// chain worth hiding (at least kOverlapLevels forward levels) and the other
+// pairs' launches are no more than the lead's; WITHIN_AMD_PRECOND_STREAMS=1
+// keeps every apply on one stream.
 void wa_solver::setup_overlap() {
   constexpr int kOverlapLevels = 8;
   const char* e = std::getenv("WITHIN_AMD_PRECOND_STREAMS");
@@ -658,6 +659,20 @@ void wa_solver::setup_overlap() {
         (depth(pairs[i]) == depth(pairs[lead]) && pairs[i].red1 - pairs[i].red0 > pairs[lead].red1 - pairs[lead].red0))
       lead = i;
   if (depth(pairs[lead]) < kOverlapLevels) return;
+  // the other pairs run as their own launches on st2, where one stream would
+  // merge them: only when they fit beside the lead's chain (measured: 3 pairs at
+  // 20M obs gain 10 %, 6 pairs at 50M obs lose 4 %)
+  auto launches = [](const PairSched& ps) {
+    int64_t c = 4;  // mean subtraction, two scatters, the combine's share
+    for (const DevSched* d : {&ps.s_loc, &ps.s_loc, &ps.s_keep, &ps.s_red, &ps.s_fwd, &ps.s_bwd, &ps.s_core,
+                              &ps.s_elim})
+      for (const Segment& g : d->segs) c += (g.r1 > g.r0 || g.i1 > g.i0 ? 1 : 0) + (g.f1 > g.f0 ? 1 : 0);
+    return c + 4 * (int64_t)ps.big.size();
+  };
+  int64_t others = 0;
+  for (size_t i = 0; i < pairs.size(); ++i)
+    if (i != lead) others += launches(pairs[i]);
+  if (others > launches(pairs[lead])) return;
   int64_t slots = 1, ld = 0;
   for (size_t i = 0; i < pairs.size(); ++i) {
     if (i == lead) continue;
