@@ -45,6 +45,15 @@ __device__ __forceinline__ void vstore(T* p, const VecT<T, NV>& v) {
   *reinterpret_cast<VecT<T, NV>*>(p) = v;
 }
 
+// the same store with the non-temporal hint: for streams written once per
+// pass and read back by another kernel (the Krylov vectors, the Gramian
+// product), which then leave fewer lines in L2
+template <typename T, int NV>
+__device__ __forceinline__ void vstore_nt(T* p, const VecT<T, NV>& v) {
+#pragma unroll
+  for (int c = 0; c < NV; ++c) __builtin_nontemporal_store(v.v[c], p + c);
+}
+
 // A kind with EMIT = true also receives each element's term coef * X[id]
 // (emit(e, col, v)), e.g. to write the restricted vector while summing it.
 template <class K, class = void>

@@ -304,14 +304,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(T* __restrict__ X, const T
       const VecT<T, NV> p = vload<T, NV>(P + id);
 #pragma unroll
       for (int c = 0; c < NV; ++c) x.v[c] += a[c] * p.v[c];
-      vstore<T, NV>(X + id, x);
+      vstore_nt<T, NV>(X + id, x);
     }
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
       r.v[c] = r.v[c] - a[c] * ap.v[c];
       acc[c] += (double)r.v[c] * (double)r.v[c];
     }
-    vstore<T, NV>(R + id, r);
+    vstore_nt<T, NV>(R + id, r);
   }
   block_col_sums<NV>(acc, sh, kp, gpb, part);
 }
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_x(T* __restrict__ X, const T* __r
     const VecT<T, NV> p = vload<T, NV>(P + id);
 #pragma unroll
     for (int c = 0; c < NV; ++c) x.v[c] += a[c] * p.v[c];
-    vstore<T, NV>(X + id, x);
+    vstore_nt<T, NV>(X + id, x);
   }
 }
 template <typename T>
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_xpby(T* __restrict__ P, const T* __r
 #pragma unroll
     for (int c = 0; c < NV; ++c)
       if (on[c]) p.v[c] = z.v[c] + b[c] * p.v[c];
-    vstore<T, NV>(P + id, p);
+    vstore_nt<T, NV>(P + id, p);
   }
 }
 template <typename T>
@@ -1130,7 +1130,10 @@ struct SpmmKind {
   __device__ Pre<NV> pre(uint32_t, int) const { return {}; }
   template <int NV>
   __device__ void finish(uint32_t r, int col, const T (&acc)[NV], const Pre<NV>&) const {
-    vstore<T, NV>(Y + (int64_t)r * kp + col, acc);
+    VecT<T, NV> y;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) y.v[c] = acc[c];
+    vstore_nt<T, NV>(Y + (int64_t)r * kp + col, y);
   }
 };
 
