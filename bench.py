@@ -107,7 +107,7 @@ def cpu_baseline(args, p_full):
     setup = time.perf_counter() - t0
     t, conv = o.time_solve_batch(Y, threads=threads)
     return {"value": round(k / t, 4), "unit": "RHS/s", "cores": threads, "kind": "port",
-            "sample": f"AKM workload n={p['codes'].shape[0]} ({scale:g} of 20M obs), k={k} RHS (one per thread), "
+            "sample": f"AKM workload n={p['codes'].shape[0]} (fraction {scale:g} of the 20M-obs panel), k={k} RHS (one per thread), "
                       f"additive Schwarz (Rust default local solver, pure approximate Cholesky) + CG FP64 "
                       f"(oracle/), tol 1e-8; solve {t:.2f} s, setup {setup:.1f} s excluded",
             "converged": int(conv.sum())}
