@@ -197,9 +197,12 @@ __device__ __forceinline__ void rows_wave(const K& k, int64_t g, uint32_t r0, ui
   if (g >= n_items) return;
   const Item it = items[g];
   if (it.slot < 0) {
-    for (uint32_t r = it.row0; r < it.row1; ++r) {
+    // one whole row (or none: padding has row0 == row1); its element range
+    // [e0, e1) is in the item, so the gathers need no row-pointer load
+    if (it.row1 > it.row0) {
+      const uint32_t r = it.row0;
       const auto q = k.template pre<NV>(r, pt * NV);
-      gather_vec<K, T, V, NV>(k, r, k.rbegin(r), k.rend(r), slot, pt, acc);
+      gather_vec<K, T, V, NV>(k, r, it.e0, it.e1, slot, pt, acc);
       if (slot == 0) k.template finish<NV>(r, pt * NV, acc, q);
     }
   } else {

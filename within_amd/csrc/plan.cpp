@@ -74,6 +74,8 @@ void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r
   for (u32 r = r0; r < r1; ++r) {
     const u32 len = row_ptr[r + 1] - row_ptr[r];
     if (len <= s.short_max || (skip && (*skip)[r])) continue;
+    // a whole-row item is one row and carries its element range (the
+    // engine reads the range from the item, not from the row pointers)
     if (len <= chunk) whole.push_back({r, r + 1, row_ptr[r], row_ptr[r + 1], -1, 0});
     else split.push_back(r);
   }
