@@ -391,7 +391,7 @@ void wa_solver::setup_device() {
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
     {
-      Schedule sc = make_schedule(g.indptr, kChunk, kShortRow);
+      Schedule sc = make_schedule(g.indptr, kGramChunk, kShortRow);
       sched_report("gramian", sc, g.indptr);
       g64.sched = g32.sched = upload_sched(sc);
     }
