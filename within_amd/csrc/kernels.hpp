@@ -18,6 +18,10 @@ constexpr uint32_t kChunk = 256;  // elements per work item (long rows are split
 // 1024 / 2048 / 4096 elements; longer chunks mean fewer slots to fix up.
 constexpr uint32_t kGramChunk = 1024;
 constexpr uint32_t kShortRow = 16;   // rows this short are summed by one lane group
+// Elimination levels: rows of 9-16 elements run faster as whole-wave items
+// (preconditioner 52.8 against 53.2 ms per 32-RHS block on the 20M AKM panel;
+// a limit of 4 gave 53.3 ms).
+constexpr uint32_t kShortLevel = 8;
 constexpr uint32_t kShortElim = 32;  // the same for elimination rows (their gathers hit L1)
 constexpr uint32_t kShortPass = 8;   // the same for factor passes (several gathers per element)
 

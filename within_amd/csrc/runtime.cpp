@@ -499,7 +499,7 @@ void wa_solver::setup_device() {
     const uint32_t* d_core_row_ptr = upload(core_row_ptr);
     auto level_sched = [&](const char* name, const std::vector<uint32_t>& ptr) {
       Schedule sc;
-      sc.short_max = kShortRow;
+      sc.short_max = kShortLevel;
       for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) append_schedule(sc, ptr, f.lvl_ptr[t], f.lvl_ptr[t + 1], kChunk);
       sched_report(name, sc, ptr);
       return upload_sched(sc);
@@ -561,7 +561,7 @@ void wa_solver::setup_device() {
           if (pre64.big[bi].b >= ps.core0 && pre64.big[bi].b < ps.core1) ps.big.push_back(bi);
         auto pair_levels = [&](const std::vector<uint32_t>& ptr) {
           Schedule sc;
-          sc.short_max = kShortRow;
+          sc.short_max = kShortLevel;
           for (size_t t = 0; t + 1 < f.lvl_ptr.size(); ++t) {
             u32 a0 = f.lvl_ptr[t], a1 = f.lvl_ptr[t + 1];
             while (a0 < a1 && f.red_sub[f.lvl_rows[a0]] < s0) ++a0;
