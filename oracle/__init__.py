@@ -87,6 +87,9 @@ def lib():
         L.oracle_schur_dense.argtypes = [P, C.c_int64, C.c_int64, P, P, C.c_int, C.c_uint64, C.c_int, P, P]
         L.oracle_time_solve_batch.argtypes = [P, P, C.c_int, C.c_int64, C.c_int, P]
         L.oracle_time_solve_batch.restype = C.c_double
+        L.oracle_restrict.argtypes = [P, P, C.c_int64, P, P]
+        L.oracle_prolongate_add.argtypes = [P, P, C.c_int64, P, P]
+        L.oracle_residual_update.argtypes = [C.c_int64, P, P, P, P, P, C.c_int64, P]
         L.oracle_lsmr_dense.argtypes = [C.c_int64, C.c_int64, P, P, P, C.c_double, C.c_int64, C.c_int64, P, P]
         _lib = L
     return _lib
@@ -286,6 +289,39 @@ def schur_dense(c_dense, diag_q, diag_r, approx=False, seed=0, split=1):
     if rc < 0:
         raise OracleError(-rc, lib().oracle_last_error().decode())
     return S, bool(rc), inv
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def restrict(gidx, w, global_vec):
+    """local[k] = w[k] * global[gidx[k]] (w None = uniform)."""
+    g, wv, gv = _u32(gidx), _f64(w), _f64(global_vec)
+    out = np.zeros(len(g))
+    lib().oracle_restrict(_ptr(g), _ptr(wv), len(g), _ptr(gv), _ptr(out))
+    return out
+
+
+def prolongate_add(gidx, w, local, out):
+    """out[gidx[k]] += w[k] * local[k], in place on a float64 array."""
+    g, wv, lv = _u32(gidx), _f64(w), _f64(local)
+    assert out.dtype == np.float64 and out.flags.c_contiguous
+    lib().oracle_prolongate_add(_ptr(g), _ptr(wv), len(g), _ptr(lv), _ptr(out))
+    return out
+
+
+def residual_update(indptr, indices, data, gidx, delta, r_work):
+    """r_work -= G[:, gidx] delta through the rows of the symmetric CSR G, in place."""
+    ip, ix, dt = _u32(indptr), _u32(indices), _f64(data)
+    g, dv = _u32(gidx), _f64(delta)
+    assert r_work.dtype == np.float64 and r_work.flags.c_contiguous
+    lib().oracle_residual_update(len(ip) - 1, _ptr(ip), _ptr(ix), _ptr(dt), _ptr(g), _ptr(dv), len(g), _ptr(r_work))
+    return r_work
 
 
 LSMR_STOPS = ("zero_rhs", "atb_zero", "residual_tol", "normal_tol", "breakdown", "maxiter")

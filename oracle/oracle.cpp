@@ -839,6 +839,33 @@ Schwarz build_schwarz(const Design& d, const LocalSolverConfig& cfg, bool multip
   return s;
 }
 
+// local[k] = w_k * global[gidx[k]]  (schwarz-precond/src/domain.rs:190-214,
+// SubdomainCore::restrict_weighted; w = 1 when the weights are uniform)
+void restrict_weighted(const u32* gidx, const double* w, i64 nl, const double* global, double* local) {
+  for (i64 k = 0; k < nl; ++k) local[k] = global[gidx[k]] * (w ? w[k] : 1.0);
+}
+
+// out[gidx[k]] += w_k * local[k]  (domain.rs:216-243, prolongate_weighted_add)
+void prolongate_weighted_add(const u32* gidx, const double* w, i64 nl, const double* local, double* out) {
+  for (i64 k = 0; k < nl; ++k) out[gidx[k]] += local[k] * (w ? w[k] : 1.0);
+}
+
+// r_work -= G[:, gidx] delta, through the rows of the symmetric G
+// (operator/residual_update.rs:36-53, SparseGramianUpdater::update; the same
+// vector as OperatorResidualUpdater, multiplicative.rs:55-103)
+void residual_update(const Csr& g, const u32* gidx, const double* delta, i64 nl, double* r_work) {
+  for (i64 k = 0; k < nl; ++k) {
+    const double c = delta[k];
+    if (c == 0.0) continue;
+    const u32 row = gidx[k];
+    for (u32 p = g.indptr[row]; p < g.indptr[row + 1]; ++p) r_work[g.indices[p]] -= c * g.data[p];
+  }
+}
+
+// Additive: z = sum_i R_i' W_i A_i^+ W_i R_i r (additive/preconditioner.rs:
+// 164-214, local_solve.rs:182-248).  Multiplicative: the forward sweep of
+// multiplicative.rs:128-245 -- each subdomain restricts the current r_work,
+// and its weighted correction is added to z and taken off r_work.
 void Schwarz::apply(const double* r, double* z) const {
   for (i64 j = 0; j < n_dofs; ++j) z[j] = 0.0;
   std::vector<double> rw;
@@ -847,19 +874,16 @@ void Schwarz::apply(const double* r, double* z) const {
   for (size_t i = 0; i < subs.size(); ++i) {
     const Subdomain& sd = subs[i];
     const BlockElim& le = solvers[i];
-    i64 nl = sd.gidx.size();
+    const i64 nl = sd.gidx.size();
+    const double* w = sd.pou.empty() ? nullptr : sd.pou.data();
     std::vector<double> rs(nl + le.n_reduced() + 1, 0.0), zs(nl + le.n_reduced() + 1, 0.0);
-    for (i64 k = 0; k < nl; ++k) rs[k] = src[sd.gidx[k]] * (sd.pou.empty() ? 1.0 : sd.pou[k]);
+    restrict_weighted(sd.gidx.data(), w, nl, src, rs.data());
     le.solve(rs.data(), zs.data());
-    for (i64 k = 0; k < nl; ++k) z[sd.gidx[k]] += zs[k] * (sd.pou.empty() ? 1.0 : sd.pou[k]);
+    prolongate_weighted_add(sd.gidx.data(), w, nl, zs.data(), z);
     if (multiplicative) {
-      for (i64 k = 0; k < nl; ++k) {
-        double c = zs[k] * (sd.pou.empty() ? 1.0 : sd.pou[k]);
-        if (c == 0.0) continue;
-        u32 row = sd.gidx[k];
-        for (u32 p = gramian->indptr[row]; p < gramian->indptr[row + 1]; ++p)
-          rw[gramian->indices[p]] -= c * gramian->data[p];
-      }
+      std::vector<double> delta(nl);
+      for (i64 k = 0; k < nl; ++k) delta[k] = zs[k] * (w ? w[k] : 1.0);
+      residual_update(*gramian, sd.gidx.data(), delta.data(), nl, rw.data());
     }
   }
 }

@@ -19,7 +19,13 @@
 //                                                   pinned by math: unbiasedness, exactness on
 //                                                   degree<=2 eliminations, Laplacian invariants.
 //   block-elimination local solve, Schwarz       -- pinned: local solve == pseudo-inverse on
-//                                                   exact factors, PoU invariants
+//                                                   exact factors, PoU invariants; restrict /
+//                                                   prolongate / residual update on the
+//                                                   reference's values (local_solve.rs:313-376,
+//                                                   multiplicative.rs:392-430, operator/tests.rs:
+//                                                   195-346); additive sum and multiplicative
+//                                                   sweep on overlapping subdomains == dense
+//                                                   sum_i R_i' W_i G_i^+ W_i R_i formulas
 //   CG / GMRES / refinement                      -- pinned: cg.rs SpdMatrix3 case, closed forms,
 //                                                   dense lstsq projection of small problems
 //   LSMR (plain and preconditioned, windowed)    -- pinned: the reference's lsmr/tests.rs
@@ -164,6 +170,10 @@ Csr build_gramian(const Design& d);
 // ---------------------------------------------------------------------------
 // Schwarz preconditioners (schwarz-precond/src/schwarz/**)
 // ---------------------------------------------------------------------------
+void restrict_weighted(const u32* gidx, const double* w, i64 nl, const double* global, double* local);
+void prolongate_weighted_add(const u32* gidx, const double* w, i64 nl, const double* local, double* out);
+void residual_update(const Csr& g, const u32* gidx, const double* delta, i64 nl, double* r_work);
+
 struct Schwarz {
   i64 n_dofs = 0;
   bool multiplicative = false;

@@ -317,6 +317,25 @@ int oracle_schur_dense(const double* cdense, int64_t nq, int64_t nr, const doubl
   }
 }
 
+// Restriction / prolongation / residual update of the Schwarz apply on their
+// own (pinned by the reference's local_solve.rs:313-376 and
+// multiplicative.rs:392-430 values); w may be NULL (uniform weights).
+void oracle_restrict(const uint32_t* gidx, const double* w, int64_t nl, const double* global, double* local) {
+  restrict_weighted(gidx, w, nl, global, local);
+}
+void oracle_prolongate_add(const uint32_t* gidx, const double* w, int64_t nl, const double* local, double* out) {
+  prolongate_weighted_add(gidx, w, nl, local, out);
+}
+void oracle_residual_update(int64_t n, const uint32_t* indptr, const uint32_t* indices, const double* data,
+                            const uint32_t* gidx, const double* delta, int64_t nl, double* r_work) {
+  Csr g;
+  g.nrows = g.ncols = n;
+  g.indptr.assign(indptr, indptr + n + 1);
+  g.indices.assign(indices, indices + indptr[n]);
+  g.data.assign(data, data + indptr[n]);
+  residual_update(g, gidx, delta, nl, r_work);
+}
+
 // Timed CPU baseline: solve k RHS (columns of Y) with `threads` threads; returns
 // wall seconds of the solve phase (setup excluded; the solver is built already).
 double oracle_time_solve_batch(void* sv, const double* Y, int k, int64_t ldy, int threads,
