@@ -122,7 +122,11 @@ wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_
                                 int64_t ldx, double* demeaned, int64_t ldd, wa_rhs_stats* stats);
 
 /* ---- Level A: Operator drop-ins (schwarz-precond/src/lib.rs:106-131) ------
- * Column-major blocks of k vectors with leading dimension ld. */
+ * Column-major blocks of k vectors (any k >= 0) with leading dimension ld.
+ * The operators run on blocks of up to 32 columns in the solver's own work
+ * vectors.  The *_device variants take gfx950 device pointers and allocate
+ * nothing; the host-pointer variants stage through a device buffer owned by
+ * the solver, allocated on the first call that needs a larger one. */
 /* GramianOperator::apply (operator/gramian.rs:84-89): Y = D'W D X */
 wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
 /* Gramian (explicit CSR) Operator::apply (gramian/explicit.rs:261-277) */
@@ -134,6 +138,13 @@ wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int
 /* WeightedDesign::rmatvec_wdt (domain.rs:409-418): X (n_dofs x k) = D'W R */
 wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx,
                          int32_t k);
+/* The same five operators on device pointers (no allocation, no staging). */
+wa_status wa_gram_apply_device(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
+wa_status wa_gram_apply_explicit_device(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
+wa_status wa_precond_apply_device(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld);
+wa_status wa_matvec_d_device(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k);
+wa_status wa_rmatvec_wdt_device(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx,
+                                int32_t k);
 
 /* ---- serialized preconditioners (pickling of FePreconditioner) -----------
  * Replaces the postcard bytes of FePreconditioner.__reduce__ / FePreconditioner

@@ -64,6 +64,8 @@ EXPORTS = [
     "wa_matvec_d", "wa_rmatvec_wdt", "wa_solver_info", "wa_solver_subdomain", "wa_solver_subdomain_factor",
     "wa_timing_enable", "wa_timing_reset", "wa_timing_get", "wa_solver_precond_bytes",
     "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
+    "wa_gram_apply_device", "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_matvec_d_device",
+    "wa_rmatvec_wdt_device",
 ]
 
 _lib = None
@@ -78,6 +80,13 @@ def load():
         raise ImportError(
             f"within_amd native library not found at {LIB_PATH}; build it with "
             "`python -m within_amd.build` (hipcc, gfx950)")
+    try:
+        # PyTorch ships its own HIP runtime.  When this library initialises the
+        # device before torch is imported, torch later finds no GPU; importing
+        # torch first lets both share the device (torch is optional here).
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.c_void_p
     i64, i32 = C.c_int64, C.c_int32
@@ -100,10 +109,11 @@ def load():
     L.wa_solver_destroy.restype = None
     for nm in ("wa_solve_batch", "wa_solve_batch_device"):
         getattr(L, nm).argtypes = [P, P, i32, i64, P, i64, P, i64, P]
-    for nm in ("wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply"):
+    for nm in ("wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply", "wa_gram_apply_device",
+               "wa_gram_apply_explicit_device", "wa_precond_apply_device"):
         getattr(L, nm).argtypes = [P, P, P, i32, i64]
-    L.wa_matvec_d.argtypes = [P, P, i64, P, i64, i32]
-    L.wa_rmatvec_wdt.argtypes = [P, P, i64, P, i64, i32]
+    for nm in ("wa_matvec_d", "wa_rmatvec_wdt", "wa_matvec_d_device", "wa_rmatvec_wdt_device"):
+        getattr(L, nm).argtypes = [P, P, i64, P, i64, i32]
     L.wa_solver_info.argtypes = [P, P]
     L.wa_solver_subdomain.argtypes = [P, i64, P, P, P]
     L.wa_solver_subdomain_factor.argtypes = [P, i64, P, P, P, P, P, P, P, P, P]

@@ -14,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -45,19 +46,22 @@ def build(verbose: bool = False) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    jobs = []
     plan_o = os.path.join(BUILD, "plan.o")
     # no FP contraction: host factors must match the oracle bit for bit
-    _run(["g++", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
-          os.path.join(CSRC, "plan.cpp"), "-o", plan_o], verbose)
-    objs.append(plan_o)
+    jobs.append((["g++", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
+                  os.path.join(CSRC, "plan.cpp"), "-o", plan_o], plan_o))
     for src in ("kernels.hip", "runtime.cpp", "dense.hip"):
         o = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         # the dense-core inverse must round like the host factorisation
         exact = ["-ffp-contract=off"] if src == "dense.hip" else []
-        _run([hipcc, *HIP_FLAGS, *exact, *lang, "-c", os.path.join(CSRC, src), "-o", o], verbose)
-        objs.append(o)
+        jobs.append(([hipcc, *HIP_FLAGS, *exact, *lang, "-c", os.path.join(CSRC, src), "-o", o], o))
+    # the translation units are independent: compile them concurrently
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd, _ in jobs]:
+            f.result()
+    objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"], verbose)
     os.replace(tmp, LIB)

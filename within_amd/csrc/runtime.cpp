@@ -107,6 +107,7 @@ struct wa_solver {
   const uint32_t* int_of = nullptr;  // user DOF -> internal (UINT32_MAX = inactive)
   const uint32_t* ext_of = nullptr;  // internal -> user DOF
   int Q = 0;
+  int dev = 0;                // the device the solver's state lives on
   // design
   uint32_t* codes = nullptr;  // Q x n (original order)
   uint32_t* offs = nullptr;
@@ -176,6 +177,16 @@ struct wa_solver {
   // device user buffers for host-pointer calls
   double* ubuf = nullptr;
   int64_t ubuf_len = 0;
+  double* staging(int64_t need) {
+    if (need > ubuf_len) {
+      if (ubuf) HIPCHK(hipFree(ubuf));
+      ubuf = nullptr;
+      ubuf_len = 0;
+      HIPCHK(hipMalloc(&ubuf, sizeof(double) * need));
+      ubuf_len = need;
+    }
+    return ubuf;
+  }
 
   template <typename T>
   T* dalloc(int64_t count) {
@@ -228,6 +239,7 @@ struct wa_solver {
   ~wa_solver() {
     if (st) (void)hipStreamSynchronize(st);
     for (void* p : allocs) (void)hipFree(p);
+    if (ubuf) (void)hipFree(ubuf);
     if (st2) (void)hipStreamSynchronize(st2);
     if (hbuf) (void)hipHostFree(hbuf);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -338,6 +350,7 @@ void wa_solver::setup_device() {
     t_ph = t1;
   };
   check_device();
+  HIPCHK(hipGetDevice(&dev));
   phase("init");
   const DeviceLayout lay = make_device_layout(plan);
   phase("layout");
@@ -1374,29 +1387,39 @@ static void need_design(wa_solver* s) {
   if (s->plan.d.Q == 0) throw Error(WA_ERR_INVALID_ARGUMENT, "a preconditioner on its own has no design");
 }
 
-// host column-major in/out around a DOF-block device op
+// Level A: column-major blocks of k vectors (leading dimension ld).  The
+// operators run on blocks of at most KMAX columns in the handle's row-major
+// work vectors, so a call allocates nothing; host pointers are staged through
+// the handle's device buffer, which grows when a call needs more and is kept.
 template <typename F>
-static wa_status level_a(wa_solver* s, const double* in, int64_t rows_in, int64_t ldin, double* out,
+static wa_status level_a(wa_solver* s, bool device, const double* in, int64_t rows_in, int64_t ldin, double* out,
                          int64_t rows_out, int64_t ldout, int32_t k, F&& body) {
   try {
     need_device(s);
     std::lock_guard<std::mutex> lock(s->call_mu);
-    if (k <= 0 || k > KMAX) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be in 1..32");
+    if (k < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be >= 0");
+    if (k == 0) return WA_OK;
     if (!in || !out || ldin < rows_in || ldout < rows_out)
       throw Error(WA_ERR_INVALID_ARGUMENT, "bad buffer or leading dimension");
-    const int kp = pow2_at_least(k);
-    double *din = nullptr, *dout = nullptr;
-    HIPCHK(hipMalloc(&din, sizeof(double) * rows_in * k));
-    HIPCHK(hipMalloc(&dout, sizeof(double) * rows_out * k));
-    std::unique_ptr<double, void (*)(double*)> g1(din, [](double* p) { (void)hipFree(p); });
-    std::unique_ptr<double, void (*)(double*)> g2(dout, [](double* p) { (void)hipFree(p); });
-    for (int c = 0; c < k; ++c)
-      HIPCHK(hipMemcpyAsync(din + (int64_t)c * rows_in, in + (int64_t)c * ldin, sizeof(double) * rows_in,
-                            hipMemcpyHostToDevice, s->st));
-    body(din, dout, kp);
-    for (int c = 0; c < k; ++c)
-      HIPCHK(hipMemcpyAsync(out + (int64_t)c * ldout, dout + (int64_t)c * rows_out, sizeof(double) * rows_out,
-                            hipMemcpyDeviceToHost, s->st));
+    const double* din = in;
+    double* dout = out;
+    int64_t li = ldin, lo = ldout;
+    if (!device) {
+      double* buf = s->staging(rows_in * k + rows_out * k);
+      HIPCHK(hipMemcpy2DAsync(buf, sizeof(double) * rows_in, in, sizeof(double) * ldin, sizeof(double) * rows_in,
+                              (size_t)k, hipMemcpyHostToDevice, s->st));
+      din = buf;
+      dout = buf + rows_in * k;
+      li = rows_in;
+      lo = rows_out;
+    }
+    for (int c0 = 0; c0 < k; c0 += KMAX) {
+      const int kb = std::min(KMAX, k - c0);
+      body(din + (int64_t)c0 * li, li, dout + (int64_t)c0 * lo, lo, kb, pow2_at_least(kb));
+    }
+    if (!device)
+      HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * ldout, dout, sizeof(double) * rows_out,
+                              sizeof(double) * rows_out, (size_t)k, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     HIPCHK(hipGetLastError());
     s->check_flow();
@@ -1405,6 +1428,82 @@ static wa_status level_a(wa_solver* s, const double* in, int64_t rows_in, int64_
   } catch (const std::exception& e) {
     return (wa_status)set_error(e);
   }
+}
+
+// the Level A operators on one column block (in: ldi, out: ldo, kb columns,
+// padded to kp)
+static wa_status gram_apply_impl(wa_solver* s, bool device, const double* X, double* Y, int32_t k, int64_t ld) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t mu = s ? s->mu : 0;
+  return level_a(s, device, X, mu, ld, Y, mu, ld, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->m, kb, s->cp64, kp, s->ext_of);
+                   const int saved = s->params.operator_repr;
+                   s->params.operator_repr = 0;
+                   s->op_apply<double>(s->cp64, s->cap64, kp);
+                   s->params.operator_repr = saved;
+                   launch_row_to_col(s->st, s->cap64, kp, s->mu, kb, out, ldo, s->int_of);
+                 });
+}
+
+static wa_status gram_apply_explicit_impl(wa_solver* s, bool device, const double* X, double* Y, int32_t k,
+                                          int64_t ld) {
+  if (s && !s->has_gram) {
+    g_err = "solver has no explicit Gramian (build with operator_repr = 1 or multiplicative Schwarz)";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t mu = s ? s->mu : 0;
+  return level_a(s, device, X, mu, ld, Y, mu, ld, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->m, kb, s->cp64, kp, s->ext_of);
+                   launch_spmm<double>(s->st, s->g64, s->cp64, s->cap64, s->part64, kp);
+                   launch_row_to_col(s->st, s->cap64, kp, s->mu, kb, out, ldo, s->int_of);
+                 });
+}
+
+static wa_status precond_apply_impl(wa_solver* s, bool device, const double* R, double* Z, int32_t k, int64_t ld) {
+  if (s && s->params.preconditioner == 0) {
+    g_err = "solver has no preconditioner";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t mu = s ? s->mu : 0;
+  return level_a(s, device, R, mu, ld, Z, mu, ld, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->m, kb, s->cr64, kp, s->ext_of);
+                   s->precondition<double>(s->cr64, s->cz64, kp);
+                   launch_row_to_col(s->st, s->cz64, kp, s->mu, kb, out, ldo, s->int_of);
+                 });
+}
+
+static wa_status matvec_d_impl(wa_solver* s, bool device, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                               int32_t k) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  return level_a(s, device, X, s ? s->mu : 0, ldx, Y, s ? s->n : 0, ldy, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->m, kb, s->cp64, kp, s->ext_of);
+                   launch_demean(s->st, s->n, s->Q, s->codes, s->offs, s->cp64, nullptr, s->E, kp);
+                   launch_row_to_col(s->st, s->E, kp, s->n, kb, out, ldo);
+                 });
+}
+
+static wa_status rmatvec_wdt_impl(wa_solver* s, bool device, const double* R, int64_t ldr, double* X, int64_t ldx,
+                                  int32_t k) {
+  if (s && s->plan.d.Q == 0) {
+    g_err = "a preconditioner on its own has no design";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  return level_a(s, device, R, s ? s->n : 0, ldr, X, s ? s->mu : 0, ldx, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->n, kb, s->Yr, kp);
+                   s->rhs64(s->Yr, s->B64, kp);
+                   launch_row_to_col(s->st, s->B64, kp, s->mu, kb, out, ldo, s->int_of);
+                 });
 }
 
 extern "C" {
@@ -1619,14 +1718,8 @@ static wa_status solve_impl(wa_solver* s, const double* Y, int32_t k, int64_t ld
     const double* dY = Y;
     double *dX = X, *dD = dem;
     if (!device) {
-      int64_t need = ldy * k + s->mu * (int64_t)k + (dem ? s->n * (int64_t)k : 0);
-      if (need > s->ubuf_len) {
-        if (s->ubuf) HIPCHK(hipFree(s->ubuf));
-        s->ubuf = nullptr;
-        HIPCHK(hipMalloc(&s->ubuf, sizeof(double) * need));
-        s->ubuf_len = need;
-      }
-      double* y = s->ubuf;
+      const int64_t need = ldy * k + s->mu * (int64_t)k + (dem ? s->n * (int64_t)k : 0);
+      double* y = s->staging(need);
       HIPCHK(hipMemcpyAsync(y, Y, sizeof(double) * ldy * k, hipMemcpyHostToDevice, s->st));
       dY = y;
       dX = y + ldy * k;
@@ -1664,66 +1757,34 @@ wa_status wa_solve_batch_device(wa_solver* s, const double* Y, int32_t k, int64_
 }
 
 wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
-  if (s && s->plan.d.Q == 0) {
-    g_err = "a preconditioner on its own has no design";
-    return WA_ERR_INVALID_ARGUMENT;
-  }
-  return level_a(s, X, s ? s->mu : 0, ld, Y, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
-    int saved = s->params.operator_repr;
-    s->params.operator_repr = 0;
-    s->op_apply<double>(s->cp64, s->cap64, kp);
-    s->params.operator_repr = saved;
-    launch_row_to_col(s->st, s->cap64, kp, s->mu, k, dout, s->mu, s->int_of);
-  });
+  return gram_apply_impl(s, false, X, Y, k, ld);
 }
-
+wa_status wa_gram_apply_device(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  return gram_apply_impl(s, true, X, Y, k, ld);
+}
 wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
-  if (s && !s->has_gram) {
-    g_err = "solver has no explicit Gramian (build with operator_repr = 1 or multiplicative Schwarz)";
-    return WA_ERR_INVALID_ARGUMENT;
-  }
-  return level_a(s, X, s ? s->mu : 0, ld, Y, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
-    launch_spmm<double>(s->st, s->g64, s->cp64, s->cap64, s->part64, kp);
-    launch_row_to_col(s->st, s->cap64, kp, s->mu, k, dout, s->mu, s->int_of);
-  });
+  return gram_apply_explicit_impl(s, false, X, Y, k, ld);
 }
-
+wa_status wa_gram_apply_explicit_device(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  return gram_apply_explicit_impl(s, true, X, Y, k, ld);
+}
 wa_status wa_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld) {
-  if (s && s->params.preconditioner == 0) {
-    g_err = "solver has no preconditioner";
-    return WA_ERR_INVALID_ARGUMENT;
-  }
-  return level_a(s, R, s ? s->mu : 0, ld, Z, s ? s->mu : 0, ld, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cr64, kp, s->ext_of);
-    s->precondition<double>(s->cr64, s->cz64, kp);
-    launch_row_to_col(s->st, s->cz64, kp, s->mu, k, dout, s->mu, s->int_of);
-  });
+  return precond_apply_impl(s, false, R, Z, k, ld);
 }
-
+wa_status wa_precond_apply_device(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld) {
+  return precond_apply_impl(s, true, R, Z, k, ld);
+}
 wa_status wa_matvec_d(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k) {
-  if (s && s->plan.d.Q == 0) {
-    g_err = "a preconditioner on its own has no design";
-    return WA_ERR_INVALID_ARGUMENT;
-  }
-  return level_a(s, X, s ? s->mu : 0, ldx, Y, s ? s->n : 0, ldy, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->mu, s->m, k, s->cp64, kp, s->ext_of);
-    launch_demean(s->st, s->n, s->Q, s->codes, s->offs, s->cp64, nullptr, s->E, kp);
-    launch_row_to_col(s->st, s->E, kp, s->n, k, dout, s->n);
-  });
+  return matvec_d_impl(s, false, X, ldx, Y, ldy, k);
 }
-
+wa_status wa_matvec_d_device(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k) {
+  return matvec_d_impl(s, true, X, ldx, Y, ldy, k);
+}
 wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
-  if (s && s->plan.d.Q == 0) {
-    g_err = "a preconditioner on its own has no design";
-    return WA_ERR_INVALID_ARGUMENT;
-  }
-  return level_a(s, R, s ? s->n : 0, ldr, X, s ? s->mu : 0, ldx, k, [&](double* din, double* dout, int kp) {
-    launch_col_to_row(s->st, din, s->n, s->n, k, s->Yr, kp);
-    s->rhs64(s->Yr, s->B64, kp);
-    launch_row_to_col(s->st, s->B64, kp, s->mu, k, dout, s->mu, s->int_of);
-  });
+  return rmatvec_wdt_impl(s, false, R, ldr, X, ldx, k);
+}
+wa_status wa_rmatvec_wdt_device(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
+  return rmatvec_wdt_impl(s, true, R, ldr, X, ldx, k);
 }
 
 wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4) {
