@@ -145,6 +145,19 @@ wa_status wa_precond_apply_device(wa_solver* s, const double* R, double* Z, int3
 wa_status wa_matvec_d_device(wa_solver* s, const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t k);
 wa_status wa_rmatvec_wdt_device(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx,
                                 int32_t k);
+/* The implicit operator Y = D'W D X on device pointers with no allocation
+ * and without the solver's work vectors.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Its scratch (the
+ * row-major copies of up to 32 columns of X and of Y, and the factor passes'
+ * partial sums) is a 64 MiB device array of the library that each GPU holds
+ * until the process ends; column blocks are narrowed to fit it, and a design
+ * needing more than 64 MiB per column (about 4M DOFs) is rejected with
+ * WA_ERR_INVALID_ARGUMENT.  Nothing in it has to start at zero.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (from several
+ * host threads on different solvers) race.  The solver's device must be
+ * current. */
+wa_status wa_gram_apply_noalloc(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
 
 /* ---- serialized preconditioners (pickling of FePreconditioner) -----------
  * Replaces the postcard bytes of FePreconditioner.__reduce__ / FePreconditioner

@@ -65,7 +65,7 @@ EXPORTS = [
     "wa_timing_enable", "wa_timing_reset", "wa_timing_get", "wa_solver_precond_bytes",
     "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
     "wa_gram_apply_device", "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_matvec_d_device",
-    "wa_rmatvec_wdt_device",
+    "wa_rmatvec_wdt_device", "wa_gram_apply_noalloc",
 ]
 
 _lib = None
@@ -110,7 +110,7 @@ def load():
     for nm in ("wa_solve_batch", "wa_solve_batch_device"):
         getattr(L, nm).argtypes = [P, P, i32, i64, P, i64, P, i64, P]
     for nm in ("wa_gram_apply", "wa_gram_apply_explicit", "wa_precond_apply", "wa_gram_apply_device",
-               "wa_gram_apply_explicit_device", "wa_precond_apply_device"):
+               "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_gram_apply_noalloc"):
         getattr(L, nm).argtypes = [P, P, P, i32, i64]
     for nm in ("wa_matvec_d", "wa_rmatvec_wdt", "wa_matvec_d_device", "wa_rmatvec_wdt_device"):
         getattr(L, nm).argtypes = [P, P, i64, P, i64, i32]

@@ -1506,6 +1506,38 @@ static wa_status rmatvec_wdt_impl(wa_solver* s, bool device, const double* R, in
                  });
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free implicit operator apply (wa_gram_apply_noalloc): its scratch
+// -- the row-major copies of a column block of X and of G X, and the
+// split-row partial sums of the factor passes -- lives in one file-scope
+// device array instead of the solver's work vectors.  Each GPU holds its own
+// copy of the array until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for that device.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchDoubles = (int64_t)1 << 23;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device ordinal
+
+// the array's address on device `dev` (which must be current), looked up on
+// the first call on that device
+double* scratch_on(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw DevError("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+int64_t align32(int64_t v) { return (v + 31) / 32 * 32; }
+}  // namespace
+
 extern "C" {
 
 const char* wa_last_error(void) { return g_err.c_str(); }
@@ -1761,6 +1793,47 @@ wa_status wa_gram_apply(wa_solver* s, const double* X, double* Y, int32_t k, int
 }
 wa_status wa_gram_apply_device(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
   return gram_apply_impl(s, true, X, Y, k, ld);
+}
+wa_status wa_gram_apply_noalloc(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
+  try {
+    need_design(s);
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    if (k < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be >= 0");
+    if (k == 0) return WA_OK;
+    if (!X || !Y || ld < s->mu) throw Error(WA_ERR_INVALID_ARGUMENT, "bad buffer or leading dimension");
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != s->dev) throw Error(WA_ERR_INVALID_ARGUMENT, "wa_gram_apply_noalloc: make the solver's device current");
+    int64_t slots = 1;
+    for (const DevPass<double>& p : s->p64) slots = std::max(slots, p.sched.n_slots);
+    // per block of kp columns: X rows, G X rows, pass partials
+    auto need = [&](int kp) { return 2 * align32(s->m * kp) + align32(slots * kp); };
+    if (need(1) > kScratchDoubles) {
+      char msg[200];
+      std::snprintf(msg, sizeof msg,
+                    "wa_gram_apply_noalloc: the design needs %lld bytes of scratch per column, more than the "
+                    "%lld-byte scratch array",
+                    (long long)(need(1) * 8), (long long)(kScratchDoubles * 8));
+      throw Error(WA_ERR_INVALID_ARGUMENT, msg);
+    }
+    int kbmax = KMAX;
+    while (need(kbmax) > kScratchDoubles) kbmax /= 2;
+    double* sp = scratch_on(dev);
+    for (int c0 = 0; c0 < k; c0 += kbmax) {
+      const int kb = std::min(kbmax, k - c0), kp = pow2_at_least(kb);
+      double* xr = sp;
+      double* gx = sp + align32(s->m * kp);
+      double* part = gx + align32(s->m * kp);
+      launch_col_to_row(s->st, X + (int64_t)c0 * ld, ld, s->m, kb, xr, kp, s->ext_of);
+      for (int q = 0; q < s->Q; ++q) launch_pull<double>(s->st, s->p64[q], xr, gx, part, kp);
+      launch_row_to_col(s->st, gx, kp, s->mu, kb, Y + (int64_t)c0 * ld, ld, s->int_of);
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipGetLastError());
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
 }
 wa_status wa_gram_apply_explicit(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld) {
   return gram_apply_explicit_impl(s, false, X, Y, k, ld);
