@@ -23,6 +23,7 @@ extern "C" {
 
 typedef struct wa_design wa_design;
 typedef struct wa_solver wa_solver;
+typedef struct wa_multi wa_multi;
 
 typedef enum {
   WA_OK = 0,
@@ -65,7 +66,13 @@ typedef struct {
   int64_t maxiter;         /* default 1000                                         */
   int64_t max_refinements; /* default 2                                            */
   int32_t precision;       /* 0 = FP64 throughout (reference semantics),
-                              1 = FP32 inner Krylov + FP64 iterative refinement    */
+                              1 = FP32 inner Krylov + FP64 iterative refinement:
+                              FP32 solves to max(tol', 1e-5) are repeated from
+                              the FP64 residual until it meets tol, at most
+                              max_refinements + 8 times; converged is that final
+                              residual test.  With the explicit Gramian the
+                              residual is b - Gx in FP64 (b = D'Wy), the same
+                              vector as D'W(y - Dx) up to ~eps (|b| + |G||x|)  */
   int32_t block;           /* RHS per block-vector solve: 0 = auto (32), 1 = one
                               RHS per solve                                        */
   int32_t host_only;       /* 1 = build the host-side setup only (no GPU); apply
@@ -158,6 +165,29 @@ wa_status wa_rmatvec_wdt_device(wa_solver* s, const double* R, int64_t ldr, doub
  * host threads on different solvers) race.  The solver's device must be
  * current. */
 wa_status wa_gram_apply_noalloc(wa_solver* s, const double* X, double* Y, int32_t k, int64_t ld);
+
+/* ---- many independent problems --------------------------------------------
+ * Solver::solve_batch (solver.rs:242-277) for each of n_problems independent
+ * designs, run together: the problems with the same number of factors are
+ * stacked block-diagonally into one design whose preconditioner, operator and
+ * vector kernels serve all of them in each launch, while every (problem,
+ * column) pair keeps its own CG scalars, tolerance and convergence test, as
+ * a separate solve would.  CG only (KrylovMethod::Cg), additive Schwarz or no
+ * preconditioner; the reference has no multi-problem entry point (each
+ * problem is its own Solver).
+ * wa_multi_solve: problem i has k[i] right-hand sides Y[i] (n_obs_i x k[i],
+ * column-major, ld ldy[i]) and receives X[i] (n_dofs_i x k[i], ld ldx[i]),
+ * demeaned[i] (may be NULL, or demeaned itself NULL) and stats[i] (k[i]
+ * entries).  The pointers may be host or device memory.
+ * wa_multi_info: wa_solver_info of stack `group` (groups in order of first
+ * appearance of each factor count), with entry 15 = its problem count. */
+wa_status wa_multi_create(const wa_design* const* designs, int32_t n_problems, const wa_params* p,
+                          wa_multi** out);
+void wa_multi_destroy(wa_multi* mp);
+wa_status wa_multi_info(const wa_multi* mp, int32_t group, int64_t* info16);
+wa_status wa_multi_solve(wa_multi* mp, const double* const* Y, const int32_t* k, const int64_t* ldy,
+                         double* const* X, const int64_t* ldx, double* const* demeaned, const int64_t* ldd,
+                         wa_rhs_stats* const* stats);
 
 /* ---- serialized preconditioners (pickling of FePreconditioner) -----------
  * Replaces the postcard bytes of FePreconditioner.__reduce__ / FePreconditioner

@@ -145,6 +145,16 @@ def test_nan_and_inf_in_y_propagate():
         assert np.any(np.isnan(r.x)) or not r.converged
 
 
+def test_nan_residual_spends_the_iteration_budget():
+    # cg.rs:75-125 keeps iterating on a NaN residual until maxiter, and every
+    # refinement correction adds its own budget (solver.rs:296-347)
+    c = cats(np.array([0, 1, 0, 2]), np.array([0, 0, 1, 1]))
+    r = W.solve(c, np.array([1.0, np.nan, 3.0, 2.0]), W.CG(maxiter=40, max_refinements=2))
+    assert not r.converged and r.iterations == 3 * 40
+    r = W.solve(c, np.array([1.0, np.nan, 3.0, 2.0]), W.CG(maxiter=40, max_refinements=2, precision="mixed"))
+    assert not r.converged and r.iterations == 3 * 40
+
+
 def test_weight_edge_cases_do_not_crash():
     c = cats(np.array([0, 1, 0, 1, 2]), np.array([0, 0, 1, 1, 0]))
     y = np.ones(5)

@@ -19,16 +19,19 @@ from ._within import (  # noqa: F401
     MultiplicativeSchwarz,
     OperatorRepr,
     Preconditioner,
+    ProblemBatch,
     ReductionStrategy,
     SchurComplement,
     SolveResult,
     Solver,
     solve,
     solve_batch,
+    solve_many,
 )
 
 __all__ = [
     "OperatorRepr", "Preconditioner", "SolveResult", "BatchSolveResult", "CG", "GMRES", "LSMR",
     "FePreconditioner", "Solver", "solve", "solve_batch", "AdditiveSchwarz", "AdditiveSchwarzDiagnostics",
     "MultiplicativeSchwarz", "ReductionStrategy", "ApproxCholConfig", "ApproxSchurConfig", "SchurComplement",
+    "ProblemBatch", "solve_many",
 ]

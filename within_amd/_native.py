@@ -65,7 +65,8 @@ EXPORTS = [
     "wa_timing_enable", "wa_timing_reset", "wa_timing_get", "wa_solver_precond_bytes",
     "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
     "wa_gram_apply_device", "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_matvec_d_device",
-    "wa_rmatvec_wdt_device", "wa_gram_apply_noalloc",
+    "wa_rmatvec_wdt_device", "wa_gram_apply_noalloc", "wa_multi_create", "wa_multi_destroy", "wa_multi_info",
+    "wa_multi_solve",
 ]
 
 _lib = None
@@ -127,6 +128,11 @@ def load():
     L.wa_precond_create.argtypes = [P, i64, i32, C.POINTER(P)]
     L.wa_solver_params.argtypes = [P, C.POINTER(Params)]
     L.wa_precond_work.argtypes = [P, P, P]
+    L.wa_multi_create.argtypes = [P, i32, C.POINTER(Params), C.POINTER(P)]
+    L.wa_multi_destroy.argtypes = [P]
+    L.wa_multi_destroy.restype = None
+    L.wa_multi_info.argtypes = [P, i32, P]
+    L.wa_multi_solve.argtypes = [P, P, P, P, P, P, P, P, P]
     _lib = L
     return L
 

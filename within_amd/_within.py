@@ -526,3 +526,95 @@ def solve_batch(categories, Y, config=None, weights=None, preconditioner=None):
     r = s.solve_batch(Y)
     r.time_total = time.perf_counter() - t0
     return r
+
+
+class ProblemBatch:
+    """Many independent fixed-effects problems solved together on the GPU.
+
+    Each problem is what ``Solver(categories_i, config, weights_i,
+    preconditioner)`` would solve (crates/within/src/solver.rs:82-447); the
+    problems with the same number of factors are stacked block-diagonally and
+    share every operator and preconditioner launch, while each right-hand side
+    of each problem keeps its own CG recurrence, tolerance and convergence
+    flag.  CG only, with additive Schwarz or no preconditioner.  The
+    reference has no such entry point: there each problem is its own Solver.
+    """
+
+    def __init__(self, problems, config=None, weights=None, preconditioner=None):
+        t0 = time.perf_counter()
+        L = N.load()
+        if config is None:
+            config = CG()
+        if not isinstance(config, CG):
+            raise TypeError("a ProblemBatch is solved with CG")
+        problems = list(problems)
+        if not problems:
+            raise ValueError("a ProblemBatch needs at least one problem")
+        if weights is None:
+            weights = [None] * len(problems)
+        if len(weights) != len(problems):
+            raise ValueError("weights must have one entry per problem")
+        self._designs = [_Design(c, w) for c, w in zip(problems, weights)]
+        p, _ = _params(config, preconditioner)
+        arr = (C.c_void_p * len(self._designs))(*[d.h.value for d in self._designs])
+        h = C.c_void_p()
+        N.check(L.wa_multi_create(arr, len(self._designs), C.byref(p), C.byref(h)))
+        self._destroy = L.wa_multi_destroy
+        self._h = h
+        self.time_setup = time.perf_counter() - t0
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self, "_destroy", None):
+            self._destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return len(self._designs)
+
+    def info(self, group=0):
+        out = np.zeros(16, dtype=np.int64)
+        N.check(N.load().wa_multi_info(self._h, int(group), N.ptr(out)))
+        return out
+
+    def solve(self, Ys):
+        """``Ys[i]``: (n_obs_i,) or (n_obs_i, k_i); returns one
+        BatchSolveResult per problem."""
+        t0 = time.perf_counter()
+        P = len(self._designs)
+        Ys = list(Ys)
+        if len(Ys) != P:
+            raise ValueError(f"expected {P} right-hand-side arrays, got {len(Ys)}")
+        ys, xs, ds, sts = [], [], [], []
+        for d, Y in zip(self._designs, Ys):
+            Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))
+            if Y.ndim == 1:
+                Y = Y[:, None]
+            if Y.ndim != 2 or Y.shape[0] != d.n_obs:
+                raise ValueError(f"Y must have shape ({d.n_obs}, k), got {Y.shape}")
+            k = Y.shape[1]
+            ys.append(Y)
+            xs.append(np.zeros((d.n_dofs, k), order="F"))
+            ds.append(np.zeros((d.n_obs, k), order="F"))
+            sts.append((N.RhsStats * max(k, 1))())
+        PP = C.c_void_p * P
+        I64 = C.c_int64 * P
+        N.check(N.load().wa_multi_solve(
+            self._h, PP(*[y.ctypes.data for y in ys]), (C.c_int32 * P)(*[y.shape[1] for y in ys]),
+            I64(*[d.n_obs for d in self._designs]), PP(*[x.ctypes.data for x in xs]),
+            I64(*[d.n_dofs for d in self._designs]), PP(*[x.ctypes.data for x in ds]),
+            I64(*[d.n_obs for d in self._designs]), PP(*[C.addressof(st) for st in sts])))
+        total = time.perf_counter() - t0
+        out = []
+        for y, x, dm, st in zip(ys, xs, ds, sts):
+            k = y.shape[1]
+            out.append(BatchSolveResult(x=x, demeaned=dm, converged=[bool(st[i].converged) for i in range(k)],
+                                        iterations=[int(st[i].iterations) for i in range(k)],
+                                        residual=[float(st[i].residual) for i in range(k)],
+                                        time_solve=[float(st[i].time_solve) for i in range(k)],
+                                        time_total=total))
+        return out
+
+
+def solve_many(problems, Ys, config=None, weights=None, preconditioner=None):
+    """One-shot solve of many independent problems (see ``ProblemBatch``)."""
+    return ProblemBatch(problems, config=config, weights=weights, preconditioner=preconditioner).solve(Ys)

@@ -443,6 +443,7 @@ __global__ void k_cg_start(ColState* __restrict__ s, const double* __restrict__ 
   q.iters = 0;
   q.alpha = 0.0;
   q.beta = 0.0;
+  q.nan_stop = 0;
   if (c >= k) { q.active = 0; q.converged = 1; return; }
   // zero-RHS shortcut: ||b|| < EPS -> x = 0, converged, 0 iterations
   if (bn < 2.220446049250313e-16) { q.active = 0; q.converged = 1; return; }
@@ -507,9 +508,12 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict_
     q.converged = 1;
   } else if (isnan(q.rnorm)) {
     // a NaN residual has reached x and no later step can remove it: stop
-    // (not converged) instead of running out the iteration budget
+    // (not converged) instead of running out the iteration budget; the host
+    // reports the budget as spent, as the reference's loop does (cg.rs:75-125
+    // keeps iterating on NaN and returns maxiter)
     q.active = 0;
     q.converged = 0;
+    q.nan_stop = 1;
   }
 }
 

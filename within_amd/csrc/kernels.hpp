@@ -122,7 +122,7 @@ struct PrecondWork {
 // per-column Krylov state (FP64 scalars)
 struct ColState {
   double bnorm, rnorm, tol, rz, rz0, alpha, beta, pad;
-  int active, converged, iters, pad2;
+  int active, converged, iters, nan_stop;  // nan_stop: stopped on a NaN residual
 };
 
 // per-column LSMR state (schwarz-precond/src/solve/lsmr/bidiag.rs,

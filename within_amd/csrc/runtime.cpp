@@ -4,6 +4,7 @@
 // (crates/within/src/solver.rs:158-347) for k right-hand sides per block.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 
 #include "../../include/within_amd.h"
 #include "kernels.hpp"
+#include "multi.hpp"
 #include "plan.hpp"
 
 using namespace wa;
@@ -106,6 +108,7 @@ struct wa_solver {
   int64_t mu = 0;            // user DOFs (every level of every factor)
   const uint32_t* int_of = nullptr;  // user DOF -> internal (UINT32_MAX = inactive)
   const uint32_t* ext_of = nullptr;  // internal -> user DOF
+  std::vector<uint32_t> h_int_of, h_ext_of;  // host copies of the two maps
   int Q = 0;
   int dev = 0;                // the device the solver's state lives on
   // design
@@ -360,6 +363,8 @@ void wa_solver::setup_device() {
   Q = d.Q;
   HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   int_of = upload(lay.int_of);
+  h_int_of = lay.int_of;
+  h_ext_of = lay.ext_of;
   ext_of = upload(lay.ext_of);
   // design (original observation order) for D x and demeaning: the internal
   // DOF of every (factor, observation), so the factor offsets are zero
@@ -932,7 +937,7 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
   HIPCHK(hipMemcpyAsync(h.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int c = 0; c < k; ++c) {
-    if (h[c].active) {  // maxiter reached
+    if (h[c].active || h[c].nan_stop) {  // maxiter reached, or a NaN residual (which would run it out)
       conv[c] = 0;
       its[c] = params.maxiter;
     } else {
@@ -1316,6 +1321,12 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       bool anyc = false;
       for (int c = 0; c < kb; ++c) {
         if (live[c] && (rn[c] <= abs_tol[c] || rn[c] < F64_EPS)) live[c] = 0;
+        if (live[c] && std::isnan(rn[c])) {
+          // a NaN residual: every further pass would run out its budget, as the
+          // FP64 path's initial solve and max_refinements corrections do
+          live[c] = 0;
+          its[c] = (1 + std::max<int64_t>(params.max_refinements, 0)) * params.maxiter;
+        }
         if (live[c]) {
           anyc = true;
           scale[c] = 1.0 / rn[c];
@@ -1507,6 +1518,312 @@ static wa_status rmatvec_wdt_impl(wa_solver* s, bool device, const double* R, in
 }
 
 // ---------------------------------------------------------------------------
+// Many independent problems (wa_multi_*).  The problems with the same number
+// of factors are stacked block-diagonally into one design -- factor q's levels
+// of problem p follow those of problem p - 1 and the observations are
+// concatenated -- and one solver serves the stack: its factor-pair subdomains
+// are connected components, which never join two problems (no observation
+// links them), so each problem gets the subdomains it would get alone.  The
+// Krylov scalars are kept per (problem, column) pair (multi.hip): every
+// problem follows the CG recurrence of its own solve (Solver::solve_batch,
+// solver.rs:242-277), while the operator and preconditioner applies serve
+// all problems in one launch each.
+// ---------------------------------------------------------------------------
+struct MultiGroup {
+  std::unique_ptr<wa_solver> s;
+  std::vector<int> probs;                       // global problem index of each stacked problem
+  std::vector<int64_t> obs_off, nobs, ndofs, map_off;
+  const uint32_t* dmap = nullptr;               // per problem: its user DOF -> stacked internal row
+  SegIndex ix;
+  double *partA = nullptr, *partB = nullptr, *pairv = nullptr, *alpha = nullptr, *beta = nullptr;
+  double *tolv = nullptr, *scale = nullptr, *scale2 = nullptr;
+  int *mask = nullptr, *msk = nullptr, *nact = nullptr, *kcols = nullptr;
+  ColState* cs = nullptr;
+};
+
+struct wa_multi {
+  std::mutex mu;
+  int n_problems = 0;
+  std::vector<MultiGroup> groups;
+  std::vector<int64_t> nobs, ndofs;  // per global problem
+};
+
+namespace {
+
+template <typename T>
+void h2d(hipStream_t st, T* dst, const std::vector<T>& v) {
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+int read_int(wa_solver* s, const int* d) {
+  HIPCHK(hipMemcpyAsync(s->hbuf, d, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return *(int*)s->hbuf;
+}
+
+// per-pair column norms of a stacked DOF block
+std::vector<double> seg_norms(wa_solver* s, MultiGroup& g, const double* V, int kp) {
+  const int np = g.ix.P * kp;
+  launch_seg_dot<double>(s->st, V, nullptr, g.ix, kp, g.partA);
+  launch_seg_totals(s->st, g.partA, g.ix, kp, g.pairv);
+  std::vector<double> out(np);
+  HIPCHK(hipMemcpyAsync(out.data(), g.pairv, sizeof(double) * np, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (double& v : out) v = std::sqrt(v);
+  return out;
+}
+
+// block CG with one recurrence per (problem, column) pair (cg.rs:45-133 per pair)
+template <typename T>
+void seg_cg(wa_solver* s, MultiGroup& g, int kp, const T* rhs, T* x, const std::vector<double>& tol,
+            std::vector<int>& conv, std::vector<int64_t>& its) {
+  hipStream_t st = s->st;
+  const int np = g.ix.P * kp;
+  T *r = s->vec<T>(1), *z = s->vec<T>(2), *p = s->vec<T>(3), *ap = s->vec<T>(4);
+  const int64_t bytes = sizeof(T) * s->m * kp;
+  h2d(st, g.tolv, tol);
+  HIPCHK(hipMemsetAsync(x, 0, bytes, st));
+  HIPCHK(hipMemcpyAsync(r, rhs, bytes, hipMemcpyDeviceToDevice, st));
+  s->precondition<T>(r, z, kp);
+  HIPCHK(hipMemcpyAsync(p, z, bytes, hipMemcpyDeviceToDevice, st));
+  launch_seg_dot<T>(st, r, z, g.ix, kp, g.partA);
+  launch_seg_dot<T>(st, rhs, nullptr, g.ix, kp, g.partB);
+  launch_seg_start(st, g.cs, g.partB, g.partA, g.ix, g.tolv, g.kcols, kp);
+  launch_seg_count(st, g.cs, g.ix, kp, g.nact);
+  int nact = read_int(s, g.nact);
+  int64_t itn = 0;
+  while (nact > 0 && itn < s->params.maxiter) {
+    ++itn;
+    s->op_apply<T>(p, ap, kp);
+    launch_seg_dot<T>(st, p, ap, g.ix, kp, g.partA);
+    launch_seg_alpha(st, g.cs, g.partA, g.ix, g.alpha, (int)itn, kp);
+    launch_seg_update<T>(st, x, p, r, ap, g.alpha, g.ix, kp, g.partA);
+    launch_seg_check(st, g.cs, g.partA, g.ix, (int)itn, kp);
+    s->precondition<T>(r, z, kp);
+    launch_seg_dot<T>(st, r, z, g.ix, kp, g.partA);
+    launch_seg_beta(st, g.cs, g.partA, g.ix, g.beta, g.mask, g.nact, (int)itn, kp);
+    launch_seg_xpby<T>(st, p, z, g.beta, g.mask, g.ix, kp);
+    nact = read_int(s, g.nact);
+  }
+  std::vector<ColState> h(np);
+  HIPCHK(hipMemcpyAsync(h.data(), g.cs, sizeof(ColState) * np, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int j = 0; j < np; ++j) {
+    if (h[j].active || h[j].nan_stop) {
+      conv[j] = 0;
+      its[j] = s->params.maxiter;
+    } else {
+      conv[j] = h[j].converged;
+      its[j] = h[j].iters;
+    }
+  }
+}
+
+// One column block of the stack (Yr holds it): Solver::solve + refinement
+// (solver.rs:158-239, 296-347) per pair.  Leaves x in X64 and the final
+// demeaned block in E; the final residual is observation-space.
+void seg_solve_block(wa_solver* s, MultiGroup& g, int kp, const std::vector<int>& kc, std::vector<int>& conv,
+                     std::vector<int64_t>& its, std::vector<double>& res) {
+  hipStream_t st = s->st;
+  const wa_params& prm = s->params;
+  const int P = g.ix.P, np = P * kp;
+  const int64_t m = s->m, vbytes = sizeof(double) * m * kp;
+  h2d(st, g.kcols, kc);
+  s->rhs64(s->Yr, s->B64, kp);
+  const std::vector<double> bn = seg_norms(s, g, s->B64, kp);
+  std::vector<double> rhs_norm(np), abs_tol(np);
+  std::vector<int> live(np, 0);
+  for (int j = 0; j < np; ++j) {
+    rhs_norm[j] = std::max(bn[j], 1e-15);
+    abs_tol[j] = prm.tol * rhs_norm[j];
+    live[j] = (j % kp) < kc[j / kp];
+  }
+  conv.assign(np, 1);
+  its.assign(np, 0);
+  std::vector<int> cconv(np, 0);
+  std::vector<int64_t> cits(np, 0);
+  auto demean_resid = [&]() {
+    launch_demean(st, s->n, s->Q, s->codes, s->offs, s->X64, s->Yr, s->E, kp);
+    s->rhs64(s->E, s->R64, kp);
+  };
+  const bool mixed = prm.precision == 1;
+  if (!mixed) {
+    seg_cg<double>(s, g, kp, s->B64, s->cx64, std::vector<double>(np, prm.tol), conv, its);
+    HIPCHK(hipMemcpyAsync(s->X64, s->cx64, vbytes, hipMemcpyDeviceToDevice, st));
+    for (int64_t it = 0; it < prm.max_refinements; ++it) {
+      demean_resid();
+      const std::vector<double> cn = seg_norms(s, g, s->R64, kp);
+      std::vector<double> scale(np, 0.0), ctol(np, 1.0);
+      bool any = false;
+      for (int j = 0; j < np; ++j) {
+        if (live[j] && cn[j] <= abs_tol[j]) live[j] = 0;
+        if (live[j]) {
+          any = true;
+          scale[j] = 1.0;
+          ctol[j] = std::min(abs_tol[j] / cn[j], 1.0);
+        }
+      }
+      if (!any) break;
+      h2d(st, g.scale, scale);
+      launch_seg_cast_scale<double, double>(st, s->R64, s->T64, g.scale, g.ix, kp);
+      seg_cg<double>(s, g, kp, s->T64, s->cx64, ctol, cconv, cits);
+      h2d(st, g.msk, live);
+      launch_seg_axpy_acc<double>(st, s->X64, s->cx64, nullptr, g.msk, g.ix, kp);
+      for (int j = 0; j < np; ++j)
+        if (live[j]) {
+          its[j] += cits[j];
+          conv[j] = cconv[j];
+        }
+    }
+  } else {
+    HIPCHK(hipMemsetAsync(s->X64, 0, vbytes, st));
+    const int64_t max_outer = std::max<int64_t>(prm.max_refinements, 0) + 8;
+    for (int64_t outer = 0; outer <= max_outer; ++outer) {
+      std::vector<double> rn;
+      if (outer == 0) {
+        HIPCHK(hipMemcpyAsync(s->R64, s->B64, vbytes, hipMemcpyDeviceToDevice, st));
+        rn = bn;
+      } else {
+        demean_resid();
+        rn = seg_norms(s, g, s->R64, kp);
+      }
+      std::vector<double> scale(np, 0.0), itol(np, 1.0);
+      std::vector<int> msk(np, 0);
+      bool any = false;
+      for (int j = 0; j < np; ++j) {
+        if (live[j] && (rn[j] <= abs_tol[j] || rn[j] < F64_EPS)) live[j] = 0;
+        if (live[j] && std::isnan(rn[j])) {
+          live[j] = 0;
+          its[j] = (1 + std::max<int64_t>(prm.max_refinements, 0)) * prm.maxiter;
+        }
+        if (live[j]) {
+          any = true;
+          scale[j] = 1.0 / rn[j];
+          msk[j] = 1;
+          itol[j] = std::min(1.0, std::max(abs_tol[j] / rn[j], inner_floor()));
+        }
+      }
+      if (!any || outer == max_outer) break;
+      h2d(st, g.scale, scale);
+      launch_seg_cast_scale<double, float>(st, s->R64, s->rhs32, g.scale, g.ix, kp);
+      seg_cg<float>(s, g, kp, s->rhs32, s->cx32, itol, cconv, cits);
+      for (int j = 0; j < np; ++j) scale[j] = msk[j] ? rn[j] : 0.0;
+      h2d(st, g.scale2, scale);
+      h2d(st, g.msk, msk);
+      launch_seg_axpy_acc<float>(st, s->X64, s->cx32, g.scale2, g.msk, g.ix, kp);
+      for (int j = 0; j < np; ++j)
+        if (msk[j]) its[j] += cits[j];
+    }
+  }
+  demean_resid();
+  const std::vector<double> fr = seg_norms(s, g, s->R64, kp);
+  res.assign(np, 0.0);
+  for (int j = 0; j < np; ++j) {
+    res[j] = fr[j] / rhs_norm[j];
+    if (mixed) conv[j] = res[j] <= prm.tol;
+  }
+}
+
+// stack the designs of one group into one block-diagonal design
+Design stack_designs(const std::vector<const Design*>& ds, std::vector<std::vector<int64_t>>& lev_off) {
+  const int Q = ds[0]->Q;
+  Design d;
+  d.Q = Q;
+  d.codes.resize(Q);
+  lev_off.assign(ds.size(), std::vector<int64_t>(Q, 0));
+  std::vector<int64_t> lev(Q, 0);
+  bool weighted = false;
+  for (size_t p = 0; p < ds.size(); ++p) {
+    d.n_obs += ds[p]->n_obs;
+    weighted = weighted || !ds[p]->weights.empty();
+    for (int q = 0; q < Q; ++q) {
+      lev_off[p][q] = lev[q];
+      lev[q] += ds[p]->levels[q];
+    }
+  }
+  if (d.n_obs >= (int64_t)UINT32_MAX) throw Error(WA_ERR_OVERFLOW, "Overflow: stacked observations exceed u32 indexing");
+  int64_t off = 0;
+  for (int q = 0; q < Q; ++q) {
+    d.codes[q].reserve(d.n_obs);
+    for (size_t p = 0; p < ds.size(); ++p)
+      for (u32 c : ds[p]->codes[q]) d.codes[q].push_back((u32)(c + lev_off[p][q]));
+    d.levels.push_back(lev[q]);
+    d.offset.push_back(off);
+    off += lev[q];
+  }
+  if (off >= (int64_t)UINT32_MAX) throw Error(WA_ERR_OVERFLOW, "Overflow: stacked DOFs exceed u32 indexing");
+  d.n_dofs = off;
+  if (weighted) {
+    d.weights.reserve(d.n_obs);
+    for (const Design* x : ds)
+      for (int64_t i = 0; i < x->n_obs; ++i) d.weights.push_back(x->w(i));
+  }
+  return d;
+}
+
+// tiles of the internal DOF rows (one problem each) and the per-problem maps
+void build_group_index(MultiGroup& g, const std::vector<const Design*>& ds,
+                       const std::vector<std::vector<int64_t>>& lev_off) {
+  wa_solver* s = g.s.get();
+  const Design& d = s->plan.d;
+  const int P = (int)ds.size(), Q = d.Q;
+  auto prob_of = [&](uint32_t u) {
+    int q = Q - 1;
+    while (q > 0 && (int64_t)u < d.offset[q]) --q;
+    const int64_t lvl = (int64_t)u - d.offset[q];
+    int lo = 0, hi = P - 1;  // last problem whose level offset is <= lvl
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (lev_off[mid][q] <= lvl) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  std::vector<SegTile> tiles;
+  std::vector<std::vector<uint32_t>> per(P);
+  for (int64_t i = 0; i < s->m;) {
+    const uint32_t p = (uint32_t)prob_of(s->h_ext_of[i]);
+    int64_t j = i + 1;
+    while (j < s->m && j - i < (int64_t)kSegTileRows && (uint32_t)prob_of(s->h_ext_of[j]) == p) ++j;
+    per[p].push_back((uint32_t)tiles.size());
+    tiles.push_back({(uint32_t)i, (uint32_t)j, p, 0u});
+    i = j;
+  }
+  std::vector<uint32_t> pt_ptr(1, 0), pt_idx;
+  for (int p = 0; p < P; ++p) {
+    pt_idx.insert(pt_idx.end(), per[p].begin(), per[p].end());
+    pt_ptr.push_back((uint32_t)pt_idx.size());
+  }
+  std::vector<uint32_t> dmap;
+  for (int p = 0; p < P; ++p) {
+    g.map_off.push_back((int64_t)dmap.size());
+    for (int q = 0; q < Q; ++q)
+      for (int64_t l = 0; l < ds[p]->levels[q]; ++l) dmap.push_back(s->h_int_of[d.offset[q] + lev_off[p][q] + l]);
+  }
+  g.dmap = s->upload(dmap);
+  g.ix.tiles = s->upload(tiles);
+  g.ix.n_tiles = (int64_t)tiles.size();
+  g.ix.pt_ptr = s->upload(pt_ptr);
+  g.ix.pt_idx = s->upload(pt_idx);
+  g.ix.P = P;
+  const int64_t nt = std::max<int64_t>(g.ix.n_tiles, 1), npm = (int64_t)P * KMAX;
+  g.partA = s->dalloc<double>(nt * KMAX);
+  g.partB = s->dalloc<double>(nt * KMAX);
+  g.pairv = s->dalloc<double>(npm);
+  g.alpha = s->dalloc<double>(npm);
+  g.beta = s->dalloc<double>(npm);
+  g.tolv = s->dalloc<double>(npm);
+  g.scale = s->dalloc<double>(npm);
+  g.scale2 = s->dalloc<double>(npm);
+  g.mask = s->dalloc<int>(npm);
+  g.msk = s->dalloc<int>(npm);
+  g.nact = s->dalloc<int>(1);
+  g.kcols = s->dalloc<int>(P);
+  g.cs = s->dalloc<ColState>(npm);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Allocation-free implicit operator apply (wa_gram_apply_noalloc): its scratch
 // -- the row-major copies of a column block of X and of G X, and the
 // split-row partial sums of the factor passes -- lives in one file-scope
@@ -1596,58 +1913,64 @@ static void local_from_cfg(const LocalConfig& c, wa_local_config& l) {
   l.dense_core = c.dense_core;
 }
 
+// a solver for a design, optionally with a serialized preconditioner
+static std::unique_ptr<wa_solver> create_solver(const Design& dd, const wa_params* p, const uint8_t* pre,
+                                                int64_t pre_len) {
+  if (p->krylov < 0 || p->krylov > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
+  if (p->krylov == 2 && p->preconditioner == 2)
+    throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR requires a symmetric preconditioner; use additive or GMRES");
+  if (p->local_size < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR local_size must be >= 0");
+  if (p->preconditioner < 0 || p->preconditioner > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown preconditioner");
+  if (p->preconditioner == 2 && p->krylov == 0)
+    throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
+  if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
+  if (dd.Q > 8) throw Error(WA_ERR_UNSUPPORTED, "at most 8 factors are supported");
+  if (p->local.ac_split < 1 || p->local.schur_split < 1)
+    throw Error(WA_ERR_INVALID_ARGUMENT, "split must be >= 1");
+  auto s = std::make_unique<wa_solver>();
+  s->params = *p;
+  s->host_only = p->host_only != 0;
+  Plan& pl = s->plan;
+  pl.d = dd;
+  pl.cfg.ac_seed = p->local.ac_seed;
+  pl.cfg.ac_split = p->local.ac_split;
+  pl.cfg.approx_schur = p->local.approx_schur != 0;
+  pl.cfg.schur_seed = p->local.schur_seed;
+  pl.cfg.schur_split = p->local.schur_split;
+  pl.cfg.dense_threshold = p->local.dense_threshold;
+  pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
+  pl.cfg.device = !s->host_only;
+  pl.precond = p->preconditioner;
+  pl.explicit_op = p->operator_repr == 1;
+  if (pre) {
+    const int64_t nd = deserialize_precond(pre, pre_len, pl);
+    if (nd != pl.d.n_dofs)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner was built for a design with a different number of DOFs");
+    if (pl.precond == 2 && p->krylov != 1)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "CG and LSMR require a symmetric preconditioner; use additive or GMRES");
+    std::vector<char> seen(pl.d.n_dofs, 0);
+    for (int q = 0; q < pl.d.Q; ++q)
+      for (u32 c : pl.d.codes[q]) seen[pl.d.offset[q] + c] = 1;
+    for (u32 g : pl.pre.loc_gidx)
+      if (!seen[g])
+        throw Error(WA_ERR_INVALID_ARGUMENT,
+                    "the preconditioner does not match the design (it covers an unobserved level)");
+    if (pl.precond == 2) pl.gram = Csr(), pl.has_gram = false;  // rebuilt from the design
+    s->params.preconditioner = pl.precond;
+    local_from_cfg(pl.cfg, s->params.local);
+  }
+  build_plan(pl);
+  if (!s->host_only) s->setup_device();
+  return s;
+}
+
 // solver from a design, optionally with a serialized preconditioner
 static wa_status create_impl(const wa_design* d, const wa_params* p, const uint8_t* pre, int64_t pre_len,
                              wa_solver** out) {
   try {
     if (!d || !p || !out) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (p->krylov < 0 || p->krylov > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown Krylov method");
-    if (p->krylov == 2 && p->preconditioner == 2)
-      throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR requires a symmetric preconditioner; use additive or GMRES");
-    if (p->local_size < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "LSMR local_size must be >= 0");
-    if (p->preconditioner < 0 || p->preconditioner > 2) throw Error(WA_ERR_INVALID_ARGUMENT, "unknown preconditioner");
-    if (p->preconditioner == 2 && p->krylov == 0)
-      throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
-    if (p->krylov == 1 && p->restart <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "GMRES restart must be positive");
-    if (d->d.Q > 8) throw Error(WA_ERR_UNSUPPORTED, "at most 8 factors are supported");
-    if (p->local.ac_split < 1 || p->local.schur_split < 1)
-      throw Error(WA_ERR_INVALID_ARGUMENT, "split must be >= 1");
-    auto s = std::make_unique<wa_solver>();
-    s->params = *p;
-    s->host_only = p->host_only != 0;
-    Plan& pl = s->plan;
-    pl.d = d->d;
-    pl.cfg.ac_seed = p->local.ac_seed;
-    pl.cfg.ac_split = p->local.ac_split;
-    pl.cfg.approx_schur = p->local.approx_schur != 0;
-    pl.cfg.schur_seed = p->local.schur_seed;
-    pl.cfg.schur_split = p->local.schur_split;
-    pl.cfg.dense_threshold = p->local.dense_threshold;
-    pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
-    pl.cfg.device = !s->host_only;
-    pl.precond = p->preconditioner;
-    pl.explicit_op = p->operator_repr == 1;
-    if (pre) {
-      const int64_t nd = deserialize_precond(pre, pre_len, pl);
-      if (nd != pl.d.n_dofs)
-        throw Error(WA_ERR_INVALID_ARGUMENT, "preconditioner was built for a design with a different number of DOFs");
-      if (pl.precond == 2 && p->krylov != 1)
-        throw Error(WA_ERR_INVALID_ARGUMENT, "CG and LSMR require a symmetric preconditioner; use additive or GMRES");
-      std::vector<char> seen(pl.d.n_dofs, 0);
-      for (int q = 0; q < pl.d.Q; ++q)
-        for (u32 c : pl.d.codes[q]) seen[pl.d.offset[q] + c] = 1;
-      for (u32 g : pl.pre.loc_gidx)
-        if (!seen[g])
-          throw Error(WA_ERR_INVALID_ARGUMENT,
-                      "the preconditioner does not match the design (it covers an unobserved level)");
-      if (pl.precond == 2) pl.gram = Csr(), pl.has_gram = false;  // rebuilt from the design
-      s->params.preconditioner = pl.precond;
-      local_from_cfg(pl.cfg, s->params.local);
-    }
-    build_plan(pl);
-    if (!s->host_only) s->setup_device();
-    *out = s.release();
+    *out = create_solver(d->d, p, pre, pre_len).release();
     return WA_OK;
   } catch (const std::exception& e) {
     return (wa_status)set_error(e);
@@ -1858,6 +2181,149 @@ wa_status wa_rmatvec_wdt(wa_solver* s, const double* R, int64_t ldr, double* X, 
 }
 wa_status wa_rmatvec_wdt_device(wa_solver* s, const double* R, int64_t ldr, double* X, int64_t ldx, int32_t k) {
   return rmatvec_wdt_impl(s, true, R, ldr, X, ldx, k);
+}
+
+wa_status wa_multi_create(const wa_design* const* designs, int32_t n_problems, const wa_params* p,
+                          wa_multi** out) {
+  try {
+    if (!designs || !p || !out || n_problems <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
+    *out = nullptr;
+    if (p->krylov != 0)
+      throw Error(WA_ERR_UNSUPPORTED, "a batch of problems is solved with CG (KrylovMethod::Cg) only");
+    if (p->preconditioner == 2)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "CG requires a symmetric preconditioner; use additive or GMRES");
+    if (p->host_only) throw Error(WA_ERR_INVALID_ARGUMENT, "a batch of problems needs the device");
+    auto mp = std::make_unique<wa_multi>();
+    mp->n_problems = n_problems;
+    std::vector<int> qs;  // factor counts in order of first appearance
+    for (int i = 0; i < n_problems; ++i) {
+      if (!designs[i]) throw Error(WA_ERR_INVALID_ARGUMENT, "null design");
+      mp->nobs.push_back(designs[i]->d.n_obs);
+      mp->ndofs.push_back(designs[i]->d.n_dofs);
+      if (std::find(qs.begin(), qs.end(), designs[i]->d.Q) == qs.end()) qs.push_back(designs[i]->d.Q);
+    }
+    for (int Q : qs) {
+      MultiGroup g;
+      std::vector<const Design*> ds;
+      for (int i = 0; i < n_problems; ++i)
+        if (designs[i]->d.Q == Q) {
+          g.probs.push_back(i);
+          ds.push_back(&designs[i]->d);
+        }
+      std::vector<std::vector<int64_t>> lev_off;
+      const Design stacked = stack_designs(ds, lev_off);
+      int64_t o = 0;
+      for (const Design* d : ds) {
+        g.obs_off.push_back(o);
+        g.nobs.push_back(d->n_obs);
+        g.ndofs.push_back(d->n_dofs);
+        o += d->n_obs;
+      }
+      g.s = create_solver(stacked, p, nullptr, 0);
+      build_group_index(g, ds, lev_off);
+      mp->groups.push_back(std::move(g));
+    }
+    *out = mp.release();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+void wa_multi_destroy(wa_multi* mp) { delete mp; }
+
+wa_status wa_multi_info(const wa_multi* mp, int32_t group, int64_t* info16) {
+  if (!mp || !info16 || group < 0 || group >= (int32_t)mp->groups.size()) {
+    g_err = "bad group index";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const wa_status rc = wa_solver_info(mp->groups[group].s.get(), info16);
+  if (rc == WA_OK) info16[15] = (int64_t)mp->groups[group].probs.size();
+  return rc;
+}
+
+wa_status wa_multi_solve(wa_multi* mp, const double* const* Y, const int32_t* k, const int64_t* ldy,
+                         double* const* X, const int64_t* ldx, double* const* demeaned, const int64_t* ldd,
+                         wa_rhs_stats* const* stats) {
+  try {
+    if (!mp || !Y || !k || !ldy || !X || !ldx || !stats) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (demeaned && !ldd) throw Error(WA_ERR_INVALID_ARGUMENT, "demeaned needs leading dimensions");
+    for (int i = 0; i < mp->n_problems; ++i) {
+      if (k[i] < 0) throw Error(WA_ERR_INVALID_ARGUMENT, "k must be >= 0");
+      if (k[i] == 0) continue;
+      if (!Y[i] || !X[i] || !stats[i]) throw Error(WA_ERR_INVALID_ARGUMENT, "null buffer");
+      if (ldy[i] < mp->nobs[i] || ldx[i] < mp->ndofs[i] || (demeaned && demeaned[i] && ldd[i] < mp->nobs[i]))
+        throw Error(WA_ERR_OBS_COUNT_MISMATCH, "ObservationCountMismatch: leading dimension too small");
+    }
+    std::lock_guard<std::mutex> lock(mp->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (MultiGroup& g : mp->groups) {
+      wa_solver* s = g.s.get();
+      std::lock_guard<std::mutex> lock2(s->call_mu);
+      hipStream_t st = s->st;
+      const int P = (int)g.probs.size();
+      int kmax = 0;
+      int64_t out_rows = 0;  // every problem's x and demeaned rows, one after another
+      for (int j = 0; j < P; ++j) {
+        kmax = std::max(kmax, k[g.probs[j]]);
+        out_rows += g.nobs[j] + g.ndofs[j];
+      }
+      for (int c0 = 0; c0 < kmax; c0 += KMAX) {
+        const int kb = std::min(KMAX, kmax - c0), kp = pow2_at_least(kb);
+        std::vector<int> kc(P);
+        for (int j = 0; j < P; ++j) kc[j] = std::max(0, std::min(kb, k[g.probs[j]] - c0));
+        // stage this column block of every problem as one stacked column-major block
+        double* stg = s->staging(std::max(s->n, out_rows) * (int64_t)kb);
+        HIPCHK(hipMemsetAsync(stg, 0, sizeof(double) * s->n * kb, st));
+        for (int j = 0; j < P; ++j) {
+          const int i = g.probs[j];
+          if (kc[j] == 0) continue;
+          HIPCHK(hipMemcpy2DAsync(stg + g.obs_off[j], sizeof(double) * s->n, Y[i] + (int64_t)c0 * ldy[i],
+                                  sizeof(double) * ldy[i], sizeof(double) * g.nobs[j], (size_t)kc[j],
+                                  hipMemcpyDefault, st));
+        }
+        launch_col_to_row(st, stg, s->n, s->n, kb, s->Yr, kp);
+        std::vector<int> conv;
+        std::vector<int64_t> its;
+        std::vector<double> res;
+        seg_solve_block(s, g, kp, kc, conv, its, res);
+        // the outputs reuse the staging block (stream order: the input was consumed)
+        double* o_next = stg;
+        for (int j = 0; j < P; ++j) {
+          const int i = g.probs[j];
+          if (kc[j] == 0) continue;
+          double* xo = o_next;
+          double* dm = xo + g.ndofs[j] * kc[j];
+          o_next = dm + g.nobs[j] * kc[j];
+          launch_row_to_col(st, s->X64, kp, g.ndofs[j], kc[j], xo, g.ndofs[j], g.dmap + g.map_off[j]);
+          HIPCHK(hipMemcpy2DAsync(X[i] + (int64_t)c0 * ldx[i], sizeof(double) * ldx[i], xo,
+                                  sizeof(double) * g.ndofs[j], sizeof(double) * g.ndofs[j], (size_t)kc[j],
+                                  hipMemcpyDefault, st));
+          if (demeaned && demeaned[i]) {
+            launch_row_to_col(st, s->E + g.obs_off[j] * kp, kp, g.nobs[j], kc[j], dm, g.nobs[j]);
+            HIPCHK(hipMemcpy2DAsync(demeaned[i] + (int64_t)c0 * ldd[i], sizeof(double) * ldd[i], dm,
+                                    sizeof(double) * g.nobs[j], sizeof(double) * g.nobs[j], (size_t)kc[j],
+                                    hipMemcpyDefault, st));
+          }
+          for (int c = 0; c < kc[j]; ++c) {
+            wa_rhs_stats& o = stats[i][c0 + c];
+            o.converged = conv[j * kp + c];
+            o.iterations = its[j * kp + c];
+            o.residual = res[j * kp + c];
+          }
+        }
+        HIPCHK(hipStreamSynchronize(st));  // the next block reuses the staging block
+      }
+      HIPCHK(hipGetLastError());
+      if (s->timer.on) s->timer.flush();
+    }
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < mp->n_problems; ++i)
+      for (int c = 0; c < k[i]; ++c) stats[i][c].time_solve = t;
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
 }
 
 wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4) {
