@@ -161,6 +161,98 @@ def cpu_baseline(args, p, Y, D_gpu, Dref, torch):
             "rhs": kc, "converged": int(conv.sum()), "setup_s": round(setup, 2)}, err_gpu
 
 
+def bench_many(args, torch):
+    """The many-uneven-problems workload: 256 independent problems, n
+    log-uniform in [10k, 2M], 2 or 3 uniform factors (n/20, n/200, 25 levels),
+    1-4 RHS each; right-hand sides and outputs resident in HBM.  The batched
+    path (ProblemBatch: block-diagonal stacks, one launch per operator /
+    preconditioner apply for all problems) against one Solver per problem, both
+    FP32 inner CG + FP64 refinement; an RHS counts when converged and within
+    1e-8 of the other path's demeaned vector."""
+    import within_amd as W
+    from within_amd import _native as N
+    from within_amd import synth
+
+    L = N.load()
+    probs = synth.many_problems(n_problems=256, n_min=10_000, n_max=2_000_000, seed=0)
+    n_total = sum(c.shape[0] for c, _ in probs)
+    n_rhs = sum(Y.shape[1] for _, Y in probs)
+    cfg = W.CG(tol=args.tol, precision="mixed", operator=W.OperatorRepr.Explicit)
+    Yd = [torch.from_numpy(np.ascontiguousarray(Y.T)).cuda() for _, Y in probs]  # (k, n): column-major n x k
+    P = len(probs)
+    ks = [Y.shape[1] for _, Y in probs]
+    log(f"bench many: {P} problems, {n_total} observations, {n_rhs} RHS; batched setup ...")
+    t0 = time.perf_counter()
+    batch = W.ProblemBatch([c for c, _ in probs], config=cfg)
+    setup_b = time.perf_counter() - t0
+    ndofs = [d.n_dofs for d in batch._designs]
+    Xb = [torch.zeros((k, m), device="cuda", dtype=torch.float64) for k, m in zip(ks, ndofs)]
+    Db = [torch.zeros((k, c.shape[0]), device="cuda", dtype=torch.float64) for k, (c, _) in zip(ks, probs)]
+    stats = [(N.RhsStats * k)() for k in ks]
+    PP, I64 = C.c_void_p * P, C.c_int64 * P
+    args_b = (batch._h, PP(*[y.data_ptr() for y in Yd]), (C.c_int32 * P)(*ks), I64(*[c.shape[0] for c, _ in probs]),
+              PP(*[x.data_ptr() for x in Xb]), I64(*ndofs), PP(*[d.data_ptr() for d in Db]),
+              I64(*[c.shape[0] for c, _ in probs]), PP(*[C.addressof(st) for st in stats]))
+    for _ in range(args.warmup):
+        N.check(L.wa_multi_solve(*args_b))
+    tb = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        N.check(L.wa_multi_solve(*args_b))
+        tb.append(time.perf_counter() - t0)
+    t_batch = float(np.median(tb))
+    conv_b = [[bool(st[j].converged) for j in range(k)] for st, k in zip(stats, ks)]
+    its_b = [[int(st[j].iterations) for j in range(k)] for st, k in zip(stats, ks)]
+    log(f"bench many: batched setup {setup_b:.1f}s, solve {t_batch * 1e3:.1f} ms")
+    del batch
+    # one Solver per problem (the reference's unit: Solver::solve_batch per problem)
+    t0 = time.perf_counter()
+    solvers = [W.Solver(c, config=cfg) for c, _ in probs]
+    setup_s = time.perf_counter() - t0
+    Xs = [torch.zeros_like(x) for x in Xb]
+    Ds = [torch.zeros_like(d) for d in Db]
+    st1 = [(N.RhsStats * k)() for k in ks]
+
+    def run_sep():
+        for i, s in enumerate(solvers):
+            N.check(L.wa_solve_batch_device(s._h, C.c_void_p(Yd[i].data_ptr()), ks[i], probs[i][0].shape[0],
+                                            C.c_void_p(Xs[i].data_ptr()), ndofs[i], C.c_void_p(Ds[i].data_ptr()),
+                                            probs[i][0].shape[0], st1[i]))
+
+    run_sep()
+    ts = []
+    for _ in range(max(1, min(args.steps, 3))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_sep()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_sep = float(np.median(ts))
+    log(f"bench many: per-problem setup {setup_s:.1f}s, solve {t_sep * 1e3:.1f} ms")
+    ok_b = ok_s = 0
+    worst = 0.0
+    for i in range(P):
+        e = (torch.linalg.vector_norm(Db[i] - Ds[i], dim=1) / torch.linalg.vector_norm(Ds[i], dim=1)).cpu().numpy()
+        worst = max(worst, float(e.max()))
+        for j in range(ks[i]):
+            ok_b += conv_b[i][j] and e[j] <= 1e-8
+            ok_s += bool(st1[i][j].converged) and e[j] <= 1e-8
+    out = {
+        "metric": "RHS solved/sec to 1e-8, 256 uneven FE problems (n logU[10k, 2M], 2-3 factors, 1-4 RHS)",
+        "value": round(ok_b / t_batch, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner Krylov + f64 refinement", "data": "synthetic",
+        "config": {"workload": "256 uneven problems", "n_obs": n_total, "rhs": n_rhs, "problems": P},
+        "per_problem_rhs_per_s": round(ok_s / t_sep, 2), "per_problem_ms": round(t_sep * 1e3, 3),
+        "speedup_vs_per_problem": round((ok_b / t_batch) / max(ok_s / t_sep, 1e-30), 3),
+        "setup_batched_s": round(setup_b, 2), "setup_per_problem_s": round(setup_s, 2),
+        "max_rel_err_batched_vs_per_problem": worst,
+        "max_iterations": max(max(x) for x in its_b),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +271,8 @@ def main():
                     help="reduced-system size factored densely (0 = library default)")
     ap.add_argument("--profile", action="store_true",
                     help="headline solves only (no reference, baseline or CPU legs): for counter runs")
+    ap.add_argument("--workload", choices=["akm20m", "many"], default="akm20m",
+                    help="akm20m: the 20M-observation k=32 batch (the metric); many: 256 uneven problems")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP32 buffer 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -196,6 +290,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": 4 << 28}), flush=True)
+        return
+
+    if args.workload == "many":
+        bench_many(args, torch)
         return
 
     import within_amd as W
