@@ -62,6 +62,15 @@ def test_full_size_mixed_solve_meets_the_metric_bar(full, oracle_mod):
     assert all(c for c, _, _ in st)
     _, D2, _ = solve_device(s, Y)
     assert torch.equal(D, D2)  # bitwise repeatable
+    # the CG iterations replayed from a captured graph (two-stream apply inside)
+    # equal the same iterations enqueued launch by launch
+    assert int(s.info()[15]) & 0xFF == 2
+    os.environ["WITHIN_AMD_GRAPHS"] = "0"
+    try:
+        _, D3, st3 = solve_device(s, Y)
+    finally:
+        del os.environ["WITHIN_AMD_GRAPHS"]
+    assert torch.equal(D, D3) and [x[1] for x in st3] == [x[1] for x in st]
     s.set_config(W.CG(tol=1e-11, max_refinements=4, operator=W.OperatorRepr.Explicit))
     _, Dref, st_ref = solve_device(s, Y)
     assert all(c for c, _, _ in st_ref)

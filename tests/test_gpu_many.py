@@ -51,6 +51,15 @@ def test_batch_matches_oracle_and_separate_solves(precision, oracle_mod):
     batch = W.ProblemBatch([c for c, _, _ in probs], config=cfg, weights=[w for _, _, w in probs])
     assert batch.info(0)[15] + batch.info(1)[15] == len(probs)  # two stacks: 2- and 3-factor problems
     res = batch.solve([Y for _, Y, _ in probs])
+    # the per-problem CG iterations replayed from a captured graph equal the
+    # same iterations enqueued launch by launch, bit for bit
+    os.environ["WITHIN_AMD_GRAPHS"] = "0"
+    try:
+        eager = batch.solve([Y for _, Y, _ in probs])
+    finally:
+        del os.environ["WITHIN_AMD_GRAPHS"]
+    for r, e in zip(res, eager):
+        assert np.array_equal(r.demeaned, e.demeaned) and r.iterations == e.iterations
     for i, ((codes, Y, w), r) in enumerate(zip(probs, res)):
         ro = oracle_mod.OracleSolver(codes, w, ac_split=2, tol=1e-10, max_refinements=3).solve_batch(Y)
         sep = W.Solver(codes, config=W.CG(tol=1e-10, max_refinements=3), weights=w).solve_batch(Y)

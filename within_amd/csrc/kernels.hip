@@ -472,7 +472,7 @@ __device__ inline double col_total(const double* __restrict__ part, int nb, int 
 // pap or rz gives a NaN alpha, which carries into x as in cg.rs:77-86)
 __global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict__ s, const double* __restrict__ part,
                                                      int nb, double* __restrict__ pap, double* __restrict__ alpha,
-                                                     int itn, int kp) {
+                                                     int kp) {
   __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
@@ -481,10 +481,12 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict_
   ColState& q = s[c];
   alpha[c] = 0.0;
   if (!q.active) return;
+  // every active column is in its q.iters-th iteration (the count lives on
+  // the device, so a replayed iteration graph keeps it)
+  q.iters += 1;
   if (v <= 0.0) {
     q.active = 0;
     q.converged = 0;
-    q.iters = itn;
     return;
   }
   alpha[c] = q.rz / v;
@@ -493,7 +495,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_alpha(ColState* __restrict_
 
 // after the update: ||r|| / ||b|| <= tol -> converged
 __global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict__ s, const double* __restrict__ part,
-                                                     int nb, double* __restrict__ rr, int itn, int kp) {
+                                                     int nb, double* __restrict__ rr, int kp) {
   __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
@@ -502,7 +504,6 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict_
   ColState& q = s[c];
   if (!q.active) return;
   q.rnorm = sqrt(v);
-  q.iters = itn;
   if (q.rnorm / q.bnorm <= q.tol) {
     q.active = 0;
     q.converged = 1;
@@ -520,8 +521,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_check(ColState* __restrict_
 // after z = M r: stagnation guard |rz_new| < EPS^2 max(|rz0|, MIN); beta
 __global__ void __launch_bounds__(kScalarBlock) k_cg_beta(ColState* __restrict__ s, const double* __restrict__ part,
                                                     int nb, double* __restrict__ rznew, double* __restrict__ beta,
-                                                    int* __restrict__ mask, int* __restrict__ n_active, int itn,
-                                                    int kp) {
+                                                    int* __restrict__ mask, int* __restrict__ n_active, int kp) {
   __shared__ double sh[kScalarBlock];
   const double v = col_total(part, nb, kp, sh);
   const int c = threadIdx.x;
@@ -535,7 +535,6 @@ __global__ void __launch_bounds__(kScalarBlock) k_cg_beta(ColState* __restrict__
       if (fabs(v) < thr) {
         q.active = 0;
         q.converged = q.rnorm / q.bnorm <= q.tol;
-        q.iters = itn;
       } else {
         beta[c] = v / q.rz;
         q.rz = v;
@@ -555,16 +554,15 @@ void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const do
                      int kp) {
   k_cg_start<<<1, 64, 0, st>>>(s, bnorm2, rz, tol, k, kp);
 }
-void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int itn,
-                     int kp) {
-  k_cg_alpha<<<1, kScalarBlock, 0, st>>>(s, part, nb, pap, alpha, itn, kp);
+void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int kp) {
+  k_cg_alpha<<<1, kScalarBlock, 0, st>>>(s, part, nb, pap, alpha, kp);
 }
-void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int itn, int kp) {
-  k_cg_check<<<1, kScalarBlock, 0, st>>>(s, part, nb, rr, itn, kp);
+void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int kp) {
+  k_cg_check<<<1, kScalarBlock, 0, st>>>(s, part, nb, rr, kp);
 }
 void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, double* rznew, double* beta, int* mask,
-                    int* n_active, int itn, int kp) {
-  k_cg_beta<<<1, kScalarBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, itn, kp);
+                    int* n_active, int kp) {
+  k_cg_beta<<<1, kScalarBlock, 0, st>>>(s, part, nb, rznew, beta, mask, n_active, kp);
 }
 
 // ===========================================================================

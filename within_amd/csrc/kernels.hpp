@@ -193,12 +193,12 @@ void launch_axpy_acc(hipStream_t st, double* X, const S* Dl, const double* scale
                      int kp);
 void launch_cg_start(hipStream_t st, ColState* s, const double* bnorm2, const double* rz, const double* tol, int k,
                      int kp);
-// the CG scalar steps take the per-block partials of their dot product
-void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int itn,
-                     int kp);
-void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int itn, int kp);
+// the CG scalar steps take the per-block partials of their dot product; the
+// iteration count of each column is kept in its ColState (device side)
+void launch_cg_alpha(hipStream_t st, ColState* s, const double* part, int nb, double* pap, double* alpha, int kp);
+void launch_cg_check(hipStream_t st, ColState* s, const double* part, int nb, double* rr, int kp);
 void launch_cg_beta(hipStream_t st, ColState* s, const double* part, int nb, double* rznew, double* beta, int* mask,
-                    int* n_active, int itn, int kp);
+                    int* n_active, int kp);
 template <typename T>
 void launch_precond(hipStream_t st, const DevPrecond<T>& p, const T* R, T* Z, PrecondWork<T>& w, int kp,
                     int64_t m);

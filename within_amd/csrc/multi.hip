@@ -81,31 +81,29 @@ __global__ void k_seg_start(ColState* __restrict__ s, const double* __restrict__
 }
 
 __global__ void k_seg_alpha(ColState* __restrict__ s, const double* __restrict__ part, SegIndex ix,
-                            double* __restrict__ alpha, int itn, int kp) {
+                            double* __restrict__ alpha, int kp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ix.P * kp) return;
   ColState& q = s[j];
   alpha[j] = 0.0;
   if (!q.active) return;
+  q.iters += 1;  // the iteration count lives on the device (graph replays keep it)
   const double v = pair_total(part, ix, j / kp, j % kp, kp);
   if (v <= 0.0) {  // cg.rs:80-86: p'Ap <= 0 stops the column, not converged
     q.active = 0;
     q.converged = 0;
-    q.iters = itn;
     return;
   }
   alpha[j] = q.rz / v;
   q.alpha = alpha[j];
 }
 
-__global__ void k_seg_check(ColState* __restrict__ s, const double* __restrict__ part, SegIndex ix, int itn,
-                            int kp) {
+__global__ void k_seg_check(ColState* __restrict__ s, const double* __restrict__ part, SegIndex ix, int kp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ix.P * kp) return;
   ColState& q = s[j];
   if (!q.active) return;
   q.rnorm = sqrt(pair_total(part, ix, j / kp, j % kp, kp));
-  q.iters = itn;
   if (q.rnorm / q.bnorm <= q.tol) {
     q.active = 0;
     q.converged = 1;
@@ -117,7 +115,7 @@ __global__ void k_seg_check(ColState* __restrict__ s, const double* __restrict__
 }
 
 __global__ void k_seg_beta(ColState* __restrict__ s, const double* __restrict__ part, SegIndex ix,
-                           double* __restrict__ beta, int* __restrict__ mask, int itn, int kp) {
+                           double* __restrict__ beta, int* __restrict__ mask, int kp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ix.P * kp) return;
   ColState& q = s[j];
@@ -129,7 +127,6 @@ __global__ void k_seg_beta(ColState* __restrict__ s, const double* __restrict__ 
   if (fabs(v) < thr) {  // cg.rs:106-119 stagnation guard
     q.active = 0;
     q.converged = q.rnorm / q.bnorm <= q.tol;
-    q.iters = itn;
   } else {
     beta[j] = v / q.rz;
     q.rz = v;
@@ -238,16 +235,15 @@ void launch_seg_start(hipStream_t st, ColState* s, const double* part_bb, const 
                       const double* tol, const int* kcols, int kp) {
   k_seg_start<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part_bb, part_rz, ix, tol, kcols, kp);
 }
-void launch_seg_alpha(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* alpha, int itn,
-                      int kp) {
-  k_seg_alpha<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, alpha, itn, kp);
+void launch_seg_alpha(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* alpha, int kp) {
+  k_seg_alpha<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, alpha, kp);
 }
-void launch_seg_check(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, int itn, int kp) {
-  k_seg_check<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, itn, kp);
+void launch_seg_check(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, int kp) {
+  k_seg_check<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, kp);
 }
 void launch_seg_beta(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* beta, int* mask,
-                     int* n_active, int itn, int kp) {
-  k_seg_beta<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, beta, mask, itn, kp);
+                     int* n_active, int kp) {
+  k_seg_beta<<<pair_blocks(ix, kp), kPairBlock, 0, st>>>(s, part, ix, beta, mask, kp);
   k_seg_count<<<1, 1024, 0, st>>>(s, ix.P * kp, n_active);
 }
 void launch_seg_count(hipStream_t st, const ColState* s, const SegIndex& ix, int kp, int* n_active) {

@@ -33,11 +33,10 @@ void launch_seg_totals(hipStream_t st, const double* part, const SegIndex& ix, i
 // CG scalar steps per (problem, column) pair; pairs are indexed p * kp + c
 void launch_seg_start(hipStream_t st, ColState* s, const double* part_bb, const double* part_rz, const SegIndex& ix,
                       const double* tol, const int* kcols, int kp);
-void launch_seg_alpha(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* alpha, int itn,
-                      int kp);
-void launch_seg_check(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, int itn, int kp);
+void launch_seg_alpha(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* alpha, int kp);
+void launch_seg_check(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, int kp);
 void launch_seg_beta(hipStream_t st, ColState* s, const double* part, const SegIndex& ix, double* beta, int* mask,
-                     int* n_active, int itn, int kp);
+                     int* n_active, int kp);
 void launch_seg_count(hipStream_t st, const ColState* s, const SegIndex& ix, int kp, int* n_active);
 // x += alpha p (x may be null), r -= alpha ap, part = per-tile r'r
 template <typename T>

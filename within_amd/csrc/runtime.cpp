@@ -152,6 +152,16 @@ struct wa_solver {
     T *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr;
     double* hd = nullptr;
   };
+  // captured CG iterations, one per (precision, block width, x vector, side
+  // x-update, operator representation); replayed with hipGraphLaunch
+  struct GraphKey {
+    int kind, t, kp, op, side;  // kind: 0 block CG, 1 per-problem (stacked) CG
+    const void* x;
+  };
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+  bool use_graphs() const;
+  template <typename T, typename F>
+  hipGraphExec_t cg_graph(int kp, T* x, bool side, F&& body, int kind = 0);
   std::vector<PairSched> pairs;            // per-factor-pair subranges (sweep / overlapped apply)
   // additive apply with the pairs' local solves on two streams: the lead
   // pair (longest level chain) on st, the others on st2
@@ -241,6 +251,7 @@ struct wa_solver {
   }
   ~wa_solver() {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     for (void* p : allocs) (void)hipFree(p);
     if (ubuf) (void)hipFree(ubuf);
     if (st2) (void)hipStreamSynchronize(st2);
@@ -864,6 +875,69 @@ std::vector<double> wa_solver::col_norms(const double* V, int64_t rows, int kp) 
   return r;
 }
 
+// CG iterations are captured into a HIP graph and replayed, unless
+// WITHIN_AMD_GRAPHS=0, kernel timing is on (its event records), or an apply
+// would record a memset (a Schwarz preconditioner without subdomains).
+bool wa_solver::use_graphs() const {
+  const char* e = std::getenv("WITHIN_AMD_GRAPHS");
+  if (e && std::atoi(e) == 0) return false;
+  if (timer.on) return false;
+  if (params.preconditioner == 2) return false;
+  if (params.preconditioner == 1 && !has_pre) return false;
+  return true;
+}
+
+// Most CG iterations replayed per host check (WITHIN_AMD_GRAPH_CHUNK
+// overrides): 8 when a block vector holds at most 4M values (launch-bound
+// iterations: the 100k-observation 2-factor problem went from 1.64 to 1.34 ms
+// per solve), else 2 (20M observations, k = 32: 90.5 ms eager, 90.1 ms with 1
+// or 2 iterations per check, 96.1 / 92.1 ms with up to 4 / 8).
+static int graph_chunk_max(int64_t values) {
+  const char* e = std::getenv("WITHIN_AMD_GRAPH_CHUNK");
+  const int v = e ? std::atoi(e) : 0;
+  if (v >= 1 && v <= 64) return v;
+  return values <= ((int64_t)4 << 20) ? 8 : 2;
+}
+
+// Iterations to replay before the next host check: as many as the slowest
+// active column still needs at its last per-iteration residual reduction
+// (with a margin), capped by graph_chunk_max.  Columns that finish inside a
+// chunk are frozen by their masks, so results do not depend on the chunking.
+static int replay_chunk(const std::vector<ColState>& hs, const std::vector<double>& rate, int64_t values) {
+  double need = 1.0;
+  for (size_t c = 0; c < hs.size(); ++c) {
+    if (!hs[c].active || !(rate[c] > 0.0 && rate[c] < 1.0) || !(hs[c].rnorm > 0.0)) continue;
+    const double goal = hs[c].tol * hs[c].bnorm / hs[c].rnorm;
+    if (goal < 1.0) need = std::max(need, std::log(goal) / std::log(rate[c]));
+  }
+  return (int)std::max(1.0, std::min((double)graph_chunk_max(values), std::floor(0.7 * need - 0.5)));
+}
+
+template <typename T, typename F>
+hipGraphExec_t wa_solver::cg_graph(int kp, T* x, bool side, F&& body, int kind) {
+  const GraphKey key{kind, (int)sizeof(T), kp, params.operator_repr, side ? 1 : 0, x};
+  for (auto& g : graphs)
+    if (g.first.kind == key.kind && g.first.t == key.t && g.first.kp == key.kp && g.first.op == key.op &&
+        g.first.side == key.side && g.first.x == key.x)
+      return g.second;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(st, &g));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(rc);
+  graphs.push_back({key, ge});
+  return ge;
+}
+
 // Block CG: k independent recurrences sharing every operator and
 // preconditioner apply (solve/cg.rs:45-133 per column).
 template <typename T>
@@ -904,34 +978,49 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
                          launch_cg_x<T>(s2, u->x, u->p, u->alpha, u->m, u->kp);
                        },
                        &xu};
-  int64_t itn = 0;
-  bool any = true;
-  {
-    // all columns may already be done (zero right-hand sides)
-    std::vector<ColState> h(kp);
-    HIPCHK(hipMemcpyAsync(h.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    any = false;
-    for (int c = 0; c < kp; ++c) any = any || h[c].active;
-  }
-  while (any && itn < params.maxiter) {
-    ++itn;
+  // one iteration (cg.rs:75-125 for every column): the same launches whether
+  // enqueued directly or replayed from a captured graph
+  auto iteration = [&]() {
     op_apply<T>(p, ap, kp);
     tic(2, ev);
     int nb = launch_dot_part<T>(st, p, ap, m, kp, redp);
-    launch_cg_alpha(st, cs, redp, nb, pap, alpha, (int)itn, kp);
+    launch_cg_alpha(st, cs, redp, nb, pap, alpha, kp);
     nb = launch_cg_update<T>(st, side_x ? nullptr : x, p, r, ap, alpha, m, kp, redp);
-    launch_cg_check(st, cs, redp, nb, rr, (int)itn, kp);
+    launch_cg_check(st, cs, redp, nb, rr, kp);
     toc(2, ev);
     precondition<T>(r, z, kp, side_x ? &xside : nullptr);
     tic(2, ev);
     nb = launch_dot_part<T>(st, r, z, m, kp, redp);
-    launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, (int)itn, kp);
+    launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, kp);
     launch_xpby<T>(st, p, z, beta, mask, m, kp);
     toc(2, ev);
-    HIPCHK(hipMemcpyAsync(hbuf, nact, sizeof(int), hipMemcpyDeviceToHost, st));
+  };
+  hipGraphExec_t graph = use_graphs() ? cg_graph<T>(kp, x, side_x, iteration) : nullptr;
+  std::vector<ColState> hs(kp);
+  auto read_state = [&]() {
+    HIPCHK(hipMemcpyAsync(hs.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    any = *(int*)hbuf > 0;
+    int a = 0;
+    for (int c = 0; c < kp; ++c) a += hs[c].active;
+    return a;
+  };
+  // all columns may already be done (zero right-hand sides)
+  bool any = read_state() > 0;
+  int64_t itn = 0;
+  std::vector<double> rate(kp, 0.0), before(kp, 0.0);
+  while (any && itn < params.maxiter) {
+    // with a graph, several iterations are replayed per host check
+    int chunk = graph ? replay_chunk(hs, rate, m * kp) : 1;
+    chunk = (int)std::min<int64_t>(chunk, params.maxiter - itn);
+    for (int c = 0; c < kp; ++c) before[c] = hs[c].rnorm;
+    for (int i = 0; i < chunk; ++i) {
+      if (graph) HIPCHK(hipGraphLaunch(graph, st));
+      else iteration();
+    }
+    itn += chunk;
+    any = read_state() > 0;
+    for (int c = 0; c < kp; ++c)
+      if (before[c] > 0.0 && hs[c].rnorm > 0.0) rate[c] = std::pow(hs[c].rnorm / before[c], 1.0 / chunk);
   }
   std::vector<ColState> h(kp);
   HIPCHK(hipMemcpyAsync(h.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
@@ -1555,12 +1644,6 @@ void h2d(hipStream_t st, T* dst, const std::vector<T>& v) {
   if (!v.empty()) HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
 }
 
-int read_int(wa_solver* s, const int* d) {
-  HIPCHK(hipMemcpyAsync(s->hbuf, d, sizeof(int), hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  return *(int*)s->hbuf;
-}
-
 // per-pair column norms of a stacked DOF block
 std::vector<double> seg_norms(wa_solver* s, MultiGroup& g, const double* V, int kp) {
   const int np = g.ix.P * kp;
@@ -1589,25 +1672,42 @@ void seg_cg(wa_solver* s, MultiGroup& g, int kp, const T* rhs, T* x, const std::
   launch_seg_dot<T>(st, r, z, g.ix, kp, g.partA);
   launch_seg_dot<T>(st, rhs, nullptr, g.ix, kp, g.partB);
   launch_seg_start(st, g.cs, g.partB, g.partA, g.ix, g.tolv, g.kcols, kp);
-  launch_seg_count(st, g.cs, g.ix, kp, g.nact);
-  int nact = read_int(s, g.nact);
-  int64_t itn = 0;
-  while (nact > 0 && itn < s->params.maxiter) {
-    ++itn;
+  auto iteration = [&]() {
     s->op_apply<T>(p, ap, kp);
     launch_seg_dot<T>(st, p, ap, g.ix, kp, g.partA);
-    launch_seg_alpha(st, g.cs, g.partA, g.ix, g.alpha, (int)itn, kp);
+    launch_seg_alpha(st, g.cs, g.partA, g.ix, g.alpha, kp);
     launch_seg_update<T>(st, x, p, r, ap, g.alpha, g.ix, kp, g.partA);
-    launch_seg_check(st, g.cs, g.partA, g.ix, (int)itn, kp);
+    launch_seg_check(st, g.cs, g.partA, g.ix, kp);
     s->precondition<T>(r, z, kp);
     launch_seg_dot<T>(st, r, z, g.ix, kp, g.partA);
-    launch_seg_beta(st, g.cs, g.partA, g.ix, g.beta, g.mask, g.nact, (int)itn, kp);
+    launch_seg_beta(st, g.cs, g.partA, g.ix, g.beta, g.mask, g.nact, kp);
     launch_seg_xpby<T>(st, p, z, g.beta, g.mask, g.ix, kp);
-    nact = read_int(s, g.nact);
-  }
+  };
+  hipGraphExec_t graph = s->use_graphs() ? s->cg_graph<T>(kp, x, false, iteration, 1) : nullptr;
   std::vector<ColState> h(np);
-  HIPCHK(hipMemcpyAsync(h.data(), g.cs, sizeof(ColState) * np, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  auto read_state = [&]() {
+    HIPCHK(hipMemcpyAsync(h.data(), g.cs, sizeof(ColState) * np, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int a = 0;
+    for (int j = 0; j < np; ++j) a += h[j].active;
+    return a;
+  };
+  int nact = read_state();
+  int64_t itn = 0;
+  std::vector<double> rate(np, 0.0), before(np, 0.0);
+  while (nact > 0 && itn < s->params.maxiter) {
+    int chunk = graph ? replay_chunk(h, rate, s->m * kp) : 1;
+    chunk = (int)std::min<int64_t>(chunk, s->params.maxiter - itn);
+    for (int j = 0; j < np; ++j) before[j] = h[j].rnorm;
+    for (int i = 0; i < chunk; ++i) {
+      if (graph) HIPCHK(hipGraphLaunch(graph, st));
+      else iteration();
+    }
+    itn += chunk;
+    nact = read_state();
+    for (int j = 0; j < np; ++j)
+      if (before[j] > 0.0 && h[j].rnorm > 0.0) rate[j] = std::pow(h[j].rnorm / before[j], 1.0 / chunk);
+  }
   for (int j = 0; j < np; ++j) {
     if (h[j].active || h[j].nan_stop) {
       conv[j] = 0;
