@@ -273,6 +273,7 @@ def main():
                     help="headline solves only (no reference, baseline or CPU legs): for counter runs")
     ap.add_argument("--workload", choices=["akm20m", "many"], default="akm20m",
                     help="akm20m: the 20M-observation k=32 batch (the metric); many: 256 uneven problems")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default operator schedules")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB FP32 buffer 4 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -313,6 +314,12 @@ def main():
     fast = Run(p, W.CG(precision="mixed", tol=args.tol, operator=opr), args.dense_core)
     m = fast.m
     log(f"bench: setup {fast.setup_s:.1f}s  n_dofs={m}  info={fast.s.info().tolist()}")
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = fast.s.autotune(k=k)
+        tuned["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"bench: autotuned operator schedules {tuned}")
     X = torch.zeros((k, m), device="cuda", dtype=torch.float64)
     D = torch.zeros((k, n), device="cuda", dtype=torch.float64)
     for i in range(args.warmup):
@@ -467,6 +474,7 @@ def main():
         "max_rel_err_vs_fp64_ref": float(err.max()),
         "setup_s": round(fast.setup_s, 2),
         "implicit_operator": imp,
+        "autotune": tuned,
         "kernel_ms": {k_: round(v[1], 3) for k_, v in cls.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,

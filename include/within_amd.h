@@ -238,6 +238,17 @@ wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz
                                      double* col_w, uint32_t* round_ptr, uint32_t* order,
                                      uint32_t* core, double* core_inv);
 
+/* ---- autotuner ---------------------------------------------------------------
+ * Picks the shape of the operator's segmented-reduction schedules (elements
+ * per work item: 256..4096; longest row summed by one lane group: 4..32) for
+ * the explicit Gramian SpMM and for the implicit factor passes, by timing each
+ * candidate (`reps` FP32 launches of k columns, HIP events) and keeping the
+ * fastest; the reference fixes the equivalent constants (domain.rs:191,199;
+ * csr_block.rs:4-6).  out8: Gramian chunk, short-row limit, microseconds with
+ * the default and the chosen schedule, then the same four for the factor
+ * passes (-1 where the solver has no such operator). */
+wa_status wa_solver_autotune(wa_solver* s, int32_t k, int32_t reps, int64_t* out8);
+
 /* ---- kernel timing (HIP events on the solver's stream) ---------------------
  * Kernel classes: 0 operator, 1 preconditioner, 2 vector ops, 3 observation
  * passes, 4 explicit operator.  Returns launches and summed milliseconds

@@ -66,7 +66,7 @@ EXPORTS = [
     "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
     "wa_gram_apply_device", "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_matvec_d_device",
     "wa_rmatvec_wdt_device", "wa_gram_apply_noalloc", "wa_multi_create", "wa_multi_destroy", "wa_multi_info",
-    "wa_multi_solve",
+    "wa_multi_solve", "wa_solver_autotune",
 ]
 
 _lib = None
@@ -133,6 +133,7 @@ def load():
     L.wa_multi_destroy.restype = None
     L.wa_multi_info.argtypes = [P, i32, P]
     L.wa_multi_solve.argtypes = [P, P, P, P, P, P, P, P, P]
+    L.wa_solver_autotune.argtypes = [P, i32, i32, P]
     _lib = L
     return L
 

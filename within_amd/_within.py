@@ -477,6 +477,16 @@ class Solver:
             return None
         return FePreconditioner(_solver=self)
 
+    def autotune(self, k=32, reps=5):
+        """Time candidate schedule shapes of the operator (explicit Gramian and
+        implicit factor passes) for k right-hand sides and keep the fastest;
+        returns the choices and the timings (microseconds per launch)."""
+        out = np.zeros(8, dtype=np.int64)
+        N.check(N.load().wa_solver_autotune(self._h, int(k), int(reps), N.ptr(out)))
+        keys = ("gram_chunk", "gram_short", "gram_us_default", "gram_us_best", "pass_chunk", "pass_short",
+                "pass_us_default", "pass_us_best")
+        return dict(zip(keys, (int(v) for v in out)))
+
     def _solve_cols(self, Y):
         n, m = self.n_obs, self.n_dofs
         Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))

@@ -160,6 +160,10 @@ struct wa_solver {
   };
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   bool use_graphs() const;
+  void graphs_clear() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
   template <typename T, typename F>
   hipGraphExec_t cg_graph(int kp, T* x, bool side, F&& body, int kind = 0);
   std::vector<PairSched> pairs;            // per-factor-pair subranges (sweep / overlapped apply)
@@ -187,6 +191,18 @@ struct wa_solver {
   template <typename T> GmresBufs<T>& gm(int restart);
   // serialized preconditioner (built on first request)
   mutable std::vector<uint8_t> precond_bytes;
+  // autotuning of the operator schedules: host row pointers of the Gramian and
+  // of the factor passes (internal numbering), capacity of part64 in slots
+  std::vector<uint32_t> h_gram_ptr;
+  std::vector<std::vector<uint32_t>> h_pass_ptr;
+  int64_t part_cap = 0;
+  void ensure_part(int64_t slots) {
+    if (slots <= part_cap) return;
+    part64 = dalloc<double>(slots * KMAX);
+    part_cap = slots;
+    w64.part = part64;
+    w32.part = (float*)part64;
+  }
   // device user buffers for host-pointer calls
   double* ubuf = nullptr;
   int64_t ubuf_len = 0;
@@ -402,6 +418,7 @@ void wa_solver::setup_device() {
     a.levels = b.levels = lay.levels[fp.q];
     a.n_elem = b.n_elem = n;
     {
+      h_pass_ptr.push_back(fp.row_ptr);
       Schedule sc = make_schedule(fp.row_ptr, kChunk, kShortPass);
       sched_report("pass", sc, fp.row_ptr);
       a.sched = b.sched = upload_sched(sc);
@@ -420,6 +437,7 @@ void wa_solver::setup_device() {
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
     {
+      h_gram_ptr = g.indptr;
       Schedule sc = make_schedule(g.indptr, kGramChunk, kShortRow);
       sched_report("gramian", sc, g.indptr);
       g64.sched = g32.sched = upload_sched(sc);
@@ -643,6 +661,7 @@ void wa_solver::setup_device() {
   }
   phase("preconditioner");
   part64 = dalloc<double>(max_slots * KMAX);
+  part_cap = max_slots;
   w64.part = part64;
   w32.part = (float*)part64;
   redp = dalloc<double>((int64_t)kRedBlocks * KMAX);
@@ -1955,6 +1974,111 @@ double* scratch_on(int dev) {
 int64_t align32(int64_t v) { return (v + 31) / 32 * 32; }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Autotuner of the operator schedules (wa_solver_autotune).  The segmented
+// reductions run over host-built schedules whose shape -- the element count of
+// a work item (longer rows are split into chunks summed by a fix-up) and the
+// longest row one lane group sums alone -- sets how rows map to waves.  The
+// reference fixes the equivalent constants (domain.rs:191,199 chunk sizes,
+// csr_block.rs:4-6); here each candidate schedule is built, uploaded and timed
+// with HIP events on the solver's stream for the block width of the solves,
+// and the fastest is kept.  Results do not depend on the choice beyond the
+// order of the fix-up sums.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename F>
+double time_launches(hipStream_t st, int reps, F&& f) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  f();  // warm
+  HIPCHK(hipEventRecord(a, st));
+  for (int i = 0; i < reps; ++i) f();
+  HIPCHK(hipEventRecord(b, st));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (double)ms / reps;
+}
+
+}  // namespace
+
+static void autotune_impl(wa_solver* s, int kp, int reps, int64_t* out) {
+  hipStream_t st = s->st;
+  HIPCHK(hipMemsetAsync(s->cp32, 0, sizeof(float) * s->m * kp, st));
+  const uint32_t chunks[] = {256, 512, 1024, 2048, 4096};
+  const uint32_t shorts[] = {4, 8, 16, 32};
+  // explicit Gramian (FP32, the inner Krylov's operator)
+  if (s->has_gram) {
+    DevSched best = s->g32.sched;
+    int64_t best_c = kGramChunk, best_s = kShortRow;
+    auto run = [&]() { launch_spmm<float>(st, s->g32, s->cp32, s->cap32, (float*)s->part64, kp); };
+    const double t0 = time_launches(st, reps, run);
+    double tb = t0;
+    for (uint32_t c : chunks)
+      for (uint32_t sh : shorts) {
+        if (c == kGramChunk && sh == kShortRow) continue;
+        Schedule sc = make_schedule(s->h_gram_ptr, c, sh);
+        s->ensure_part(sc.n_slots);
+        const DevSched keep = s->g32.sched;
+        s->g32.sched = s->upload_sched(sc);
+        const double t = time_launches(st, reps, run);
+        if (t < tb) {
+          tb = t;
+          best = s->g32.sched;
+          best_c = c;
+          best_s = sh;
+        }
+        s->g32.sched = keep;
+      }
+    s->g32.sched = s->g64.sched = best;
+    out[0] = best_c;
+    out[1] = best_s;
+    out[2] = (int64_t)(t0 * 1e3);
+    out[3] = (int64_t)(tb * 1e3);
+  }
+  // factor passes of the implicit operator (all passes share one shape)
+  if (!s->p32.empty()) {
+    std::vector<DevSched> best;
+    for (auto& p : s->p32) best.push_back(p.sched);
+    int64_t best_c = kChunk, best_s = kShortPass;
+    auto run = [&]() {
+      for (int q = 0; q < s->Q; ++q) launch_pull<float>(st, s->p32[q], s->cp32, s->cap32, (float*)s->part64, kp);
+    };
+    const double t0 = time_launches(st, reps, run);
+    double tb = t0;
+    for (uint32_t c : chunks)
+      for (uint32_t sh : shorts) {
+        if (c == kChunk && sh == kShortPass) continue;
+        std::vector<DevSched> keep, cand;
+        for (int q = 0; q < s->Q; ++q) {
+          Schedule sc = make_schedule(s->h_pass_ptr[q], c, sh);
+          s->ensure_part(sc.n_slots);
+          keep.push_back(s->p32[q].sched);
+          cand.push_back(s->upload_sched(sc));
+          s->p32[q].sched = cand.back();
+        }
+        const double t = time_launches(st, reps, run);
+        if (t < tb) {
+          tb = t;
+          best = cand;
+          best_c = c;
+          best_s = sh;
+        }
+        for (int q = 0; q < s->Q; ++q) s->p32[q].sched = keep[q];
+      }
+    for (int q = 0; q < s->Q; ++q) s->p32[q].sched = s->p64[q].sched = best[q];
+    out[4] = best_c;
+    out[5] = best_s;
+    out[6] = (int64_t)(t0 * 1e3);
+    out[7] = (int64_t)(tb * 1e3);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 extern "C" {
 
 const char* wa_last_error(void) { return g_err.c_str(); }
@@ -2420,6 +2544,20 @@ wa_status wa_multi_solve(wa_multi* mp, const double* const* Y, const int32_t* k,
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int i = 0; i < mp->n_problems; ++i)
       for (int c = 0; c < k[i]; ++c) stats[i][c].time_solve = t;
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_solver_autotune(wa_solver* s, int32_t k, int32_t reps, int64_t* out8) {
+  try {
+    need_design(s);
+    if (!out8 || k <= 0 || reps <= 0) throw Error(WA_ERR_INVALID_ARGUMENT, "bad arguments");
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    for (int i = 0; i < 8; ++i) out8[i] = -1;
+    autotune_impl(s, pow2_at_least(std::min(k, KMAX)), reps, out8);
+    s->graphs_clear();
     return WA_OK;
   } catch (const std::exception& e) {
     return (wa_status)set_error(e);
