@@ -15,9 +15,8 @@ namespace wa {
 
 // phase timings on stderr when WITHIN_AMD_VERBOSE is set
 static bool verbose() {
-  static int v = -1;
-  if (v < 0) v = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
-  return v == 1;
+  static const bool v = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;  // thread-safe initialisation
+  return v;
 }
 struct PhaseTimer {
   const char* name;
@@ -1377,11 +1376,23 @@ void build_plan(Plan& p) {
     PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
-    // large subdomains first, one at a time (their Schur sampling, rounds and
-    // dense cores use every thread), then the rest in parallel
-    std::vector<size_t> serial, rest;
-    for (size_t i = 0; i < p.subs.size(); ++i)
-      (p.subs[i].n_keep() >= 1024 || p.subs[i].n_local() >= 65536 ? serial : rest).push_back(i);
+    // huge subdomains first, one at a time (their Schur sampling, rounds and
+    // dense cores use every thread), then the large ones side by side (one
+    // thread each: a stack of many problems has hundreds of them), then the
+    // rest.  Each build's arithmetic does not depend on its thread count.
+    std::vector<size_t> serial, large, rest;
+    for (size_t i = 0; i < p.subs.size(); ++i) {
+      const Subdomain& sd = p.subs[i];
+      const i64 keep = std::min(sd.ct.n_q(), sd.ct.n_r());  // the smaller block is kept
+      if (sd.n_local() >= ((i64)1 << 20) || keep >= 16384) serial.push_back(i);
+      else if (keep >= 1024 || sd.n_local() >= 65536) large.push_back(i);
+      else rest.push_back(i);
+    }
+    // a few large subdomains keep their inner parallelism
+    if (large.size() < 4) {
+      serial.insert(serial.end(), large.begin(), large.end());
+      large.clear();
+    }
     auto build_one = [&](size_t i) {
       try {
         auto t0 = std::chrono::steady_clock::now();
@@ -1399,6 +1410,14 @@ void build_plan(Plan& p) {
     {
       PhaseTimer ts("local solvers, large subdomains");
       for (size_t i : serial) build_one(i);
+    }
+    {
+      PhaseTimer tl("local solvers, large subdomains side by side");
+      // largest first (dynamic schedule)
+      std::sort(large.begin(), large.end(),
+                [&](size_t a, size_t b) { return p.subs[a].n_local() > p.subs[b].n_local(); });
+#pragma omp parallel for schedule(dynamic, 1)
+      for (size_t t = 0; t < large.size(); ++t) build_one(large[t]);
     }
     {
       PhaseTimer tr("local solvers, the rest");
