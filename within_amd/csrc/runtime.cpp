@@ -18,6 +18,7 @@
 
 #include "../../include/within_amd.h"
 #include "kernels.hpp"
+#include "gmres.hpp"
 #include "multi.hpp"
 #include "plan.hpp"
 
@@ -162,6 +163,7 @@ struct wa_solver {
   bool use_graphs() const;
   void graphs_clear() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (hipEvent_t e : gmd.ev) (void)hipEventDestroy(e);
     graphs.clear();
   }
   template <typename T, typename F>
@@ -186,6 +188,17 @@ struct wa_solver {
   double* mw64 = nullptr;                  // r_work, delta, G delta (multiplicative)
   float* mw32 = nullptr;
   double* ones = nullptr;
+  // device GMRES state (allocated on first use): column states, Hessenberg
+  // columns, the two Gram-Schmidt passes' coefficients, y, per-column scalars,
+  // reduction partials, per-step extension counts and their events
+  struct GmDev {
+    GmCol* cols = nullptr;
+    double *H = nullptr, *h = nullptr, *h2 = nullptr, *y = nullptr, *coef = nullptr, *beta = nullptr;
+    double *abs_tol = nullptr, *part = nullptr, *part2 = nullptr;
+    int *live = nullptr, *lim = nullptr, *ninc = nullptr;
+    std::vector<hipEvent_t> ev;
+  } gmd;
+  GmDev& gm_dev();
   GmresBufs<double> gm64;
   GmresBufs<float> gm32;
   template <typename T> GmresBufs<T>& gm(int restart);
@@ -306,12 +319,16 @@ struct wa_solver {
   void cg(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
           std::vector<int64_t>& its);
   template <typename T>
+  void gmres_dev(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+                 std::vector<int64_t>& its);
+  template <typename T>
   void gmres(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
              std::vector<int64_t>& its);
   template <typename T>
   void krylov(int k, int kp, const T* rhs, T* x, const std::vector<double>& tol, std::vector<int>& conv,
               std::vector<int64_t>& its) {
-    if (params.krylov == 1) gmres<T>(k, kp, rhs, x, tol, conv, its);
+    if (params.krylov == 1 && params.restart < kGmresMaxV) gmres_dev<T>(k, kp, rhs, x, tol, conv, its);
+    else if (params.krylov == 1) gmres<T>(k, kp, rhs, x, tol, conv, its);
     else cg<T>(k, kp, rhs, x, tol, conv, its);
   }
   void lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>& its);
@@ -1051,6 +1068,133 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
     } else {
       conv[c] = h[c].converged;
       its[c] = h[c].iters;
+    }
+  }
+}
+
+wa_solver::GmDev& wa_solver::gm_dev() {
+  GmDev& d = gmd;
+  if (d.cols) return d;
+  d.cols = dalloc<GmCol>(KMAX);
+  d.H = dalloc<double>((int64_t)KMAX * kGmresMaxV * (kGmresMaxV + 1));
+  d.h = dalloc<double>((int64_t)kGmresMaxV * KMAX);
+  d.h2 = dalloc<double>((int64_t)kGmresMaxV * KMAX);
+  d.y = dalloc<double>((int64_t)kGmresMaxV * KMAX);
+  d.coef = dalloc<double>(KMAX);
+  d.beta = dalloc<double>(KMAX);
+  d.abs_tol = dalloc<double>(KMAX);
+  d.part = dalloc<double>((int64_t)kGmRedBlocks * kGmresMaxV * KMAX);
+  d.part2 = dalloc<double>((int64_t)kGmRedBlocks * KMAX);
+  d.live = dalloc<int>(KMAX);
+  d.lim = dalloc<int>(KMAX);
+  d.ninc = dalloc<int>(kGmresMaxV);
+  d.ev.resize(kGmresMaxV);
+  for (hipEvent_t& e : d.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return d;
+}
+
+// Right-preconditioned GMRES(restart) for a block of columns, one Arnoldi
+// process per column (solve/gmres.rs:201-530), with the Arnoldi step on the
+// device: two passes of block Gram-Schmidt (all dot products of a pass in one
+// launch), the FP64 Hessenberg / Givens update and the stop decision per
+// column in one kernel.  The host enqueues step j + 1 once step j - 1 has
+// finished (its count of extending columns, copied to pinned memory), so the
+// device is never idle waiting for the host; at most one step after the last
+// column stops is wasted.  The true residual at each restart is FP64-reduced
+// on the device as before.
+template <typename T>
+void wa_solver::gmres_dev(int k, int kp, const T* b, T* x, const std::vector<double>& tol, std::vector<int>& conv,
+                          std::vector<int64_t>& its) {
+  const int mr = params.restart;
+  const int64_t vb = m * kp, vs = m * KMAX;
+  GmresBufs<T>& G = gm<T>(mr);
+  GmDev& D = gm_dev();
+  T *w = G.w, *r = G.r;
+  auto V = [&](int i) { return G.V + (int64_t)i * vs; };
+  auto Z = [&](int i) { return G.Z + (int64_t)i * vs; };
+  HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * vb, st));
+  auto norms = [&](const T* v) {
+    launch_dot<T>(st, v, v, m, kp, redp, s_(10));
+    HIPCHK(hipMemcpyAsync(hbuf, s_(10), sizeof(double) * kp, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> o(kp);
+    for (int c = 0; c < kp; ++c) o[c] = std::sqrt(hbuf[c]);
+    return o;
+  };
+  std::vector<double> bn = norms(b), abs_tol(KMAX, 0.0);
+  std::vector<int> live(KMAX, 0);
+  for (int c = 0; c < k; ++c) {
+    abs_tol[c] = tol[c] * bn[c];
+    if (bn[c] < F64_EPS) { conv[c] = 1; its[c] = 0; } else live[c] = 1;
+  }
+  HIPCHK(hipMemcpyAsync(D.abs_tol, abs_tol.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+  auto any_live = [&]() { for (int c = 0; c < kp; ++c) if (live[c]) return true; return false; };
+  int* ninc_h = reinterpret_cast<int*>(hbuf) + 2 * KMAX;  // pinned slots, one per step
+  std::vector<GmCol> hc(kp);
+  int64_t total = 0;
+  hipEvent_t et = nullptr;
+  while (any_live()) {
+    // true residual r = b - A x (restart), gmres.rs:324-335
+    op_apply<T>(x, r, kp);
+    launch_rsub<T>(st, r, b, vb);
+    std::vector<double> beta = norms(r);
+    for (int c = 0; c < kp; ++c) {
+      if (!live[c]) continue;
+      if (beta[c] <= abs_tol[c]) { live[c] = 0; conv[c] = 1; its[c] = total; }
+      else if (total >= params.maxiter) { live[c] = 0; conv[c] = 0; its[c] = total; }
+    }
+    if (!any_live()) break;
+    HIPCHK(hipMemcpyAsync(D.live, live.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(D.beta, beta.data(), sizeof(double) * kp, hipMemcpyHostToDevice, st));
+    launch_gm_cycle_start(st, D.cols, D.H, D.beta, D.live, D.coef, mr, kp);
+    launch_scale_col<T>(st, V(0), r, D.coef, m, kp);
+    const int64_t t0 = total;
+    const int64_t cycle = std::min<int64_t>(params.maxiter - total, mr);
+    int j = 0;
+    while (j < cycle) {
+      if (j >= 2) {  // the count of step j - 2 (step j - 1 is running): stop when no column extends
+        HIPCHK(hipEventSynchronize(D.ev[j - 2]));
+        if (ninc_h[j - 2] == 0) break;
+      }
+      precondition<T>(V(j), Z(j), kp);
+      op_apply<T>(Z(j), w, kp);
+      tic(2, et);
+      launch_multi_dot<T>(st, w, V(0), vs, j + 1, m, kp, D.part, D.h, 0);
+      launch_multi_axpy<T>(st, w, V(0), vs, j + 1, D.h, nullptr, -1.0, m, kp, nullptr);
+      launch_multi_dot<T>(st, w, V(0), vs, j + 1, m, kp, D.part, D.h2, 0);
+      const int nb = launch_multi_axpy<T>(st, w, V(0), vs, j + 1, D.h2, nullptr, -1.0, m, kp, D.part2);
+      ++total;
+      launch_gm_givens(st, D.cols, D.H, D.h, D.h2, D.part2, nb, D.abs_tol, D.coef, D.ninc + j, j, mr, total,
+                       params.maxiter, kp);
+      launch_scale_col<T>(st, V(j + 1), w, D.coef, m, kp);
+      toc(2, et);
+      HIPCHK(hipMemcpyAsync(ninc_h + j, D.ninc + j, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(D.ev[j], st));
+      ++j;
+    }
+    total = t0 + j;
+    // y = R^-1 g per column (gmres.rs:499-519), x += Z y (:523-530)
+    launch_gm_backsolve(st, D.cols, D.H, D.live, D.y, D.lim, mr, kp);
+    launch_multi_axpy<T>(st, x, Z(0), vs, j, D.y, D.lim, 1.0, m, kp, nullptr);
+    HIPCHK(hipMemcpyAsync(hc.data(), D.cols, sizeof(GmCol) * kp, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool need_true = false;
+    for (int c = 0; c < kp; ++c) {
+      if (!live[c]) continue;
+      if (hc[c].outcome == 0) { live[c] = 0; conv[c] = 1; its[c] = t0 + hc[c].jc; }
+      else if (hc[c].outcome == 1) { live[c] = 0; conv[c] = 0; its[c] = t0 + hc[c].jc; }
+      else if (hc[c].outcome == 2) need_true = true;
+    }
+    if (need_true) {  // lucky breakdown: decide on the true residual
+      op_apply<T>(x, r, kp);
+      launch_rsub<T>(st, r, b, vb);
+      std::vector<double> tr = norms(r);
+      for (int c = 0; c < kp; ++c)
+        if (live[c] && hc[c].outcome == 2) {
+          live[c] = 0;
+          conv[c] = tr[c] <= abs_tol[c];
+          its[c] = t0 + hc[c].jc;
+        }
     }
   }
 }
