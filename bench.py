@@ -420,13 +420,17 @@ def main():
         erri = (torch.linalg.vector_norm(Di - Dref, dim=1) / torch.linalg.vector_norm(Dref, dim=1)).cpu().numpy()
         b_imp = 4 * Q * n + 2 * 4 * k * m_dev
         ms_imp = ms.value / max(nl.value, 1)
+        ti_json = os.path.join(ROOT, "profiles", "implicit_fp32_k32_akm20m_traffic.json")
+        t_imp = None
+        if args.scale == 1.0 and k == 32 and os.path.exists(ti_json):
+            t_imp = round(json.load(open(ti_json))["traffic_bytes_per_apply"])
         imp = {"rhs_per_s": round(sum(bool(sti[j][0]) and erri[j] <= 1e-8 for j in range(k)) / ti, 3),
                "ms_per_step": round(ti * 1e3, 3), "iterations": int(max(x[1] for x in sti)),
                "max_rel_err_vs_fp64_ref": float(erri.max()),
                "roofline": {"bound": "hbm", "kernel": f"implicit operator apply (k={k} FP32, {Q} factor passes)",
                             "algorithmic_bytes": b_imp, "avg_ms": round(ms_imp, 4),
                             "achieved": round(b_imp / (ms_imp * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(b_imp / (ms_imp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+                            "frac": round(b_imp / (ms_imp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": t_imp}}
         log(f"bench: implicit mixed block {ti * 1e3:.1f} ms ({imp['rhs_per_s']} RHS/s), operator apply "
             f"{ms_imp:.3f} ms ({imp['roofline']['frac']:.3f} of peak)")
         del Xi, Di
@@ -446,7 +450,7 @@ def main():
     # HBM bytes per dispatch of this kernel from the committed rocprofv3 PMC
     # summary (FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy's readings)
     traffic = None
-    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "spmm_fp32_k32_akm20m_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "spmm_fp32_k32_akm20m_traffic.json")
     if args.operator == "explicit" and args.scale == 1.0 and k == 32 and os.path.exists(tf):
         traffic = round(json.load(open(tf))["traffic_bytes"])
     out = {

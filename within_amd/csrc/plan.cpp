@@ -1477,7 +1477,7 @@ struct Reader {
     const uint64_t n = get<uint64_t>();
     if (n > (uint64_t)left / sizeof(T)) throw Error(E_INVALID, "preconditioner bytes are truncated or malformed");
     v.resize(n);
-    std::memcpy(v.data(), p, sizeof(T) * n);
+    if (n) std::memcpy(v.data(), p, sizeof(T) * n);  // an empty vector's data() may be null
     p += sizeof(T) * n;
     left -= (i64)(sizeof(T) * n);
   }
