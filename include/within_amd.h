@@ -52,7 +52,7 @@ typedef struct {
   int32_t schur_split;
   int32_t dense_threshold; /* n_keep <= threshold -> dense anchored Cholesky      */
   int64_t dense_core;      /* approximate Cholesky hands its last <= dense_core
-                              vertices to an exact dense factor (0 = default 8192,
+                              vertices to an exact dense factor (0 = default 16384,
                               1 = pure approximate Cholesky)                     */
 } wa_local_config;
 

@@ -116,7 +116,7 @@ class OracleSolver:
     def __init__(self, categories, weights=None, *, krylov=0, restart=30, explicit_op=False,
                  tol=1e-8, maxiter=1000, max_refinements=2, precond=1, ac_seed=0, ac_split=1,
                  approx_schur=True, schur_seed=0, schur_split=1, dense_threshold=24,
-                 dense_core=8192, local_size=0):
+                 dense_core=16384, local_size=0):
         L = lib()
         cats = np.asfortranarray(np.asarray(categories, dtype=np.uint32))
         if cats.ndim != 2:

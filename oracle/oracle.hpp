@@ -142,7 +142,7 @@ struct LocalSolverConfig {  // crates/within/src/config.rs:83-118
   uint64_t schur_seed = 0;
   int schur_split = 1;
   int dense_threshold = 24;
-  i64 dense_core = 8192;     // AC stops and factors exactly below this size
+  i64 dense_core = 16384;    // AC stops and factors exactly below this size
 };
 
 struct BlockElim {  // operator/local_solver.rs:306-468

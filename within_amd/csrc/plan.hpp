@@ -34,7 +34,7 @@ struct LocalConfig {
   uint64_t schur_seed = 0;
   int schur_split = 1;
   int dense_threshold = 24;
-  i64 dense_core = 8192;
+  i64 dense_core = 16384;
   bool device = false;  // factor dense cores of at least kDeviceCore on the GPU
 };
 constexpr i64 kDeviceCore = 2048;

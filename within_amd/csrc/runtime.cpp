@@ -2306,7 +2306,7 @@ static std::unique_ptr<wa_solver> create_solver(const Design& dd, const wa_param
   pl.cfg.schur_seed = p->local.schur_seed;
   pl.cfg.schur_split = p->local.schur_split;
   pl.cfg.dense_threshold = p->local.dense_threshold;
-  pl.cfg.dense_core = p->local.dense_core == 0 ? 8192 : p->local.dense_core;
+  pl.cfg.dense_core = p->local.dense_core == 0 ? 16384 : p->local.dense_core;
   pl.cfg.device = !s->host_only;
   pl.precond = p->preconditioner;
   pl.explicit_op = p->operator_repr == 1;
