@@ -231,11 +231,14 @@ __device__ __forceinline__ void fix_row(const K& k, const Fixup& f, const T* __r
   T a[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) a[u] = 0;
-  uint32_t s = f.slot0 + s0;
-  for (; s + (U - 1) * str < f.slot1; s += U * str)
+  // two-stage fix-ups sum their first-stage blocks instead of the chunk slots
+  const uint32_t b0 = f.chunk0 == kNoChunks ? f.slot0 : f.chunk0;
+  const uint32_t b1 = f.chunk0 == kNoChunks ? f.slot1 : f.chunk0 + (f.slot1 - f.slot0 + kFixChunk - 1) / kFixChunk;
+  uint32_t s = b0 + s0;
+  for (; s + (U - 1) * str < b1; s += U * str)
 #pragma unroll
     for (int u = 0; u < U; ++u) a[u] += part[(int64_t)(s + u * str) * kp + col];
-  for (; s < f.slot1; s += str) a[0] += part[(int64_t)s * kp + col];
+  for (; s < b1; s += str) a[0] += part[(int64_t)s * kp + col];
   sh[threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   __syncthreads();
   for (int w = str / 2; w >= 1; w >>= 1) {
@@ -247,6 +250,29 @@ __device__ __forceinline__ void fix_row(const K& k, const Fixup& f, const T* __r
     k.template finish<1>(f.row, col, a, k.template pre<1>(f.row, col));
   }
   __syncthreads();  // sh is reused by the next call
+}
+
+// First stage of the long fix-ups: block y of fix-up x sums its kFixChunk
+// chunk slots (one column per lane group, a fixed tree) into partial slot
+// chunk0 + y.  Fix-ups done in one stage, and blocks past a fix-up's end, exit.
+template <typename T>
+__global__ void __launch_bounds__(kFixChunk) k_fix_stage1(const Fixup* __restrict__ fx, T* __restrict__ part, int kp) {
+  __shared__ T sh[kFixChunk];
+  const Fixup f = fx[blockIdx.x];
+  if (f.chunk0 == kNoChunks) return;
+  const uint32_t s0 = f.slot0 + blockIdx.y * kFixChunk;
+  if (s0 >= f.slot1) return;
+  const uint32_t s1 = min(f.slot1, s0 + kFixChunk);
+  const int col = threadIdx.x % kp, str = kFixChunk / kp, g = threadIdx.x / kp;
+  T acc = 0;
+  for (uint32_t s = s0 + g; s < s1; s += str) acc += part[(int64_t)s * kp + col];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = str / 2; w >= 1; w >>= 1) {
+    if (g < w) sh[threadIdx.x] += sh[threadIdx.x + w * kp];
+    __syncthreads();
+  }
+  if (g == 0) part[(int64_t)(f.chunk0 + blockIdx.y) * kp + col] = sh[col];
 }
 
 template <class K, typename T>
@@ -290,6 +316,8 @@ void run_seg(hipStream_t st, const K& k, const DevSched& sc, const Segment& g, T
   } else {
     launch_rows<K, T, 1, 1>(st, k, sc, g, part);
   }
+  if (g.max_chunks > 0)
+    k_fix_stage1<T><<<dim3((unsigned)(g.f1 - g.f0), g.max_chunks), kFixChunk, 0, st>>>(sc.fix + g.f0, part, kp);
   if (g.f1 > g.f0) k_rows_fix<K, T><<<(unsigned)(g.f1 - g.f0), kFixBlock, 0, st>>>(k, sc.fix + g.f0, part, kp);
 }
 

@@ -67,7 +67,7 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
 // ---------------------------------------------------------------------------
 void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r1, u32 chunk,
                      const std::vector<char>* skip) {
-  Segment g{r0, r1, (i64)s.items.size(), 0, (i64)s.fixups.size(), 0};
+  Segment g{r0, r1, (i64)s.items.size(), 0, (i64)s.fixups.size(), 0, 0};
   std::vector<Item> whole;
   std::vector<u32> split;
   for (u32 r = r0; r < r1; ++r) {
@@ -87,7 +87,17 @@ void append_schedule(Schedule& s, const std::vector<u32>& row_ptr, u32 r0, u32 r
   for (size_t t = 0; t < split.size(); ++t) {
     slot0[t] = (u32)s.n_slots;
     s.n_slots += nchunks(split[t]);
-    s.fixups.push_back({split[t], slot0[t], (u32)s.n_slots, 0});
+    s.fixups.push_back({split[t], slot0[t], (u32)s.n_slots, kNoChunks});
+  }
+  // long fix-ups: first-stage partial slots after the chunk slots
+  for (size_t t = 0; t < split.size(); ++t) {
+    Fixup& f = s.fixups[s.fixups.size() - split.size() + t];
+    const u32 len = f.slot1 - f.slot0;
+    if (len <= 2 * kFixChunk) continue;
+    const u32 nb = (len + kFixChunk - 1) / kFixChunk;
+    f.chunk0 = (u32)s.n_slots;
+    s.n_slots += nb;
+    g.max_chunks = std::max(g.max_chunks, nb);
   }
   auto chunk_item = [&](size_t t, u32 j) {
     const u32 r = split[t];

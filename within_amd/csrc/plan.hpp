@@ -56,9 +56,15 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
 // directly) or, beyond `chunk` elements, several chunks [e0, e1) written to
 // partial slots `slot` that a fix-up sums in slot order.
 struct Item { u32 row0, row1, e0, e1; int32_t slot; int32_t pad; };
-struct Fixup { u32 row, slot0, slot1, pad; };
-// one row range with its items [i0, i1) and fix-ups [f0, f1)
-struct Segment { u32 r0, r1; i64 i0, i1, f0, f1; };
+// A fix-up of a row with more than 2 * kFixChunk slots is done in two stages:
+// blocks of kFixChunk slots are first summed into their own partial slots
+// starting at chunk0, and the fix-up then sums those (kNoChunks: one stage).
+struct Fixup { u32 row, slot0, slot1, chunk0; };
+constexpr u32 kFixChunk = 256;
+constexpr u32 kNoChunks = 0xFFFFFFFFu;
+// one row range with its items [i0, i1) and fix-ups [f0, f1); max_chunks:
+// the most first-stage blocks of any of its fix-ups (0: all one stage)
+struct Segment { u32 r0, r1; i64 i0, i1, f0, f1; u32 max_chunks = 0; };
 
 // Workgroups are dispatched to the MI355X's 8 XCDs (each with its own L2)
 // round-robin, and the engine runs 4 waves (items) per workgroup.  Chunks of
