@@ -1536,14 +1536,33 @@ __global__ void __launch_bounds__(kBlock) k_core_mfma(const float* __restrict__ 
   }
 }
 
+// C = sum of the split partials, in split order; 16-byte vectors, all of a
+// vector's partial loads in flight together (n * kp is a multiple of NV)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_core_sum(const T* __restrict__ P, T* __restrict__ C, uint32_t n,
                                                      uint32_t ld, int split, int kp) {
-  const int64_t tot = (int64_t)n * kp;
+  constexpr int NV = 16 / sizeof(T), U = 8;
+  const int64_t tot = (int64_t)n * kp / NV, stride = (int64_t)ld * kp;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kBlock) {
-    T a = 0;
-    for (int s = 0; s < split; ++s) a += P[(int64_t)s * ld * kp + i];
-    C[i] = a;
+    VecT<T, NV> a;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a.v[c] = 0;
+    int s = 0;
+    for (; s + U <= split; s += U) {
+      VecT<T, NV> v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = vload<T, NV>(P + (s + u) * stride + i * NV);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) a.v[c] += v[u].v[c];
+    }
+    for (; s < split; ++s) {
+      const VecT<T, NV> v = vload<T, NV>(P + s * stride + i * NV);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) a.v[c] += v.v[c];
+    }
+    vstore<T, NV>(C + i * NV, a);
   }
 }
 
@@ -1623,7 +1642,8 @@ static void run_dense_core(hipStream_t st, const DenseCore<T>& d, const uint32_t
           k_core_tiled<T, 32><<<grid, kBlock, 0, st>>>(d.M, Zc, P, d.n, d.ld, ks);
         break;
     }
-    k_core_sum<T><<<grid_stride_blocks((int64_t)d.n * kp, 1), kBlock, 0, st>>>(P, out, d.n, d.ld, nsplit, kp);
+    k_core_sum<T><<<grid_stride_blocks((int64_t)d.n * kp / (16 / sizeof(T)), 1), kBlock, 0, st>>>(P, out, d.n, d.ld,
+                                                                                                 nsplit, kp);
   } else {
     const unsigned blocks = (unsigned)(((d.n + 3) / 4 * (int64_t)kWave + kBlock - 1) / kBlock);
     switch (kp) {
