@@ -69,100 +69,148 @@ __device__ __forceinline__ void vstore(T* p, const T (&a)[NV]) {
   vstore<T, NV>(p, v);
 }
 
-// one row by one lane group: elements in order, U gathers in flight
+// Gathers are issued branch-free: an element past the end of the row loads
+// the row's last element again (always in bounds) and its term is dropped by
+// a select.  A guarded load would become a branch after which the compiler
+// waits for every load in flight, leaving one gather in flight per lane.
+//
+// In a work item, the wave can also share its index loads: each lane loads
+// one element's indices and coefficient, and the lanes that gather the
+// element take them with a lane shuffle, so one index load feeds a batch of
+// up to 8 gathers per lane.  That pays where the gathers miss the caches (the
+// operator rows); where they hit a small, hot vector (the elimination and
+// triangular-solve rows) the added shuffle latency and the rounds past the
+// row's end cost more, so a kind opts out with SHARED_IDX = false.  The
+// summation order is the same either way.
+constexpr int kRowU = 4;   // short rows, one index per element: gathers in flight
+constexpr int kRowUQ = 2;  // short rows, several indices per element
+constexpr int kVecU = 4;   // work items without shared index loads: elements per lane in flight
+template <class K, class = void>
+struct shares_idx : std::true_type {};
+template <class K>
+struct shares_idx<K, std::void_t<decltype(K::SHARED_IDX)>> : std::integral_constant<bool, K::SHARED_IDX> {};
+// gathers of one shared-index batch: at most 8 gathered vectors in flight per lane
+constexpr int coop_batch(int v, int qm) { return v * qm <= 8 ? v : (8 / qm > 0 ? 8 / qm : 1); }
+
+template <class K, typename T, int QM>
+__device__ __forceinline__ void coop_take(const uint32_t (&mid)[QM], T mcf, int src, uint32_t (&id)[QM], T& cf) {
+#pragma unroll
+  for (int t = 0; t < QM; ++t) id[t] = (uint32_t)__shfl((int)mid[t], src, kWave);
+  if constexpr (K::COEF) cf = __shfl(mcf, src, kWave);
+  else cf = mcf;
+}
+
+// add the U terms of one batch in order; terms with ok[u] false are dropped
+template <class K, typename T, int U, int QM, int NV>
+__device__ __forceinline__ void add_terms(const K& k, const VecT<T, NV> (&g)[U][QM], const T (&cf)[U],
+                                          const bool (&ok)[U], const uint32_t (&e)[U], int col, T (&acc)[NV]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    VecT<T, NV> term;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      T s = 0;
+#pragma unroll
+      for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
+      term.v[c] = K::COEF ? cf[u] * s : s;
+      acc[c] += ok[u] ? term.v[c] : T(0);
+    }
+    if constexpr (emits<K>::value)
+      if (ok[u]) k.template emit<NV>(e[u], col, term);
+  }
+}
+
+// one row by one lane group: elements in order
 template <class K, typename T, int KP, int NV>
 __device__ __forceinline__ void gather_row(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int part,
                                            T (&acc)[NV]) {
   constexpr int QM = K::QM > 0 ? K::QM : 1;
-  constexpr int U = QM == 1 ? 4 : 2;
 #pragma unroll
   for (int c = 0; c < NV; ++c) acc[c] = 0;
   if (K::QM == 0) return;
+  constexpr int U = QM == 1 ? kRowU : kRowUQ;  // gathers in flight
   for (uint32_t e = e0; e < e1; e += U) {
     uint32_t id[U][QM];
     T cf[U];
+    bool ok[U];
+    uint32_t ee[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       cf[u] = 1;
-#pragma unroll
-      for (int t = 0; t < QM; ++t) id[u][t] = 0;
-      if (e + u < e1) k.load(row, e + u, id[u], cf[u]);
+      ee[u] = e + u;
+      ok[u] = ee[u] < e1;
+      k.load(row, ok[u] ? ee[u] : e1 - 1, id[u], cf[u]);
     }
     VecT<T, NV> g[U][QM];
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int t = 0; t < QM; ++t) {
-        if (e + u < e1) {
-          g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
-        } else {
-#pragma unroll
-          for (int c = 0; c < NV; ++c) g[u][t].v[c] = 0;
-        }
-      }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      VecT<T, NV> term;
-#pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        T s = 0;
-#pragma unroll
-        for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
-        term.v[c] = K::COEF ? cf[u] * s : s;
-        if (e + u < e1) acc[c] += term.v[c];
-      }
-      if constexpr (emits<K>::value)
-        if (e + u < e1) k.template emit<NV>(e + u, part * NV, term);
-    }
+      for (int t = 0; t < QM; ++t) g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
+    add_terms<K, T, U, QM, NV>(k, g, cf, ok, ee, part * NV, acc);
   }
 }
 
-// one row (or chunk) by a whole wave: V lanes per row, S slots over elements
+// one row (or chunk) by a whole wave: V lanes per row, S slots over elements;
+// slot s sums elements e0 + s, e0 + s + S, e0 + s + 2S, ... in that order
 template <class K, typename T, int V, int NV>
 __device__ __forceinline__ void gather_vec(const K& k, uint32_t row, uint32_t e0, uint32_t e1, int slot, int part,
                                            T (&acc)[NV]) {
   constexpr int S = kWave / V;
   constexpr int QM = K::QM > 0 ? K::QM : 1;
   constexpr int KP = V * NV;
-  constexpr int U = 2;  // elements per lane in flight together
 #pragma unroll
   for (int c = 0; c < NV; ++c) acc[c] = 0;
   if (K::QM == 0) return;
-  for (uint32_t e = e0 + slot; e < e1; e += U * S) {
-    uint32_t id[U][QM];
-    T cf[U];
+  if constexpr (shares_idx<K>::value && V >= 4) {
+    // lane l loads element base + l; in round u, slot s gathers element
+    // base + u * S + s, whose indices lane u * S + s holds.  Control flow is
+    // uniform over the wave (one item per wave).
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr int UB = coop_batch(V, QM);
+    for (uint32_t base = e0; base < e1; base += kWave) {
+      uint32_t mid[QM];
+      T mcf = 1;
+      k.load(row, min(base + lane, e1 - 1), mid, mcf);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      cf[u] = 1;
+      for (int u0 = 0; u0 < V; u0 += UB) {
+        if (base + u0 * S >= e1) break;
+        VecT<T, NV> g[UB][QM];
+        T cf[UB];
+        bool ok[UB];
+        uint32_t e[UB];
 #pragma unroll
-      for (int t = 0; t < QM; ++t) id[u][t] = 0;
-      if (e + u * S < e1) k.load(row, e + u * S, id[u], cf[u]);
-    }
-    VecT<T, NV> g[U][QM];
+        for (int u = 0; u < UB; ++u) {
+          uint32_t id[QM];
+          const int src = (u0 + u) * S + slot;
+          coop_take<K, T, QM>(mid, mcf, src, id, cf[u]);
+          e[u] = base + src;
+          ok[u] = e[u] < e1;
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int t = 0; t < QM; ++t) {
-        if (e + u * S < e1) {
-          g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
-        } else {
-#pragma unroll
-          for (int c = 0; c < NV; ++c) g[u][t].v[c] = 0;
+          for (int t = 0; t < QM; ++t) g[u][t] = vload<T, NV>(k.X + (int64_t)id[t] * KP + part * NV);
         }
+        add_terms<K, T, UB, QM, NV>(k, g, cf, ok, e, part * NV, acc);
       }
+    }
+  } else {
+    constexpr int U = kVecU;
+    for (uint32_t e = e0 + slot; e < e1; e += U * S) {
+      uint32_t id[U][QM];
+      T cf[U];
+      bool ok[U];
+      uint32_t ee[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      VecT<T, NV> term;
-#pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        T s = 0;
-#pragma unroll
-        for (int t = 0; t < QM; ++t) s += g[u][t].v[c];
-        term.v[c] = K::COEF ? cf[u] * s : s;
-        if (e + u * S < e1) acc[c] += term.v[c];
+      for (int u = 0; u < U; ++u) {
+        cf[u] = 1;
+        ee[u] = e + u * S;
+        ok[u] = ee[u] < e1;
+        k.load(row, ok[u] ? ee[u] : e1 - 1, id[u], cf[u]);
       }
-      if constexpr (emits<K>::value)
-        if (e + u * S < e1) k.template emit<NV>(e + u * S, part * NV, term);
+      VecT<T, NV> g[U][QM];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < QM; ++t) g[u][t] = vload<T, NV>(k.X + (int64_t)id[u][t] * KP + part * NV);
+      add_terms<K, T, U, QM, NV>(k, g, cf, ok, ee, part * NV, acc);
     }
   }
   // combine the S slots (lanes V apart) in a fixed order

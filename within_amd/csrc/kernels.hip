@@ -1249,6 +1249,7 @@ struct KeepKind {
 template <typename T>
 struct ElimKind {
   static constexpr int QM = 1;
+  static constexpr bool SHARED_IDX = false;  // gathers hit a small, hot vector
   static constexpr bool COEF = true;
   const T* X;  // Rr (solved reduced system)
   T* S;
@@ -1293,6 +1294,7 @@ struct ElimKind {
 template <typename T>
 struct FwdKind {
   static constexpr int QM = 1;
+  static constexpr bool SHARED_IDX = false;  // gathers hit a small, hot vector
   static constexpr bool COEF = true;
   T* X;  // Z
   const uint32_t *rows, *ptr, *idx;
@@ -1329,6 +1331,7 @@ struct FwdKind {
 template <typename T>
 struct BwdKind {
   static constexpr int QM = 1;
+  static constexpr bool SHARED_IDX = false;  // gathers hit a small, hot vector
   static constexpr bool COEF = true;
   T* X;
   const uint32_t *rows, *ptr, *idx;
@@ -1365,6 +1368,7 @@ struct BwdKind {
 template <typename T>
 struct CoreKind {
   static constexpr int QM = 1;
+  static constexpr bool SHARED_IDX = false;  // gathers hit a small, hot vector
   static constexpr bool COEF = true;
   const T* X;  // Z
   T* out;
