@@ -249,9 +249,59 @@ wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz
  * passes (-1 where the solver has no such operator). */
 wa_status wa_solver_autotune(wa_solver* s, int32_t k, int32_t reps, int64_t* out8);
 
+/* ---- learned coarse-space correction -----------------------------------------
+ * No counterpart in the reference: it extends the CG loop
+ * (schwarz-precond/src/solve/cg.rs:45-133) beside FePreconditioner
+ * (crates/within/src/operator/preconditioner.rs:47-154).  A three-layer graph
+ * network over the factor-level graph (the Gramian's sparsity; BF16 matrix
+ * cores for its dense layers, the solver's FP32 operator for aggregation)
+ * outputs 32 coarse vectors P.  It is trained on the device (Adam) on error /
+ * residual pairs of the plain solver's own CG iterations, to maximise the
+ * share of their G-energy that the coarse space captures.  When enabled, CG
+ * (additive Schwarz only) runs in its flexible form with the balancing
+ * two-level preconditioner z = Q r + (I - Q G) M (I - G Q) r,
+ * Q = P (P'GP)^+ P'.  Plain Schwarz-PCG is the fallback: by default the
+ * correction is enabled only when it cuts the iterations of a validation
+ * block; the FP64 residual of the refinement loop judges every solve. */
+typedef struct {
+  int32_t steps;          /* Adam steps (default 200; 0 = prepare and evaluate only) */
+  int32_t sample_iters;   /* plain CG iterations behind the error samples (default 8) */
+  uint64_t seed;
+  double lr;              /* Adam step size (default 3e-3) */
+  int32_t force;          /* 0: enable when it cuts validation iterations; 1: enable; -1: off */
+  int32_t reserved;
+  double ridge;           /* shift of P'GP in the loss, relative to its mean diagonal (default 1e-4) */
+} wa_coarse_params;
+typedef struct {
+  int32_t steps, enabled;
+  double loss_first, loss_last;       /* minus the captured share of the samples' G-energy */
+  double iters_plain, iters_coarse;   /* mean FP32 CG iterations of the validation block (tol 1e-6) */
+  double ms_plain, ms_coarse;         /* its solve times */
+  double seconds;                     /* wall time of wa_coarse_train */
+} wa_coarse_report;
+void wa_coarse_params_default(wa_coarse_params* p);
+/* Builds the training set, trains, validates and enables or not (see force). */
+wa_status wa_coarse_train(wa_solver* s, const wa_coarse_params* p, wa_coarse_report* out);
+/* Switches a trained correction on or off (later solves). */
+wa_status wa_coarse_enable(wa_solver* s, int32_t on);
+/* Loss and gradient (kGnnParams = 6240 floats) at the given parameters on the
+ * current training set; the parameters replace the network's. */
+wa_status wa_coarse_eval(wa_solver* s, const float* theta, double* loss, float* grad);
+/* Copies internal data for checking (internal DOF numbering, m rows):
+ * 0 features (m x 32 f32), 1 error samples E (m x 32 f32), 2 residuals R = G E,
+ * 3 parameters (6240 f32), 4 internal -> user DOF (m u32), 5 nu = sum e'r (1 f64),
+ * 6 coarse space P (m x 32, f32 values of the BF16 copy), 7 GP (the same),
+ * 8 (P'GP)^+ (32 x 32 f64), 9 D^-1/2 (m f32), and of the last evaluation
+ * (m x 32 f32): 10 dL/dP, 11 the network output P, 12 G P.  *len = element
+ * count; buf may be
+ * NULL to query it. */
+wa_status wa_coarse_export(const wa_solver* s, int32_t what, void* buf, int64_t cap, int64_t* len);
+/* The two-level preconditioner apply (FP64) on k user-numbered columns. */
+wa_status wa_coarse_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld);
+
 /* ---- kernel timing (HIP events on the solver's stream) ---------------------
  * Kernel classes: 0 operator, 1 preconditioner, 2 vector ops, 3 observation
- * passes, 4 explicit operator.  Returns launches and summed milliseconds
+ * passes, 4 explicit operator, 5 coarse correction.  Returns launches and summed milliseconds
  * since the last reset.  Timing adds event records: off unless enabled. */
 void wa_timing_enable(wa_solver* s, int32_t on);
 void wa_timing_reset(wa_solver* s);

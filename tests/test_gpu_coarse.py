@@ -1,0 +1,243 @@
+"""The learned coarse-space correction on the GPU against the oracle (oracle/gnn.py).
+
+* features, D^-1/2, the error samples' residuals R = G E and nu against the
+  oracle's own computation from the design;
+* the training loss and its gradient (BF16 matrix-core layers, FP32
+  aggregation) against the oracle's FP64 restatement with the same BF16
+  rounding points;
+* the trained coarse space: (P'GP)^+ and the two-level apply against dense
+  algebra and the oracle's one-level Schwarz;
+* solves with the correction against the oracle's solve (1e-8), graph replay
+  bit for bit, and the fallback rule (enabled only when it cuts iterations;
+  switched off, solves are bitwise those of an untrained solver).
+"""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import within_amd as W
+from within_amd import _native as N
+from within_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SEED = 5
+RIDGE = 1e-4
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def export(s, what, dtype):
+    L = N.load()
+    n = C_int64()
+    N.check(L.wa_coarse_export(s._h, what, None, 0, n))
+    out = np.zeros(n.value, dtype=dtype)
+    N.check(L.wa_coarse_export(s._h, what, N.ptr(out), n.value, n))
+    return out
+
+
+def C_int64():
+    import ctypes
+
+    return ctypes.c_int64()
+
+
+def precond():
+    return W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, 1)))
+
+
+@pytest.fixture(scope="module")
+def prob(oracle_mod):
+    p = synth.akm_panel(60000, 6000, 600, 12, mobility=0.02, seed=SEED, weights="lognormal")
+    codes, w = p["codes"], p["weights"]
+    Y = synth.akm_response(p, k=4)
+    o = oracle_mod.OracleSolver(codes, w, explicit_op=True, tol=1e-10)
+    ip, ix, dt = o.explicit_gramian()
+    mu = o.n_dofs
+    Gu = sp.csr_matrix((dt, ix.astype(np.int64), ip.astype(np.int64)), shape=(mu, mu))
+    s = W.Solver(codes, weights=w, config=W.CG(tol=1e-10, operator=W.OperatorRepr.Explicit, precision="mixed"),
+                 preconditioner=precond())
+    rep0 = s.train_coarse(steps=0, seed=SEED, force=False, ridge=RIDGE)
+    ext = export(s, 4, np.uint32).astype(np.int64)
+    G = Gu[ext][:, ext].tocsr()
+    levels = codes.max(axis=0).astype(np.int64) + 1
+    off = np.concatenate([[0], np.cumsum(levels)])
+    factor = np.searchsorted(off, ext, side="right") - 1
+    return dict(codes=codes, w=w, Y=Y, o=o, s=s, G=G, ext=ext, factor=factor, Q=codes.shape[1], mu=mu, rep0=rep0)
+
+
+def test_features_samples_and_scaling(prob):
+    from oracle import gnn
+
+    s, G = prob["s"], prob["G"]
+    m = G.shape[0]
+    d = np.asarray(G.diagonal())
+    X0 = export(s, 0, np.float32).reshape(m, 32)
+    want = gnn.features(d, prob["factor"], prob["ext"], SEED)
+    np.testing.assert_allclose(X0, want, rtol=0, atol=2e-6)
+    dinv = export(s, 9, np.float32)
+    np.testing.assert_allclose(dinv, 1.0 / np.sqrt(d), rtol=1e-6)
+    E = export(s, 1, np.float32).reshape(m, 32).astype(np.float64)
+    R = export(s, 2, np.float32).reshape(m, 32).astype(np.float64)
+    assert rel(R, G @ E) < 1e-5
+    nu = export(s, 5, np.float64)[0]
+    assert abs(nu - np.sum(E * R)) < 1e-9 * nu
+    # the samples are errors of the plain method: far smaller than the random start
+    assert prob["rep0"]["loss_first"] <= 0.0
+
+
+def test_loss_and_gradient_match_oracle(prob):
+    """Stage by stage, each on the device's own inputs of that stage: a BF16
+    operand that sits on a rounding boundary can round one way from the
+    device's FP32 value and the other way from the oracle's FP64 value, and
+    P'GP is ill-conditioned at a random start (condition ~1e6), which
+    amplifies such differences; comparing each stage on identical inputs
+    keeps the checks tight."""
+    from oracle import gnn
+
+    s, G = prob["s"], prob["G"]
+    m = G.shape[0]
+    L = N.load()
+    X0 = export(s, 0, np.float32).reshape(m, 32).astype(np.float64)
+    dinv = export(s, 9, np.float32).astype(np.float64)
+    E = export(s, 1, np.float32).reshape(m, 32).astype(np.float64)
+    R = export(s, 2, np.float32).reshape(m, 32).astype(np.float64)
+    nu = export(s, 5, np.float64)[0]
+    rng = np.random.default_rng(11)
+    for trial in range(2):
+        a = np.sqrt(6.0 / 96.0)
+        theta = rng.uniform(-a, a, gnn.N_PARAMS).astype(np.float32)
+        th = theta.astype(np.float64)
+        grad = np.zeros(gnn.N_PARAMS, dtype=np.float32)
+        loss = np.zeros(1)
+        N.check(L.wa_coarse_eval(s._h, N.ptr(theta), N.ptr(loss), N.ptr(grad)))
+        Pd = export(s, 11, np.float32).reshape(m, 32).astype(np.float64)
+        GPd = export(s, 12, np.float32).reshape(m, 32).astype(np.float64)
+        dPd = export(s, 10, np.float32).reshape(m, 32).astype(np.float64)
+        # 1. forward (BF16 matrix-core layers, FP32 aggregation)
+        P, cache = gnn.forward(th, X0, G, dinv, prob["Q"], mimic=True)
+        e_p = rel(Pd, P)
+        # 2. G P (FP32 operator) and the loss with its dL/dP (FP64) at the device's P
+        e_gp = rel(GPd, G @ Pd)
+        lo, dP = gnn.coarse_loss(Pd, GPd, R, nu, RIDGE)
+        e_loss = abs(loss[0] - lo) / abs(lo)
+        e_dp = rel(dPd, dP)
+        # 3. backward (BF16 matrix cores, FP32 aggregation) from the device's dL/dP
+        sc = []
+        go = gnn.backward(th, cache, dPd, G, dinv, mimic=True, scale=sc)
+        print(f"trial {trial}: loss {loss[0]:.8f}; rel err: P {e_p:.2e}, GP {e_gp:.2e}, loss {e_loss:.2e}, "
+              f"dL/dP {e_dp:.2e}")
+        assert e_p < 1e-4 and e_gp < 1e-6 and e_loss < 1e-9 and e_dp < 1e-6, (e_p, e_gp, e_loss, e_dp)
+        # Each gradient entry is a sum over the m DOFs whose terms cancel
+        # heavily at a random start; the device sums them in FP32 in another
+        # order.  So each entry's error is measured against its sum of
+        # absolute terms: at most 2^-6 of it anywhere (one BF16 ulp of one
+        # term: a rounding flip) and under 1e-4 of it for 99 % of the entries.
+        fails = []
+        for l in range(gnn.LAYERS):
+            o = l * gnn.LAYER_PARAMS
+            for name, a0, a1 in (("Ws", o, o + gnn.MAT), ("Wa", o + gnn.MAT, o + 2 * gnn.MAT),
+                                 ("b", o + 2 * gnn.MAT, o + gnn.LAYER_PARAMS)):
+                e = rel(grad[a0:a1], go[a0:a1])
+                r = np.abs(grad[a0:a1] - go[a0:a1]) / np.maximum(sc[0][a0:a1], 1e-300)
+                frac = float(np.mean(r > 1e-4))
+                print(f"  layer {l} d{name}: rel err {e:.2e}, per-entry err / abs-sum scale: max {r.max():.2e}, "
+                      f"share above 1e-4 {frac:.3f}")
+                if not (r.max() <= 2.0 ** -6 and frac <= 0.01):
+                    fails.append((l, name, e, r.max(), frac))
+        assert not fails, fails
+
+
+@pytest.fixture(scope="module")
+def trained(prob):
+    s = prob["s"]
+    rep = s.train_coarse(steps=80, seed=SEED, force=True, ridge=RIDGE)
+    print("training report:", rep)
+    return rep
+
+
+def test_training_lowers_the_loss(trained):
+    assert trained["enabled"] == 1
+    assert trained["loss_last"] < trained["loss_first"]
+    assert -1.0 <= trained["loss_last"] <= 0.0
+
+
+def test_coarse_space_and_two_level_apply_match_oracle(prob, trained):
+    from oracle import gnn
+
+    s, G, o, ext, mu = prob["s"], prob["G"], prob["o"], prob["ext"], prob["mu"]
+    m = G.shape[0]
+    P = export(s, 6, np.float32).reshape(m, 32).astype(np.float64)
+    GP = export(s, 7, np.float32).reshape(m, 32).astype(np.float64)
+    Ainv = export(s, 8, np.float64).reshape(32, 32)
+    assert rel(GP, G @ P) < 1e-2  # BF16 copy of the FP32 product
+    A = P.T @ GP
+    np.testing.assert_allclose(Ainv, gnn.sym_pinv(0.5 * (A + A.T)), rtol=0, atol=1e-6 * np.abs(Ainv).max())
+
+    def M(v):
+        u = np.zeros(mu)
+        u[ext] = v
+        return o.precond_apply(u)[ext]
+
+    rng = np.random.default_rng(2)
+    k = 3
+    Ru = np.zeros((mu, k), order="F")
+    Ru[ext] = rng.standard_normal((m, k))
+    Zu = np.zeros_like(Ru)
+    N.check(N.load().wa_coarse_precond_apply(s._h, N.ptr(Ru), N.ptr(Zu), k, mu))
+    for c in range(k):
+        want = gnn.two_level_apply(P, GP, Ainv, M, Ru[ext, c])
+        e = rel(Zu[ext, c], want)
+        print(f"two-level apply column {c}: rel err {e:.2e}")
+        assert e < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_solves_with_the_correction_match_oracle(prob, trained, precision):
+    s, o, Y = prob["s"], prob["o"], prob["Y"]
+    s.set_config(W.CG(tol=1e-10, operator=W.OperatorRepr.Explicit, precision=precision))
+    s.enable_coarse(True)
+    r = s.solve_batch(Y)
+    ro = o.solve_batch(Y)
+    for c in range(Y.shape[1]):
+        e = rel(r.demeaned[:, c], ro["demeaned"][:, c])
+        print(f"{precision} column {c}: iterations {r.iterations[c]} (oracle {ro['iterations'][c]}), err {e:.2e}")
+        assert r.converged[c] and e <= 1e-8
+
+
+def test_graph_replay_with_the_correction_is_bitwise(prob, trained):
+    s, Y = prob["s"], prob["Y"]
+    s.set_config(W.CG(tol=1e-10, operator=W.OperatorRepr.Explicit, precision="mixed"))
+    s.enable_coarse(True)
+    a = s.solve_batch(Y)
+    os.environ["WITHIN_AMD_GRAPHS"] = "0"
+    try:
+        b = s.solve_batch(Y)
+    finally:
+        del os.environ["WITHIN_AMD_GRAPHS"]
+    c = s.solve_batch(Y)
+    np.testing.assert_array_equal(a.demeaned, b.demeaned)
+    np.testing.assert_array_equal(a.demeaned, c.demeaned)
+    assert a.iterations == b.iterations == c.iterations
+
+
+def test_fallback_rule_and_plain_solves_unchanged(prob):
+    codes, w, Y = prob["codes"], prob["w"], prob["Y"]
+    cfg = W.CG(tol=1e-10, operator=W.OperatorRepr.Explicit, precision="mixed")
+    s = W.Solver(codes, weights=w, config=cfg, preconditioner=precond())
+    plain = W.Solver(codes, weights=w, config=cfg, preconditioner=precond()).solve_batch(Y)
+    rep = s.train_coarse(steps=40, seed=SEED + 1)
+    print("fallback report:", rep)
+    assert rep["enabled"] == int(rep["iters_coarse"] < rep["iters_plain"])
+    s.enable_coarse(False)
+    r = s.solve_batch(Y)
+    np.testing.assert_array_equal(r.demeaned, plain.demeaned)
+    assert r.iterations == plain.iterations
+    s2 = W.Solver(codes, weights=w, config=cfg, preconditioner=precond())
+    with pytest.raises(ValueError):
+        s2.enable_coarse(True)

@@ -57,6 +57,34 @@ class RhsStats(C.Structure):
     ]
 
 
+class CoarseParams(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("sample_iters", C.c_int32),
+        ("seed", C.c_uint64),
+        ("lr", C.c_double),
+        ("force", C.c_int32),
+        ("reserved", C.c_int32),
+        ("ridge", C.c_double),
+    ]
+
+
+class CoarseReport(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("enabled", C.c_int32),
+        ("loss_first", C.c_double),
+        ("loss_last", C.c_double),
+        ("iters_plain", C.c_double),
+        ("iters_coarse", C.c_double),
+        ("ms_plain", C.c_double),
+        ("ms_coarse", C.c_double),
+        ("seconds", C.c_double),
+    ]
+
+
+GNN_PARAMS = 3 * (2 * 32 * 32 + 32)  # kGnnParams (coarse.hpp)
+
 EXPORTS = [
     "wa_last_error", "wa_params_default", "wa_design_create", "wa_design_destroy", "wa_design_n_dofs",
     "wa_design_n_obs", "wa_design_n_factors", "wa_design_n_levels", "wa_solver_create", "wa_solver_destroy", "wa_solver_set_params",
@@ -66,7 +94,8 @@ EXPORTS = [
     "wa_solver_create_with_precond", "wa_precond_create", "wa_solver_params", "wa_precond_work",
     "wa_gram_apply_device", "wa_gram_apply_explicit_device", "wa_precond_apply_device", "wa_matvec_d_device",
     "wa_rmatvec_wdt_device", "wa_gram_apply_noalloc", "wa_multi_create", "wa_multi_destroy", "wa_multi_info",
-    "wa_multi_solve", "wa_solver_autotune",
+    "wa_multi_solve", "wa_solver_autotune", "wa_coarse_params_default", "wa_coarse_train", "wa_coarse_enable",
+    "wa_coarse_eval", "wa_coarse_export", "wa_coarse_precond_apply",
 ]
 
 _lib = None
@@ -134,6 +163,13 @@ def load():
     L.wa_multi_info.argtypes = [P, i32, P]
     L.wa_multi_solve.argtypes = [P, P, P, P, P, P, P, P, P]
     L.wa_solver_autotune.argtypes = [P, i32, i32, P]
+    L.wa_coarse_params_default.argtypes = [C.POINTER(CoarseParams)]
+    L.wa_coarse_params_default.restype = None
+    L.wa_coarse_train.argtypes = [P, C.POINTER(CoarseParams), C.POINTER(CoarseReport)]
+    L.wa_coarse_enable.argtypes = [P, i32]
+    L.wa_coarse_eval.argtypes = [P, P, P, P]
+    L.wa_coarse_export.argtypes = [P, i32, P, i64, C.POINTER(i64)]
+    L.wa_coarse_precond_apply.argtypes = [P, P, P, i32, i64]
     _lib = L
     return L
 

@@ -487,6 +487,31 @@ class Solver:
                 "pass_us_default", "pass_us_best")
         return dict(zip(keys, (int(v) for v in out)))
 
+    def train_coarse(self, steps=200, sample_iters=8, seed=0, lr=3e-3, force=None, ridge=1e-4):
+        """Train the learned coarse-space correction on this solver's own CG
+        iterations and keep it for later solves when it cuts iterations.
+
+        A three-layer graph network over the factor-level graph outputs 32
+        coarse vectors; with it, CG runs in its flexible form with the
+        balancing two-level preconditioner.  ``force=True`` / ``False``
+        overrides the validation's decision.  Returns the training report
+        (losses, mean validation iterations and solve times with and without
+        the correction, whether it is enabled).  No counterpart in the
+        reference."""
+        p = N.CoarseParams()
+        L = N.load()
+        L.wa_coarse_params_default(C.byref(p))
+        p.steps, p.sample_iters, p.seed, p.lr = int(steps), int(sample_iters), int(seed), float(lr)
+        p.ridge = float(ridge)
+        p.force = 0 if force is None else (1 if force else -1)
+        r = N.CoarseReport()
+        N.check(L.wa_coarse_train(self._h, C.byref(p), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in N.CoarseReport._fields_}
+
+    def enable_coarse(self, on=True):
+        """Switch a trained coarse correction on or off for later solves."""
+        N.check(N.load().wa_coarse_enable(self._h, 1 if on else 0))
+
     def _solve_cols(self, Y):
         n, m = self.n_obs, self.n_dofs
         Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))

@@ -1,7 +1,7 @@
 """Build the in-tree native library ``within_amd/_lib/libwithin_amd.so``.
 
 * ``csrc/plan.cpp`` (host setup, OpenMP) with g++;
-* ``csrc/kernels.hip``, ``csrc/multi.hip``, ``csrc/gmres.hip``, ``csrc/dense.hip`` and ``csrc/runtime.cpp`` (device
+* ``csrc/kernels.hip``, ``csrc/multi.hip``, ``csrc/gmres.hip``, ``csrc/coarse.hip``, ``csrc/dense.hip`` and ``csrc/runtime.cpp`` (device
   code + C ABI) with hipcc for gfx950, packed FP32 instructions disabled
   (``dense.hip`` also without FP contraction);
 * linked into one shared library exporting the symbols of
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
     # no FP contraction: host factors must match the oracle bit for bit
     jobs.append((["g++", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-c",
                   os.path.join(CSRC, "plan.cpp"), "-o", plan_o], plan_o))
-    for src in ("kernels.hip", "multi.hip", "gmres.hip", "runtime.cpp", "dense.hip"):
+    for src in ("kernels.hip", "multi.hip", "gmres.hip", "coarse.hip", "runtime.cpp", "dense.hip"):
         o = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         # the dense-core inverse must round like the host factorisation

@@ -18,6 +18,7 @@
 
 #include "../../include/within_amd.h"
 #include "kernels.hpp"
+#include "coarse.hpp"
 #include "gmres.hpp"
 #include "multi.hpp"
 #include "plan.hpp"
@@ -66,8 +67,10 @@ struct Timer {
   struct Rec { int cls; hipEvent_t a, b; };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
-  int64_t launches[5] = {0, 0, 0, 0, 0};
-  double ms[5] = {0, 0, 0, 0, 0};
+  // classes: 0 implicit operator, 1 preconditioner, 2 vector work,
+  // 3 observation space, 4 explicit Gramian, 5 coarse correction
+  int64_t launches[6] = {0, 0, 0, 0, 0, 0};
+  double ms[6] = {0, 0, 0, 0, 0, 0};
   hipEvent_t get() {
     if (pool.empty()) {
       hipEvent_t e;
@@ -156,7 +159,9 @@ struct wa_solver {
   // captured CG iterations, one per (precision, block width, x vector, side
   // x-update, operator representation); replayed with hipGraphLaunch
   struct GraphKey {
-    int kind, t, kp, op, side;  // kind: 0 block CG, 1 per-problem (stacked) CG
+    // kind: 0 block CG, 1 per-problem (stacked) CG, 2 flexible block CG with
+    // the coarse correction
+    int kind, t, kp, op, side;
     const void* x;
   };
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
@@ -188,6 +193,31 @@ struct wa_solver {
   double* mw64 = nullptr;                  // r_work, delta, G delta (multiplicative)
   float* mw32 = nullptr;
   double* ones = nullptr;
+  // Gramian diagonal of each internal DOF and the internal offset of each
+  // factor (inputs of the coarse-space network)
+  std::vector<double> h_diag;
+  std::vector<int64_t> h_foff;
+  // learned coarse-space correction (coarse.hpp), allocated on first use:
+  // network inputs and activations (m x 32 FP32), training samples, the
+  // parameters with their Adam moments, and the coarse space of the apply
+  struct CoarseNet {
+    bool ready = false, trained = false, enabled = false;
+    uint64_t seed = 0;
+    float *X0 = nullptr, *dinv = nullptr, *cvec = nullptr;
+    float *H[kGnnLayers] = {}, *S[kGnnLayers] = {}, *T[kGnnLayers] = {};
+    float *P = nullptr, *GP = nullptr, *E = nullptr, *R = nullptr, *dP = nullptr, *dZa = nullptr, *dZb = nullptr;
+    float *Ys = nullptr, *Tb = nullptr;
+    float *theta = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *xpart = nullptr;
+    double* W12 = nullptr;
+    double ridge = 1e-4;
+    bf16_t* pack = nullptr;
+    double *xpart64 = nullptr, *Ac = nullptr, *U = nullptr, *V = nullptr, *loss = nullptr;
+    double nu = 0.0;
+    bf16_t *Pb = nullptr, *GPb = nullptr;
+    double *Ainv = nullptr, *cpart = nullptr, *c1 = nullptr, *cd = nullptr, *r2 = nullptr, *dpart = nullptr;
+    std::vector<float> h_feat;
+  } cn;
+  bool fcg = false;  // set by cg() while the coarse correction is part of its preconditioner
   // device GMRES state (allocated on first use): column states, Hessenberg
   // columns, the two Gram-Schmidt passes' coefficients, y, per-column scalars,
   // reduction partials, per-step extension counts and their events
@@ -332,6 +362,17 @@ struct wa_solver {
     else cg<T>(k, kp, rhs, x, tol, conv, its);
   }
   void lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>& its);
+  // learned coarse correction
+  bool coarse_supported() const { return params.preconditioner == 1 && has_pre && m > 0; }
+  void coarse_alloc(uint64_t seed);
+  void coarse_init_params(uint64_t seed);
+  void coarse_samples(int iters, uint64_t seed);
+  void gnn_forward();
+  void gnn_loss(bool grad);
+  double gnn_read_loss();
+  void coarse_finalize();
+  double coarse_validate(bool with, std::vector<int64_t>& its);
+  template <typename T> void coarse_apply(const T* R, T* Z, int kp, const SideWork* side);
   void solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
                    int64_t ldd, wa_rhs_stats* stats);
   std::vector<double> col_norms(const double* V, int64_t rows, int kp);
@@ -418,6 +459,10 @@ void wa_solver::setup_device() {
     if (!d.weights.empty()) wobs = upload(d.weights);
   }
   int64_t max_slots = 1;
+  h_foff = lay.off;
+  h_diag.assign((size_t)m, 0.0);
+  for (const FactorPass& fp : lay.passes)
+    for (size_t j = 0; j < fp.diag.size(); ++j) h_diag[(size_t)lay.off[fp.q] + j] = fp.diag[j];
   for (const FactorPass& fp : lay.passes) {
     DevPass<double> a;
     DevPass<float> b;
@@ -886,6 +931,10 @@ void wa_solver::pre_apply(const T* R, T* Z, int kp, const SideWork* side) {
 // (schwarz/additive/preconditioner.rs:164-214, multiplicative.rs:128-245).
 template <typename T>
 void wa_solver::precondition(const T* R, T* Z, int kp, const SideWork* side) {
+  if (fcg) {
+    coarse_apply<T>(R, Z, kp, side);
+    return;
+  }
   const int64_t bytes = sizeof(T) * m * kp;
   if (params.preconditioner == 0) HIPCHK(hipMemcpyAsync(Z, R, bytes, hipMemcpyDeviceToDevice, st));
   else if (has_pre) {
@@ -983,6 +1032,15 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
   const int64_t bytes = sizeof(T) * m * kp;
   double *bn2 = s_(0), *rz = s_(1), *pap = s_(2), *alpha = s_(3), *rr = s_(4), *rzn = s_(5), *beta = s_(6),
          *tolv = s_(7);
+  // with the learned coarse correction the preconditioner is the balancing
+  // two-level one (coarse_apply), stored in BF16, so CG runs in its flexible
+  // form (beta = -<z, Ap> / <p, Ap>)
+  const bool flex = cn.enabled && coarse_supported();
+  struct FcgFlag {
+    bool& f;
+    FcgFlag(bool& x, bool v) : f(x) { f = v; }
+    ~FcgFlag() { f = false; }
+  } fcg_flag(fcg, flex);
   int* mask = imask;
   int* nact = imask + KMAX;
   std::vector<double> tp(KMAX, 0.0);
@@ -1026,12 +1084,17 @@ void wa_solver::cg(int k, int kp, const T* rhs, T* x, const std::vector<double>&
     toc(2, ev);
     precondition<T>(r, z, kp, side_x ? &xside : nullptr);
     tic(2, ev);
-    nb = launch_dot_part<T>(st, r, z, m, kp, redp);
-    launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, kp);
+    if (flex) {
+      nb = launch_dot2_part<T>(st, z, r, ap, m, kp, cn.dpart);
+      launch_fcg_beta(st, cs, cn.dpart, nb, pap, rzn, beta, mask, nact, kp);
+    } else {
+      nb = launch_dot_part<T>(st, r, z, m, kp, redp);
+      launch_cg_beta(st, cs, redp, nb, rzn, beta, mask, nact, kp);
+    }
     launch_xpby<T>(st, p, z, beta, mask, m, kp);
     toc(2, ev);
   };
-  hipGraphExec_t graph = use_graphs() ? cg_graph<T>(kp, x, side_x, iteration) : nullptr;
+  hipGraphExec_t graph = use_graphs() ? cg_graph<T>(kp, x, side_x, iteration, flex ? 2 : 0) : nullptr;
   std::vector<ColState> hs(kp);
   auto read_state = [&]() {
     HIPCHK(hipMemcpyAsync(hs.data(), cs, sizeof(ColState) * kp, hipMemcpyDeviceToHost, st));
@@ -1477,6 +1540,327 @@ void wa_solver::lsmr(int k, int kp, std::vector<int>& conv, std::vector<int64_t>
       its[c] = h[c].iters;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Learned coarse-space correction (coarse.hpp).
+//
+// Network over the factor-level graph (the Gramian's sparsity): H_0 = input
+// features (m x 32) and, for layer l,
+//   Z_l = H_l Ws_l + (A H_l) Wa_l + b_l,   A = (1/Q) D^-1/2 G D^-1/2,
+//   H_{l+1} = ReLU(Z_l) (hidden layers),   P = Z_{L-1} (32 coarse vectors).
+// A H is the solver's own FP32 operator applied to D^-1/2 H.  The loss, on
+// error samples E (m x S) of the plain solver with residuals R = G E, is
+//   L(P) = -<U, (P'GP + eps)^-1 U> / nu,   U = P'R,   nu = sum_s e_s' r_s:
+// minus the share of the samples' G-energy that the G-orthogonal projection
+// onto span(P) removes, i.e. what a coarse correction with P would take out
+// of the errors the one-level method leaves.  With V = (P'GP + eps)^-1 U,
+//   dL/dP = -(2/nu) (R - GP V) V'.
+// eps (the ridge, relative to the mean diagonal of P'GP) bounds the
+// amplification of FP32 rounding in GP by ill-conditioned P'GP.
+// ---------------------------------------------------------------------------
+static uint64_t smix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// N(0, 1) from the counter i (Box-Muller on two 53-bit uniforms)
+static double host_randn(uint64_t seed, uint64_t i) {
+  const uint64_t h1 = smix64(seed ^ smix64(2 * i)), h2 = smix64(seed ^ smix64(2 * i + 1));
+  const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)(h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// Input features of internal DOF i (user DOF u, factor q, Gramian diagonal d):
+//   [1, z, one-hot(q mod 4) (4 columns), z^2 - 1, 25 N(0,1) values keyed by u],
+// z = standardised log d.  The random columns let the layers form smooth
+// functions that differ between regions of the graph.
+void wa_solver::coarse_alloc(uint64_t seed) {
+  const int64_t mm = m * kCoarse;
+  if (!cn.ready) {
+    cn.X0 = dalloc<float>(mm);
+    cn.dinv = dalloc<float>(m);
+    cn.cvec = dalloc<float>(m);
+    cn.H[0] = cn.X0;
+    for (int l = 1; l < kGnnLayers; ++l) cn.H[l] = dalloc<float>(mm);
+    for (int l = 0; l < kGnnLayers; ++l) {
+      cn.S[l] = dalloc<float>(mm);
+      cn.T[l] = dalloc<float>(mm);
+    }
+    for (float** b : {&cn.P, &cn.GP, &cn.E, &cn.R, &cn.dP, &cn.dZa, &cn.dZb, &cn.Ys, &cn.Tb}) *b = dalloc<float>(mm);
+    for (float** b : {&cn.theta, &cn.grad, &cn.m1, &cn.m2}) *b = dalloc<float>(kGnnParams);
+    cn.pack = dalloc<bf16_t>(kGnnLayers * kLayerPack);
+    cn.xpart = dalloc<float>((int64_t)kXtyBlocksMax * (kMat + kCoarse));
+    cn.W12 = dalloc<double>(2 * kMat);
+    cn.xpart64 = dalloc<double>((int64_t)kXtyBlocksMax * kMat);
+    for (double** b : {&cn.Ac, &cn.U, &cn.V, &cn.Ainv, &cn.c1, &cn.cd}) *b = dalloc<double>(kMat);
+    cn.loss = dalloc<double>(1);
+    cn.Pb = dalloc<bf16_t>(mm);
+    cn.GPb = dalloc<bf16_t>(mm);
+    cn.cpart = dalloc<double>((int64_t)kCprojBlocksMax * kMat);
+    cn.r2 = dalloc<double>(mm);
+    cn.dpart = dalloc<double>(2 * (int64_t)kRedBlocks * KMAX);
+    cn.ready = true;
+  }
+  cn.seed = seed;
+  std::vector<float> dinv((size_t)m), cvec((size_t)m);
+  std::vector<double> z((size_t)m);
+  double mean = 0.0, var = 0.0;
+  for (int64_t i = 0; i < m; ++i) {
+    const double d = h_diag[(size_t)i];
+    z[(size_t)i] = std::log(std::max(d, 1e-300));
+    dinv[(size_t)i] = d > 0.0 ? (float)(1.0 / std::sqrt(d)) : 0.0f;
+    cvec[(size_t)i] = d > 0.0 ? (float)(1.0 / std::sqrt(d) / Q) : 0.0f;
+    mean += z[(size_t)i];
+  }
+  mean /= (double)m;
+  for (int64_t i = 0; i < m; ++i) var += (z[(size_t)i] - mean) * (z[(size_t)i] - mean);
+  const double sd = std::max(std::sqrt(var / (double)m), 1e-12);
+  const uint64_t fseed = smix64(seed + 1);
+  cn.h_feat.assign((size_t)mm, 0.0f);
+  for (int64_t i = 0; i < m; ++i) {
+    float* f = cn.h_feat.data() + i * kCoarse;
+    int q = 0;
+    while (q + 1 < (int)h_foff.size() && i >= h_foff[(size_t)q + 1]) ++q;
+    const double zi = (z[(size_t)i] - mean) / sd;
+    const uint64_t u = h_ext_of[(size_t)i];
+    f[0] = 1.0f;
+    f[1] = (float)zi;
+    f[2 + (q & 3)] = 1.0f;
+    f[6] = (float)(zi * zi - 1.0);
+    for (int c = 7; c < kCoarse; ++c) f[c] = (float)host_randn(fseed, u * kCoarse + (uint64_t)c);
+  }
+  HIPCHK(hipMemcpyAsync(cn.X0, cn.h_feat.data(), sizeof(float) * mm, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cn.dinv, dinv.data(), sizeof(float) * m, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cn.cvec, cvec.data(), sizeof(float) * m, hipMemcpyHostToDevice, st));
+  launch_rowscale(st, m, cn.dinv, cn.X0, cn.S[0]);
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// Glorot-uniform weights (fan-in 64: two 32-wide inputs), zero biases, zero
+// Adam moments
+void wa_solver::coarse_init_params(uint64_t seed) {
+  std::vector<float> th((size_t)kGnnParams, 0.0f), zero((size_t)kGnnParams, 0.0f);
+  const double a = std::sqrt(6.0 / (64.0 + 32.0));
+  const uint64_t ps = smix64(seed + 2);
+  for (int l = 0; l < kGnnLayers; ++l)
+    for (int i = 0; i < 2 * kMat; ++i) {
+      const uint64_t h = smix64(ps ^ smix64((uint64_t)l * kLayerParams + (uint64_t)i));
+      const double u = ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      th[(size_t)l * kLayerParams + i] = (float)(a * (2.0 * u - 1.0));
+    }
+  HIPCHK(hipMemcpyAsync(cn.theta, th.data(), sizeof(float) * kGnnParams, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cn.m1, zero.data(), sizeof(float) * kGnnParams, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cn.m2, zero.data(), sizeof(float) * kGnnParams, hipMemcpyHostToDevice, st));
+  launch_gnn_pack(st, cn.theta, cn.pack);
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// Error samples of the plain method: right-hand sides b = G xi with xi ~ N(0,1)
+// (so xi solves G x = b up to the null space of G), `iters` iterations of
+// FP32 block CG with the one-level preconditioner, then E = xi - x_iters and
+// R = G E (32 samples).
+void wa_solver::coarse_samples(int iters, uint64_t seed) {
+  float* xi = cn.dP;
+  launch_randn(st, m * kCoarse, smix64(seed + 3), xi);
+  op_apply<float>(xi, cn.Ys, kCoarse);
+  const int64_t saved = params.maxiter;
+  const bool en = cn.enabled;
+  cn.enabled = false;
+  params.maxiter = std::max(iters, 1);
+  std::vector<double> tol(KMAX, 0.0);
+  std::vector<int> conv(KMAX, 0);
+  std::vector<int64_t> its(KMAX, 0);
+  try {
+    cg<float>(KMAX, KMAX, cn.Ys, cx32, tol, conv, its);
+  } catch (...) {
+    params.maxiter = saved;
+    cn.enabled = en;
+    throw;
+  }
+  params.maxiter = saved;
+  cn.enabled = en;
+  launch_sub(st, m * kCoarse, xi, cx32, cn.E);
+  op_apply<float>(cn.E, cn.R, kCoarse);
+  launch_dot<float>(st, cn.E, cn.R, m, KMAX, redp, s_(12));
+  HIPCHK(hipMemcpyAsync(hbuf, s_(12), sizeof(double) * KMAX, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double nu = 0.0;
+  for (int c = 0; c < KMAX; ++c) nu += hbuf[c];
+  if (!(nu > 0.0)) throw Error(WA_ERR_ITERATIVE_SOLVE, "coarse training: the error samples carry no G-energy");
+  cn.nu = nu;
+}
+
+void wa_solver::gnn_forward() {
+  for (int l = 0; l < kGnnLayers; ++l) {
+    const bf16_t* pk = cn.pack + (size_t)l * kLayerPack;
+    const float* b = cn.theta + (size_t)l * kLayerParams + 2 * kMat;
+    const bool last = l == kGnnLayers - 1;
+    op_apply<float>(cn.S[l], cn.T[l], kCoarse);
+    launch_gnn_layer(st, m, cn.H[l], pk, cn.T[l], cn.cvec, pk + kMat, b, nullptr, last ? 0 : 1,
+                     last ? cn.P : cn.H[l + 1], last ? nullptr : cn.S[l + 1], cn.dinv);
+  }
+}
+
+// loss (device scalar cn.loss) and, with grad, its gradient in cn.grad
+void wa_solver::gnn_loss(bool grad) {
+  op_apply<float>(cn.P, cn.GP, kCoarse);
+  launch_xty64(st, m, cn.P, cn.GP, cn.xpart64, cn.Ac);
+  launch_xty64(st, m, cn.P, cn.R, cn.xpart64, cn.U);
+  launch_coarse_small(st, cn.Ac, cn.U, cn.nu, cn.ridge, cn.V, cn.loss, cn.W12, cn.W12 + kMat);
+  if (!grad) return;
+  launch_tall_small(st, m, cn.R, cn.W12, cn.GP, cn.W12 + kMat, cn.dP);  // dL/dP
+  float* dz = cn.dP;
+  float* nxt = cn.dZa;
+  for (int l = kGnnLayers - 1; l >= 0; --l) {
+    float* g = cn.grad + (size_t)l * kLayerParams;
+    const bf16_t* pk = cn.pack + (size_t)l * kLayerPack;
+    launch_gnn_xty(st, m, cn.H[l], nullptr, dz, cn.xpart, g, g + 2 * kMat);  // dWs, db
+    launch_gnn_xty(st, m, cn.T[l], cn.cvec, dz, cn.xpart, g + kMat, nullptr);  // dWa (input A H_l)
+    if (l == 0) break;
+    // dH_l = dZ Ws' + A (dZ Wa') (A is symmetric); dZ_{l-1} = dH_l o [H_l > 0]
+    launch_gnn_layer(st, m, dz, pk + 3 * kMat, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, cn.Ys,
+                     cn.dinv);
+    op_apply<float>(cn.Ys, cn.Tb, kCoarse);
+    launch_gnn_layer(st, m, dz, pk + 2 * kMat, cn.Tb, cn.cvec, nullptr, nullptr, cn.H[l], 0, nxt, nullptr, nullptr);
+    dz = nxt;
+    nxt = nxt == cn.dZa ? cn.dZb : cn.dZa;
+  }
+}
+
+double wa_solver::gnn_read_loss() {
+  HIPCHK(hipMemcpyAsync(hbuf, cn.loss, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return hbuf[0];
+}
+
+// Pseudo-inverse of a symmetric n x n matrix by cyclic Jacobi rotations;
+// eigenvalues below 1e-10 of the largest are dropped.
+static std::vector<double> sym_pinv(std::vector<double> A, int n) {
+  std::vector<double> V((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  double norm2 = 0.0;
+  for (double v : A) norm2 += v * v;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+    if (off <= 1e-32 * norm2) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double th = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // A J
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // J' (A J)
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {  // V J
+          const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = c * vkp - s * vkq;
+          V[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  double wmax = 0.0;
+  for (int i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(A[(size_t)i * n + i]));
+  std::vector<double> out((size_t)n * n, 0.0);
+  for (int e = 0; e < n; ++e) {
+    const double w = A[(size_t)e * n + e];
+    if (!(w > 1e-10 * wmax)) continue;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) out[(size_t)i * n + j] += V[(size_t)i * n + e] * V[(size_t)j * n + e] / w;
+  }
+  return out;
+}
+
+// The coarse space of the apply: P and GP = G P in BF16, and the
+// pseudo-inverse of the (symmetrised) coarse matrix P'(GP) formed from the
+// rounded values, so the apply's projections are consistent with each other.
+void wa_solver::coarse_finalize() {
+  gnn_forward();
+  float* Pf = cn.dP;
+  float* GPf = cn.dZa;
+  launch_to_bf16(st, m * kCoarse, cn.P, cn.Pb, Pf);
+  op_apply<float>(Pf, cn.GP, kCoarse);
+  launch_to_bf16(st, m * kCoarse, cn.GP, cn.GPb, GPf);
+  launch_xty64(st, m, Pf, GPf, cn.xpart64, cn.Ac);
+  std::vector<double> A((size_t)kMat);
+  HIPCHK(hipMemcpyAsync(A.data(), cn.Ac, sizeof(double) * kMat, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < kCoarse; ++i)
+    for (int j = i + 1; j < kCoarse; ++j) {
+      const double v = 0.5 * (A[(size_t)i * kCoarse + j] + A[(size_t)j * kCoarse + i]);
+      A[(size_t)i * kCoarse + j] = A[(size_t)j * kCoarse + i] = v;
+    }
+  const std::vector<double> inv = sym_pinv(A, kCoarse);
+  HIPCHK(hipMemcpyAsync(cn.Ainv, inv.data(), sizeof(double) * kMat, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  cn.trained = true;
+}
+
+// One FP32 block CG of 32 fresh right-hand sides b = G xi to a relative
+// residual of 1e-6, with or without the coarse correction; returns the
+// milliseconds of the second of two identical runs (the first captures the
+// iteration graph) and the per-column iteration counts.
+double wa_solver::coarse_validate(bool with, std::vector<int64_t>& its) {
+  float* xi = cn.dZb;
+  float* b = cn.Tb;
+  launch_randn(st, m * kCoarse, smix64(cn.seed + 4), xi);
+  op_apply<float>(xi, b, kCoarse);
+  const bool saved = cn.enabled;
+  cn.enabled = with;
+  graphs_clear();
+  std::vector<double> tol(KMAX, 1e-6);
+  std::vector<int> conv(KMAX, 0);
+  its.assign(KMAX, 0);
+  double ms = 0.0;
+  try {
+    for (int rep = 0; rep < 2; ++rep) {
+      HIPCHK(hipStreamSynchronize(st));
+      const auto t0 = std::chrono::steady_clock::now();
+      cg<float>(KMAX, KMAX, b, cx32, tol, conv, its);
+      HIPCHK(hipStreamSynchronize(st));
+      ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  } catch (...) {
+    cn.enabled = saved;
+    graphs_clear();
+    throw;
+  }
+  cn.enabled = saved;
+  graphs_clear();
+  return ms;
+}
+
+// z = Q r + (I - Q G) M (I - G Q) r with Q = P (P'GP)^+ P' (the balancing
+// two-level preconditioner): c1 = Ainv P'r, r2 = r - GP c1, w = M r2,
+// z = w + P (c1 - Ainv GP'w).
+template <typename T>
+void wa_solver::coarse_apply(const T* R, T* Z, int kp, const SideWork* side) {
+  hipEvent_t a = nullptr;
+  T* R2 = reinterpret_cast<T*>(cn.r2);
+  tic(5, a);
+  int nb = launch_cproj_part<T>(st, m, cn.Pb, R, kp, cn.cpart);
+  launch_cproj_fin(st, cn.cpart, nb, kp, cn.Ainv, nullptr, 1.0, cn.c1);
+  launch_cexpand<T>(st, m, cn.GPb, cn.c1, -1.0, R, R2, kp);
+  toc(5, a);
+  pre_apply<T>(R2, Z, kp, side);
+  tic(5, a);
+  nb = launch_cproj_part<T>(st, m, cn.GPb, Z, kp, cn.cpart);
+  launch_cproj_fin(st, cn.cpart, nb, kp, cn.Ainv, cn.c1, -1.0, cn.cd);
+  launch_cexpand<T>(st, m, cn.Pb, cn.cd, 1.0, Z, Z, kp);
+  toc(5, a);
 }
 
 void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double* X, int64_t ldx, double* dem,
@@ -2708,6 +3092,178 @@ wa_status wa_solver_autotune(wa_solver* s, int32_t k, int32_t reps, int64_t* out
   }
 }
 
+void wa_coarse_params_default(wa_coarse_params* p) {
+  if (!p) return;
+  *p = wa_coarse_params{};
+  p->steps = 200;
+  p->sample_iters = 8;
+  p->seed = 0;
+  p->lr = 3e-3;
+  p->force = 0;
+  p->ridge = 1e-4;
+}
+
+wa_status wa_coarse_train(wa_solver* s, const wa_coarse_params* p, wa_coarse_report* out) {
+  try {
+    need_design(s);
+    if (!p || !out || p->steps < 0 || p->sample_iters < 1 || !(p->lr >= 0.0) || !(p->ridge >= 0.0))
+      throw Error(WA_ERR_INVALID_ARGUMENT, "bad coarse training parameters");
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    if (!s->coarse_supported())
+      throw Error(WA_ERR_UNSUPPORTED, "the learned coarse correction needs additive Schwarz with subdomains");
+    const auto t0 = std::chrono::steady_clock::now();
+    *out = wa_coarse_report{};
+    s->cn.enabled = false;
+    s->graphs_clear();
+    s->coarse_alloc(p->seed);
+    s->cn.ridge = p->ridge;
+    s->coarse_init_params(p->seed);
+    s->coarse_samples(p->sample_iters, p->seed);
+    s->gnn_forward();
+    s->gnn_loss(p->steps > 0);
+    out->loss_first = s->gnn_read_loss();
+    double b1t = 1.0, b2t = 1.0;
+    for (int step = 1; step <= p->steps; ++step) {
+      if (step > 1) {
+        s->gnn_forward();
+        s->gnn_loss(true);
+      }
+      b1t *= 0.9;
+      b2t *= 0.999;
+      launch_gnn_adam(s->st, s->cn.theta, s->cn.grad, s->cn.m1, s->cn.m2, p->lr, 1.0 - b1t, 1.0 - b2t);
+      launch_gnn_pack(s->st, s->cn.theta, s->cn.pack);
+    }
+    s->gnn_forward();
+    s->gnn_loss(false);
+    out->loss_last = s->gnn_read_loss();
+    out->steps = p->steps;
+    s->coarse_finalize();
+    std::vector<int64_t> ip, ic;
+    out->ms_plain = s->coarse_validate(false, ip);
+    out->ms_coarse = s->coarse_validate(true, ic);
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < KMAX; ++c) {
+      a += (double)ip[c];
+      b += (double)ic[c];
+    }
+    out->iters_plain = a / KMAX;
+    out->iters_coarse = b / KMAX;
+    const bool on = p->force > 0 ? true : p->force < 0 ? false : out->iters_coarse < out->iters_plain;
+    s->cn.enabled = on;
+    s->graphs_clear();
+    out->enabled = on ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipGetLastError());
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    if (s) s->cn.enabled = false;
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_coarse_enable(wa_solver* s, int32_t on) {
+  try {
+    need_design(s);
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    if (on && !s->cn.trained) throw Error(WA_ERR_INVALID_ARGUMENT, "the coarse correction is not trained");
+    s->cn.enabled = on != 0;
+    s->graphs_clear();
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_coarse_eval(wa_solver* s, const float* theta, double* loss, float* grad) {
+  try {
+    need_design(s);
+    if (!theta || !loss) throw Error(WA_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    if (!s->cn.ready || !(s->cn.nu > 0.0)) throw Error(WA_ERR_INVALID_ARGUMENT, "no coarse training set");
+    HIPCHK(hipMemcpyAsync(s->cn.theta, theta, sizeof(float) * kGnnParams, hipMemcpyHostToDevice, s->st));
+    launch_gnn_pack(s->st, s->cn.theta, s->cn.pack);
+    s->gnn_forward();
+    s->gnn_loss(grad != nullptr);
+    *loss = s->gnn_read_loss();
+    if (grad) {
+      HIPCHK(hipMemcpyAsync(grad, s->cn.grad, sizeof(float) * kGnnParams, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+    HIPCHK(hipGetLastError());
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_coarse_export(const wa_solver* cs, int32_t what, void* buf, int64_t cap, int64_t* len) {
+  try {
+    wa_solver* s = const_cast<wa_solver*>(cs);
+    need_design(s);
+    if (!len) throw Error(WA_ERR_INVALID_ARGUMENT, "null length");
+    std::lock_guard<std::mutex> lock(s->call_mu);
+    if (!s->cn.ready) throw Error(WA_ERR_INVALID_ARGUMENT, "no coarse training set");
+    const int64_t mm = s->m * kCoarse;
+    const void* src = nullptr;
+    int64_t n = 0, esz = 4;
+    bool bf = false;
+    switch (what) {
+      case 0: src = s->cn.X0; n = mm; break;
+      case 1: src = s->cn.E; n = mm; break;
+      case 2: src = s->cn.R; n = mm; break;
+      case 3: src = s->cn.theta; n = kGnnParams; break;
+      case 4: src = s->ext_of; n = s->m; break;
+      case 5: n = 1; esz = 8; break;
+      case 6: src = s->cn.Pb; n = mm; bf = true; break;
+      case 7: src = s->cn.GPb; n = mm; bf = true; break;
+      case 8: src = s->cn.Ainv; n = kMat; esz = 8; break;
+      case 9: src = s->cn.dinv; n = s->m; break;
+      case 10: src = s->cn.dP; n = mm; break;
+      case 11: src = s->cn.P; n = mm; break;
+      case 12: src = s->cn.GP; n = mm; break;
+      default: throw Error(WA_ERR_INVALID_ARGUMENT, "unknown export");
+    }
+    if ((what == 6 || what == 7 || what == 8) && !s->cn.trained)
+      throw Error(WA_ERR_INVALID_ARGUMENT, "the coarse correction is not trained");
+    *len = n;
+    if (!buf) return WA_OK;
+    if (cap < n) throw Error(WA_ERR_INVALID_ARGUMENT, "buffer too small");
+    if (what == 5) {
+      *static_cast<double*>(buf) = s->cn.nu;
+    } else if (bf) {
+      std::vector<uint16_t> h((size_t)n);
+      HIPCHK(hipMemcpyAsync(h.data(), src, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+      float* o = static_cast<float*>(buf);
+      for (int64_t i = 0; i < n; ++i) {
+        const uint32_t u = (uint32_t)h[(size_t)i] << 16;
+        std::memcpy(&o[i], &u, 4);
+      }
+    } else {
+      HIPCHK(hipMemcpyAsync(buf, src, (size_t)(esz * n), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+    return WA_OK;
+  } catch (const std::exception& e) {
+    return (wa_status)set_error(e);
+  }
+}
+
+wa_status wa_coarse_precond_apply(wa_solver* s, const double* R, double* Z, int32_t k, int64_t ld) {
+  if (s && (!s->cn.trained || !s->coarse_supported())) {
+    g_err = "the coarse correction is not trained";
+    return WA_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t mu = s ? s->mu : 0;
+  return level_a(s, false, R, mu, ld, Z, mu, ld, k,
+                 [&](const double* in, int64_t ldi, double* out, int64_t ldo, int kb, int kp) {
+                   launch_col_to_row(s->st, in, ldi, s->m, kb, s->cr64, kp, s->ext_of);
+                   s->coarse_apply<double>(s->cr64, s->cz64, kp, nullptr);
+                   launch_row_to_col(s->st, s->cz64, kp, s->mu, kb, out, ldo, s->int_of);
+                 });
+}
+
 wa_status wa_precond_work(const wa_solver* s, int64_t* work, int64_t* out4) {
   if (!s || !out4) { g_err = "null argument"; return WA_ERR_INVALID_ARGUMENT; }
   if (s->plan.precond == 0) { g_err = "solver has no preconditioner"; return WA_ERR_INVALID_ARGUMENT; }
@@ -2795,10 +3351,10 @@ void wa_timing_enable(wa_solver* s, int32_t on) {
 void wa_timing_reset(wa_solver* s) {
   if (!s) return;
   std::lock_guard<std::mutex> lock(s->call_mu);
-  for (int i = 0; i < 5; ++i) { s->timer.launches[i] = 0; s->timer.ms[i] = 0; }
+  for (int i = 0; i < 6; ++i) { s->timer.launches[i] = 0; s->timer.ms[i] = 0; }
 }
 wa_status wa_timing_get(wa_solver* s, int32_t kclass, int64_t* launches, double* ms) {
-  if (!s || kclass < 0 || kclass > 4) { g_err = "bad timing class"; return WA_ERR_INVALID_ARGUMENT; }
+  if (!s || kclass < 0 || kclass > 5) { g_err = "bad timing class"; return WA_ERR_INVALID_ARGUMENT; }
   std::lock_guard<std::mutex> lock(s->call_mu);
   if (launches) *launches = s->timer.launches[kclass];
   if (ms) *ms = s->timer.ms[kclass];
