@@ -261,8 +261,9 @@ wa_status wa_solver_autotune(wa_solver* s, int32_t k, int32_t reps, int64_t* out
  * (additive Schwarz only) runs in its flexible form with the balancing
  * two-level preconditioner z = Q r + (I - Q G) M (I - G Q) r,
  * Q = P (P'GP)^+ P'.  Plain Schwarz-PCG is the fallback: by default the
- * correction is enabled only when it cuts the iterations of a validation
- * block; the FP64 residual of the refinement loop judges every solve. */
+ * correction is enabled only when it cuts both the iterations and the solve
+ * time of a validation block; the FP64 residual of the refinement loop judges
+ * every solve. */
 typedef struct {
   int32_t steps;          /* Adam steps (default 200; 0 = prepare and evaluate only) */
   int32_t sample_iters;   /* plain CG iterations behind the error samples (default 8) */
@@ -278,6 +279,8 @@ typedef struct {
   double iters_plain, iters_coarse;   /* mean FP32 CG iterations of the validation block (tol 1e-6) */
   double ms_plain, ms_coarse;         /* its solve times */
   double seconds;                     /* wall time of wa_coarse_train */
+  int32_t kept;                       /* coarse vectors kept (G-orthonormal, at most 32) */
+  int32_t reserved;
 } wa_coarse_report;
 void wa_coarse_params_default(wa_coarse_params* p);
 /* Builds the training set, trains, validates and enables or not (see force). */

@@ -233,7 +233,8 @@ def test_fallback_rule_and_plain_solves_unchanged(prob):
     plain = W.Solver(codes, weights=w, config=cfg, preconditioner=precond()).solve_batch(Y)
     rep = s.train_coarse(steps=40, seed=SEED + 1)
     print("fallback report:", rep)
-    assert rep["enabled"] == int(rep["iters_coarse"] < rep["iters_plain"])
+    assert rep["enabled"] == int(rep["iters_coarse"] < rep["iters_plain"] and rep["ms_coarse"] < rep["ms_plain"])
+    assert 1 <= rep["kept"] <= 32
     s.enable_coarse(False)
     r = s.solve_batch(Y)
     np.testing.assert_array_equal(r.demeaned, plain.demeaned)

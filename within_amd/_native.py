@@ -80,6 +80,8 @@ class CoarseReport(C.Structure):
         ("ms_plain", C.c_double),
         ("ms_coarse", C.c_double),
         ("seconds", C.c_double),
+        ("kept", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -168,22 +168,38 @@ __global__ void __launch_bounds__(kGnnBlock) k_gnn_xty(int64_t m, const float* _
     part[(int64_t)blockIdx.x * kXtyOut + i] = ((sh[0][i] + sh[1][i]) + sh[2][i]) + sh[3][i];
 }
 
-__global__ void __launch_bounds__(1024) k_gnn_xty_fin(const float* __restrict__ part, int nb, float* __restrict__ out,
+// Column sums of an nb x n partial matrix in two fixed-order stages: block j
+// of the first sums rows [j c, (j + 1) c) (c = ceil(nb / kCprojMid)) of
+// every column into mid; the second sums the kCprojMid rows of mid.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_part_mid(const T* __restrict__ part, int nb, int n, T* __restrict__ mid) {
+  const int chunk = (nb + kCprojMid - 1) / kCprojMid;
+  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    T s = 0;
+    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * n + i];
+    mid[(int64_t)blockIdx.x * n + i] = s;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_gnn_xty_fin(const float* __restrict__ mid, float* __restrict__ out,
                                                       float* __restrict__ out_cs) {
   for (int i = threadIdx.x; i < kXtyOut; i += 1024) {
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * kXtyOut + i];
+#pragma unroll 8
+    for (int b = 0; b < kCprojMid; ++b) s += mid[(int64_t)b * kXtyOut + i];
     if (i < kMat) out[i] = s;
     else if (out_cs) out_cs[i - kMat] = s;
   }
 }
 
 void launch_gnn_xty(hipStream_t st, int64_t m, const float* X, const float* sx, const float* Y, float* part,
-                    float* out, float* out_cs) {
+                    double* mid, float* out, float* out_cs) {
   const int64_t groups = (m + 7) / 8;
   const unsigned nb = clamp_blocks((groups + 63) / 64, kXtyBlocksMax);
   k_gnn_xty<<<nb, kGnnBlock, 0, st>>>(m, X, sx, Y, part);
-  k_gnn_xty_fin<<<1, 1024, 0, st>>>(part, (int)nb, out, out_cs);
+  k_part_mid<float><<<kCprojMid, 1024, 0, st>>>(part, (int)nb, kXtyOut, reinterpret_cast<float*>(mid));
+  k_gnn_xty_fin<<<1, 1024, 0, st>>>(reinterpret_cast<const float*>(mid), out, out_cs);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,18 +233,21 @@ __global__ void __launch_bounds__(kGnnBlock) k_xty64(int64_t m, const float* __r
   for (int c = 0; c < 4; ++c) part[(int64_t)blockIdx.x * kMat + a * kCoarse + b0 + c] = acc[c];
 }
 
-__global__ void __launch_bounds__(1024) k_xty64_fin(const double* __restrict__ part, int nb, double* __restrict__ out) {
+__global__ void __launch_bounds__(1024) k_xty64_fin(const double* __restrict__ mid, double* __restrict__ out) {
   const int i = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * kMat + i];
+#pragma unroll 8
+  for (int b = 0; b < kCprojMid; ++b) s += mid[(int64_t)b * kMat + i];
   out[i] = s;
 }
 
-void launch_xty64(hipStream_t st, int64_t m, const float* X, const float* Y, double* part, double* out) {
+void launch_xty64(hipStream_t st, int64_t m, const float* X, const float* Y, double* part, double* mid,
+                  double* out) {
   const int64_t tiles = (m + 63) / 64;
   const unsigned nb = clamp_blocks((tiles + 3) / 4, kXtyBlocksMax);
   k_xty64<<<nb, kGnnBlock, 0, st>>>(m, X, Y, part);
-  k_xty64_fin<<<1, kMat, 0, st>>>(part, (int)nb, out);
+  k_part_mid<double><<<kCprojMid, 1024, 0, st>>>(part, (int)nb, kMat, mid);
+  k_xty64_fin<<<1, kMat, 0, st>>>(mid, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -293,25 +312,32 @@ __global__ void __launch_bounds__(64) k_coarse_small(const double* __restrict__ 
   const int t = threadIdx.x;
   for (int i = t; i < kMat; i += 64) L[i] = Ac[i];
   __syncthreads();
+  __shared__ double reg_s;
   if (t == 0) {
     double tr = 0.0;
     for (int i = 0; i < K; ++i) tr += L[i * (K + 1)];
-    const double reg = eps * (tr > 0.0 ? tr : 0.0) / K + 1e-300;
-    for (int i = 0; i < K; ++i) L[i * (K + 1)] += reg;
-    // lower Cholesky factor in place, column by column
-    for (int j = 0; j < K; ++j) {
-      double d = L[j * (K + 1)];
-      for (int k = 0; k < j; ++k) d -= L[j * K + k] * L[j * K + k];
-      d = d > reg ? sqrt(d) : sqrt(reg);
-      L[j * (K + 1)] = d;
-      for (int i = j + 1; i < K; ++i) {
-        double s = L[i * K + j];
-        for (int k = 0; k < j; ++k) s -= L[i * K + k] * L[j * K + k];
-        L[i * K + j] = s / d;
-      }
-    }
+    reg_s = eps * (tr > 0.0 ? tr : 0.0) / K + 1e-300;
   }
   __syncthreads();
+  const double reg = reg_s;
+  if (t < K) L[t * (K + 1)] += reg;
+  __syncthreads();
+  // lower Cholesky factor in place, column by column: the pivot, then the
+  // entries below it in parallel (each sum in ascending k)
+  for (int j = 0; j < K; ++j) {
+    if (t == 0) {
+      double d = L[j * (K + 1)];
+      for (int k = 0; k < j; ++k) d -= L[j * K + k] * L[j * K + k];
+      L[j * (K + 1)] = d > reg ? sqrt(d) : sqrt(reg);
+    }
+    __syncthreads();
+    if (t > j && t < K) {
+      double s = L[t * K + j];
+      for (int k = 0; k < j; ++k) s -= L[t * K + k] * L[j * K + k];
+      L[t * K + j] = s / L[j * (K + 1)];
+    }
+    __syncthreads();
+  }
   if (t < K) {  // column t of U: L L^T v = u (forward, then backward substitution)
     for (int i = 0; i < K; ++i) {
       double s = U[i * K + t];
@@ -446,31 +472,45 @@ void launch_sub(hipStream_t st, int64_t count, const float* a, const float* b, f
 // the coarse correction in the Krylov loop
 // ---------------------------------------------------------------------------
 // X^T R per block: E = 32 G threads own (coarse index a, NV columns from c0),
-// the other 256 / E thread rows split the tile's rows; FP64 sums
+// the other 256 / E thread rows split the tile's rows; FP64 sums.  Tiles of
+// TR rows are staged through LDS with 16-byte loads (X: 8 BF16 values each).
 template <typename T, int NV>
 __global__ void __launch_bounds__(kGnnBlock) k_cproj_part(int64_t m, const bf16_t* __restrict__ X,
                                                           const T* __restrict__ R, int kp,
                                                           double* __restrict__ part) {
-  constexpr int TR = 64;
+  constexpr int TR = 512 / sizeof(T), VN = 16 / sizeof(T);
   __shared__ float xs[TR][kCoarse + 1];
-  __shared__ T rs[TR * 32];
+  __shared__ __align__(16) T rs[TR * 32];
   __shared__ double red[kGnnBlock * NV];
   const int G = kp / NV, E = kCoarse * G, RL = kGnnBlock / E;
   const int e = threadIdx.x % E, rl = threadIdx.x / E, a = e / G, c0 = (e % G) * NV;
   double acc[NV];
 #pragma unroll
   for (int c = 0; c < NV; ++c) acc[c] = 0.0;
-  const int64_t tiles = (m + TR - 1) / TR;
+  const int64_t tiles = (m + TR - 1) / TR, tot = m * kp;
   for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
     const int64_t r0 = t * TR;
-    for (int i = threadIdx.x; i < TR * kCoarse; i += kGnnBlock) {
-      const int r = i / kCoarse, c = i % kCoarse;
+    for (int i = threadIdx.x; i < TR * 4; i += kGnnBlock) {
+      const int r = i >> 2, q = i & 3;
       const int64_t row = r0 + r;
-      xs[r][c] = row < m ? bf2f(X[row * kCoarse + c]) : 0.0f;
+      VecT<uint32_t, 4> w;
+      if (row < m) {
+        w = vload<uint32_t, 4>(reinterpret_cast<const uint32_t*>(X + row * kCoarse) + 4 * q);
+      } else {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) w.v[h] = 0u;
+      }
+#pragma unroll
+      for (int h = 0; h < 8; ++h) xs[r][8 * q + h] = bf2f(h & 1 ? (w.v[h >> 1] >> 16) : w.v[h >> 1]);
     }
-    for (int i = threadIdx.x; i < TR * kp; i += kGnnBlock) {
-      const int64_t row = r0 + i / kp;
-      rs[i] = row < m ? R[r0 * kp + i] : (T)0;
+    const int64_t base = r0 * kp;
+    for (int i = threadIdx.x * VN; i < TR * kp; i += kGnnBlock * VN) {
+      if (base + i + VN <= tot) {
+        *reinterpret_cast<VecT<T, VN>*>(&rs[i]) = vload<T, VN>(R + base + i);
+      } else {
+#pragma unroll
+        for (int j = 0; j < VN; ++j) rs[i + j] = base + i + j < tot ? R[base + i + j] : (T)0;
+      }
     }
     __syncthreads();
     for (int r = rl; r < TR; r += RL) {
@@ -495,8 +535,8 @@ __global__ void __launch_bounds__(kGnnBlock) k_cproj_part(int64_t m, const bf16_
 
 template <typename T>
 int launch_cproj_part(hipStream_t st, int64_t m, const bf16_t* X, const T* R, int kp, double* part) {
-  const int64_t tiles = (m + 63) / 64;
-  const unsigned nb = clamp_blocks((tiles + 3) / 4, kCprojBlocksMax);
+  const int64_t tiles = (m + 512 / sizeof(T) - 1) / (512 / sizeof(T));
+  const unsigned nb = clamp_blocks((tiles + 1) / 2, kCprojBlocksMax);
   const int nv = kp >= 8 ? kp / 8 : 1;
   if (nv == 4) k_cproj_part<T, 4><<<nb, kGnnBlock, 0, st>>>(m, X, R, kp, part);
   else if (nv == 2) k_cproj_part<T, 2><<<nb, kGnnBlock, 0, st>>>(m, X, R, kp, part);
@@ -506,14 +546,29 @@ int launch_cproj_part(hipStream_t st, int64_t m, const bf16_t* X, const T* R, in
 template int launch_cproj_part<float>(hipStream_t, int64_t, const bf16_t*, const float*, int, double*);
 template int launch_cproj_part<double>(hipStream_t, int64_t, const bf16_t*, const double*, int, double*);
 
-__global__ void __launch_bounds__(1024) k_cproj_fin(const double* __restrict__ part, int nb, int kp,
+// second stage: block j sums the partials [j c, (j + 1) c) of every entry
+// (c = ceil(nb / kCprojMid)), in order
+__global__ void __launch_bounds__(1024) k_cproj_mid(const double* __restrict__ part, int nb, int n,
+                                                    double* __restrict__ mid) {
+  const int chunk = (nb + kCprojMid - 1) / kCprojMid;
+  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    double s = 0.0;
+    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * n + i];
+    mid[(int64_t)blockIdx.x * n + i] = s;
+  }
+}
+
+// u = sum of the kCprojMid second-stage partials (in order); out = base + sign Ainv u
+__global__ void __launch_bounds__(1024) k_cproj_fin(const double* __restrict__ mid, int kp,
                                                     const double* __restrict__ Ainv, const double* base, double sign,
                                                     double* out) {
   __shared__ double u[kCoarse * 32];
   const int n = kCoarse * kp;
   for (int i = threadIdx.x; i < n; i += 1024) {
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * n + i];
+#pragma unroll 8
+    for (int b = 0; b < kCprojMid; ++b) s += mid[(int64_t)b * n + i];
     u[i] = s;
   }
   __syncthreads();
@@ -525,9 +580,10 @@ __global__ void __launch_bounds__(1024) k_cproj_fin(const double* __restrict__ p
   }
 }
 
-void launch_cproj_fin(hipStream_t st, const double* part, int nb, int kp, const double* Ainv, const double* base,
-                      double sign, double* out) {
-  k_cproj_fin<<<1, 1024, 0, st>>>(part, nb, kp, Ainv, base, sign, out);
+void launch_cproj_fin(hipStream_t st, const double* part, int nb, int kp, double* mid, const double* Ainv,
+                      const double* base, double sign, double* out) {
+  k_cproj_mid<<<kCprojMid, 1024, 0, st>>>(part, nb, kCoarse * kp, mid);
+  k_cproj_fin<<<1, 1024, 0, st>>>(mid, kp, Ainv, base, sign, out);
 }
 
 template <typename T, int NV>

@@ -34,13 +34,16 @@ void launch_gnn_layer(hipStream_t st, int64_t m, const float* A0, const bf16_t* 
                       float* out, float* out2, const float* s2);
 // out[a * 32 + b] = sum_i (sx o X)[i, a] Y[i, b] (BF16 operands, FP32
 // accumulation on the matrix cores); out_cs[b] = sum_i Y[i, b] (FP32) when
-// out_cs is not null.  part: kXtyBlocksMax * (kMat + kCoarse) floats.
+// out_cs is not null.  part: kXtyBlocksMax * (kMat + kCoarse) floats; mid:
+// kCprojMid * kMat doubles (second-stage partials).
 constexpr int kXtyBlocksMax = 512;
+constexpr int kCprojMid = 32;  // second-stage partials of every two-stage reduction
 void launch_gnn_xty(hipStream_t st, int64_t m, const float* X, const float* sx, const float* Y, float* part,
-                    float* out, float* out_cs);
+                    double* mid, float* out, float* out_cs);
 // out[a * 32 + b] = sum_i X[i, a] Y[i, b] in FP64 (FP32 inputs).
-// part: kXtyBlocksMax * kMat doubles.
-void launch_xty64(hipStream_t st, int64_t m, const float* X, const float* Y, double* part, double* out);
+// part: kXtyBlocksMax * kMat doubles; mid as above.
+void launch_xty64(hipStream_t st, int64_t m, const float* X, const float* Y, double* part, double* mid,
+                  double* out);
 // out = A0 M0 + A1 M1, FP64 accumulation (M0, M1: 32 x 32 FP64)
 void launch_tall_small(hipStream_t st, int64_t m, const float* A0, const double* M0, const float* A1,
                        const double* M1, float* out);
@@ -66,11 +69,13 @@ void launch_sub(hipStream_t st, int64_t count, const float* a, const float* b, f
 // ---- the coarse correction inside the Krylov loop (T = float or double) ----
 // part[blk] = X^T R over the block's rows (X: m x 32 BF16, R: m x kp), FP64;
 // returns the block count
-constexpr int kCprojBlocksMax = 256;
+constexpr int kCprojBlocksMax = 1024;
 template <typename T>
 int launch_cproj_part(hipStream_t st, int64_t m, const bf16_t* X, const T* R, int kp, double* part);
-// out = base + sign * Ainv u with u the sum of the nb partials (32 x kp), base may be null
-void launch_cproj_fin(hipStream_t st, const double* part, int nb, int kp, const double* Ainv,
+// out = base + sign * Ainv u with u the sum of the nb partials (32 x kp),
+// reduced in two fixed-order stages through mid (kCprojMid * 32 * kp
+// doubles); base may be null
+void launch_cproj_fin(hipStream_t st, const double* part, int nb, int kp, double* mid, const double* Ainv,
                       const double* base, double sign, double* out);
 // out = base + sign * X coef (X: m x 32 BF16, coef: 32 x kp FP64); out may alias base
 template <typename T>

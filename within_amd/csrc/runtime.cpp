@@ -213,8 +213,10 @@ struct wa_solver {
     bf16_t* pack = nullptr;
     double *xpart64 = nullptr, *Ac = nullptr, *U = nullptr, *V = nullptr, *loss = nullptr;
     double nu = 0.0;
+    int kept = 0;  // coarse vectors kept after G-orthonormalisation
     bf16_t *Pb = nullptr, *GPb = nullptr;
-    double *Ainv = nullptr, *cpart = nullptr, *c1 = nullptr, *cd = nullptr, *r2 = nullptr, *dpart = nullptr;
+    double *Ainv = nullptr, *cpart = nullptr, *cmid = nullptr, *c1 = nullptr, *cd = nullptr, *r2 = nullptr;
+    double* dpart = nullptr;
     std::vector<float> h_feat;
   } cn;
   bool fcg = false;  // set by cg() while the coarse correction is part of its preconditioner
@@ -1600,6 +1602,7 @@ void wa_solver::coarse_alloc(uint64_t seed) {
     cn.Pb = dalloc<bf16_t>(mm);
     cn.GPb = dalloc<bf16_t>(mm);
     cn.cpart = dalloc<double>((int64_t)kCprojBlocksMax * kMat);
+    cn.cmid = dalloc<double>((int64_t)kCprojMid * kMat);
     cn.r2 = dalloc<double>(mm);
     cn.dpart = dalloc<double>(2 * (int64_t)kRedBlocks * KMAX);
     cn.ready = true;
@@ -1707,8 +1710,8 @@ void wa_solver::gnn_forward() {
 // loss (device scalar cn.loss) and, with grad, its gradient in cn.grad
 void wa_solver::gnn_loss(bool grad) {
   op_apply<float>(cn.P, cn.GP, kCoarse);
-  launch_xty64(st, m, cn.P, cn.GP, cn.xpart64, cn.Ac);
-  launch_xty64(st, m, cn.P, cn.R, cn.xpart64, cn.U);
+  launch_xty64(st, m, cn.P, cn.GP, cn.xpart64, cn.cmid, cn.Ac);
+  launch_xty64(st, m, cn.P, cn.R, cn.xpart64, cn.cmid, cn.U);
   launch_coarse_small(st, cn.Ac, cn.U, cn.nu, cn.ridge, cn.V, cn.loss, cn.W12, cn.W12 + kMat);
   if (!grad) return;
   launch_tall_small(st, m, cn.R, cn.W12, cn.GP, cn.W12 + kMat, cn.dP);  // dL/dP
@@ -1717,8 +1720,8 @@ void wa_solver::gnn_loss(bool grad) {
   for (int l = kGnnLayers - 1; l >= 0; --l) {
     float* g = cn.grad + (size_t)l * kLayerParams;
     const bf16_t* pk = cn.pack + (size_t)l * kLayerPack;
-    launch_gnn_xty(st, m, cn.H[l], nullptr, dz, cn.xpart, g, g + 2 * kMat);  // dWs, db
-    launch_gnn_xty(st, m, cn.T[l], cn.cvec, dz, cn.xpart, g + kMat, nullptr);  // dWa (input A H_l)
+    launch_gnn_xty(st, m, cn.H[l], nullptr, dz, cn.xpart, cn.cmid, g, g + 2 * kMat);  // dWs, db
+    launch_gnn_xty(st, m, cn.T[l], cn.cvec, dz, cn.xpart, cn.cmid, g + kMat, nullptr);  // dWa (input A H_l)
     if (l == 0) break;
     // dH_l = dZ Ws' + A (dZ Wa') (A is symmetric); dZ_{l-1} = dH_l o [H_l > 0]
     launch_gnn_layer(st, m, dz, pk + 3 * kMat, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, cn.Ys,
@@ -1736,10 +1739,10 @@ double wa_solver::gnn_read_loss() {
   return hbuf[0];
 }
 
-// Pseudo-inverse of a symmetric n x n matrix by cyclic Jacobi rotations;
-// eigenvalues below 1e-10 of the largest are dropped.
-static std::vector<double> sym_pinv(std::vector<double> A, int n) {
-  std::vector<double> V((size_t)n * n, 0.0);
+// Eigen-decomposition of a symmetric n x n matrix by cyclic Jacobi rotations:
+// A = V diag(w) V' (columns of V).
+static void sym_eig(std::vector<double> A, int n, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
   double norm2 = 0.0;
   for (double v : A) norm2 += v * v;
@@ -1772,38 +1775,72 @@ static std::vector<double> sym_pinv(std::vector<double> A, int n) {
         }
       }
   }
+  w.assign((size_t)n, 0.0);
+  for (int i = 0; i < n; ++i) w[(size_t)i] = A[(size_t)i * n + i];
+}
+
+// Pseudo-inverse of a symmetric matrix; eigenvalues below 1e-10 of the
+// largest are dropped.
+static std::vector<double> sym_pinv(const std::vector<double>& A, int n) {
+  std::vector<double> w, V;
+  sym_eig(A, n, w, V);
   double wmax = 0.0;
-  for (int i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(A[(size_t)i * n + i]));
+  for (double x : w) wmax = std::max(wmax, std::fabs(x));
   std::vector<double> out((size_t)n * n, 0.0);
   for (int e = 0; e < n; ++e) {
-    const double w = A[(size_t)e * n + e];
-    if (!(w > 1e-10 * wmax)) continue;
+    if (!(w[(size_t)e] > 1e-10 * wmax)) continue;
     for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) out[(size_t)i * n + j] += V[(size_t)i * n + e] * V[(size_t)j * n + e] / w;
+      for (int j = 0; j < n; ++j)
+        out[(size_t)i * n + j] += V[(size_t)i * n + e] * V[(size_t)j * n + e] / w[(size_t)e];
   }
   return out;
 }
 
-// The coarse space of the apply: P and GP = G P in BF16, and the
-// pseudo-inverse of the (symmetrised) coarse matrix P'(GP) formed from the
-// rounded values, so the apply's projections are consistent with each other.
-void wa_solver::coarse_finalize() {
-  gnn_forward();
-  float* Pf = cn.dP;
-  float* GPf = cn.dZa;
-  launch_to_bf16(st, m * kCoarse, cn.P, cn.Pb, Pf);
-  op_apply<float>(Pf, cn.GP, kCoarse);
-  launch_to_bf16(st, m * kCoarse, cn.GP, cn.GPb, GPf);
-  launch_xty64(st, m, Pf, GPf, cn.xpart64, cn.Ac);
+// P'(GP) in FP64, symmetrised, on the host
+static std::vector<double> coarse_matrix(wa_solver* s, const float* P, const float* GP) {
+  launch_xty64(s->st, s->m, P, GP, s->cn.xpart64, s->cn.cmid, s->cn.Ac);
   std::vector<double> A((size_t)kMat);
-  HIPCHK(hipMemcpyAsync(A.data(), cn.Ac, sizeof(double) * kMat, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(A.data(), s->cn.Ac, sizeof(double) * kMat, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
   for (int i = 0; i < kCoarse; ++i)
     for (int j = i + 1; j < kCoarse; ++j) {
       const double v = 0.5 * (A[(size_t)i * kCoarse + j] + A[(size_t)j * kCoarse + i]);
       A[(size_t)i * kCoarse + j] = A[(size_t)j * kCoarse + i] = v;
     }
-  const std::vector<double> inv = sym_pinv(A, kCoarse);
+  return A;
+}
+
+// The coarse space of the apply.  The network's columns are first made
+// G-orthonormal (P <- P V L^-1/2 from P'GP = V L V', dropping directions
+// below 1e-6 of the largest eigenvalue: they carry almost no G-energy), so
+// the apply's coarse matrix is well conditioned and the rounding of its BF16
+// copies is not amplified.  Then P and GP = G P are stored in BF16, and
+// (P'GP)^+ is formed from the rounded values (symmetrised), so the apply's
+// projections are consistent with each other.
+void wa_solver::coarse_finalize() {
+  gnn_forward();
+  op_apply<float>(cn.P, cn.GP, kCoarse);
+  std::vector<double> w, V;
+  sym_eig(coarse_matrix(this, cn.P, cn.GP), kCoarse, w, V);
+  double wmax = 0.0;
+  for (double x : w) wmax = std::max(wmax, x);
+  std::vector<double> T((size_t)kMat, 0.0);
+  int col = 0;
+  for (int e = 0; e < kCoarse; ++e) {
+    if (!(w[(size_t)e] > 1e-6 * wmax)) continue;
+    const double f = 1.0 / std::sqrt(w[(size_t)e]);
+    for (int i = 0; i < kCoarse; ++i) T[(size_t)i * kCoarse + col] = V[(size_t)i * kCoarse + e] * f;
+    ++col;
+  }
+  cn.kept = col;
+  HIPCHK(hipMemcpyAsync(cn.W12, T.data(), sizeof(double) * kMat, hipMemcpyHostToDevice, st));
+  float* Pf = cn.dP;
+  float* GPf = cn.dZa;
+  launch_tall_small(st, m, cn.P, cn.W12, nullptr, nullptr, cn.dZb);
+  launch_to_bf16(st, m * kCoarse, cn.dZb, cn.Pb, Pf);
+  op_apply<float>(Pf, cn.GP, kCoarse);
+  launch_to_bf16(st, m * kCoarse, cn.GP, cn.GPb, GPf);
+  const std::vector<double> inv = sym_pinv(coarse_matrix(this, Pf, GPf), kCoarse);
   HIPCHK(hipMemcpyAsync(cn.Ainv, inv.data(), sizeof(double) * kMat, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   cn.trained = true;
@@ -1852,13 +1889,13 @@ void wa_solver::coarse_apply(const T* R, T* Z, int kp, const SideWork* side) {
   T* R2 = reinterpret_cast<T*>(cn.r2);
   tic(5, a);
   int nb = launch_cproj_part<T>(st, m, cn.Pb, R, kp, cn.cpart);
-  launch_cproj_fin(st, cn.cpart, nb, kp, cn.Ainv, nullptr, 1.0, cn.c1);
+  launch_cproj_fin(st, cn.cpart, nb, kp, cn.cmid, cn.Ainv, nullptr, 1.0, cn.c1);
   launch_cexpand<T>(st, m, cn.GPb, cn.c1, -1.0, R, R2, kp);
   toc(5, a);
   pre_apply<T>(R2, Z, kp, side);
   tic(5, a);
   nb = launch_cproj_part<T>(st, m, cn.GPb, Z, kp, cn.cpart);
-  launch_cproj_fin(st, cn.cpart, nb, kp, cn.Ainv, cn.c1, -1.0, cn.cd);
+  launch_cproj_fin(st, cn.cpart, nb, kp, cn.cmid, cn.Ainv, cn.c1, -1.0, cn.cd);
   launch_cexpand<T>(st, m, cn.Pb, cn.cd, 1.0, Z, Z, kp);
   toc(5, a);
 }
@@ -3148,13 +3185,18 @@ wa_status wa_coarse_train(wa_solver* s, const wa_coarse_params* p, wa_coarse_rep
     }
     out->iters_plain = a / KMAX;
     out->iters_coarse = b / KMAX;
-    const bool on = p->force > 0 ? true : p->force < 0 ? false : out->iters_coarse < out->iters_plain;
+    // plain Schwarz-PCG stays unless the correction cuts the validation
+    // block's iterations and its solve time
+    const bool on = p->force > 0   ? true
+                    : p->force < 0 ? false
+                                   : out->iters_coarse < out->iters_plain && out->ms_coarse < out->ms_plain;
     s->cn.enabled = on;
     s->graphs_clear();
     out->enabled = on ? 1 : 0;
     HIPCHK(hipStreamSynchronize(s->st));
     HIPCHK(hipGetLastError());
     out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    out->kept = s->cn.kept;
     return WA_OK;
   } catch (const std::exception& e) {
     if (s) s->cn.enabled = false;
