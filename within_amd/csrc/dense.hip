@@ -214,6 +214,16 @@ __global__ void __launch_bounds__(RB) k_inv_sym(const double* __restrict__ U, do
     }
 }
 
+// L[i][j] = A[i][j] for j <= i < m (the anchored minor's lower triangle of the
+// n x n matrix A), zero above the diagonal
+__global__ void k_minor_lower(const double* __restrict__ A, int64_t n, double* __restrict__ L, int64_t m) {
+  const int64_t tot = m * m;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / m, j = t % m;
+    L[t] = j <= i ? A[i * n + j] : 0.0;
+  }
+}
+
 struct DevBuf {
   void* p = nullptr;
   explicit DevBuf(size_t bytes) { DCHK(hipMalloc(&p, bytes)); }
@@ -226,7 +236,17 @@ struct DevBuf {
 
 }  // namespace
 
-bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<double>& inv) {
+DeviceDense::~DeviceDense() {
+  if (p) (void)hipFree(p);
+}
+
+std::vector<double> DeviceDense::download() const {
+  std::vector<double> h((size_t)n * n);
+  if (!h.empty()) DCHK(hipMemcpy(h.data(), p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  return h;
+}
+
+bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::shared_ptr<DeviceDense>& out) {
   const bool report = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
   auto t0 = std::chrono::steady_clock::now();
   auto lap = [&](const char* what) {
@@ -237,23 +257,34 @@ bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<do
                  std::chrono::duration<double>(t1 - t0).count());
     t0 = t1;
   };
-  inv.assign((size_t)n * n, 0.0);
-  const i64 m = n - 1;
-  if (m <= 0) return true;
   int dev = 0;
   DCHK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   DCHK(hipGetDeviceProperties(&prop, dev));
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     throw HipFail(std::string("within_amd requires a gfx950 (MI355X) device, found ") + prop.gcnArchName);
-  std::vector<double> l((size_t)m * m, 0.0);
-  for (i64 i = 0; i < m; ++i)
-    for (i64 j = 0; j <= i; ++j) l[i * m + j] = a[i * n + j];
-  const size_t mm = (size_t)m * m * sizeof(double), nn = (size_t)n * n * sizeof(double);
-  DevBuf dL(mm), dU(mm), dI(nn), dF(sizeof(int));
-  DCHK(hipMemcpy(dL.p, l.data(), mm, hipMemcpyHostToDevice));
+  const size_t nn = (size_t)n * n * sizeof(double);
+  auto inv = std::make_shared<DeviceDense>();
+  inv->n = n;
+  DCHK(hipMalloc(&inv->p, std::max<size_t>(nn, sizeof(double))));
+  DCHK(hipMemset(inv->p, 0, std::max<size_t>(nn, sizeof(double))));
+  const i64 m = n - 1;
+  if (m <= 0) {
+    out = inv;
+    return true;
+  }
+  const size_t mm = (size_t)m * m * sizeof(double);
+  DevBuf dL(mm), dU(mm), dF(sizeof(int));
+  {
+    // the matrix goes up once; its anchored minor's lower triangle is taken on the device
+    DevBuf dA(nn);
+    DCHK(hipMemcpy(dA.p, a.data(), nn, hipMemcpyHostToDevice));
+    k_minor_lower<<<(unsigned)((m * m + 255) / 256 > 65536 ? 65536 : (m * m + 255) / 256), 256>>>(
+        dA.as<double>(), n, dL.as<double>(), m);
+    DCHK(hipGetLastError());
+    DCHK(hipDeviceSynchronize());
+  }
   DCHK(hipMemset(dU.p, 0, mm));
-  DCHK(hipMemset(dI.p, 0, nn));
   DCHK(hipMemset(dF.p, 0, sizeof(int)));
   lap("upload");
   const int mi = (int)m, ni = (int)n;
@@ -272,12 +303,12 @@ bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<do
     k_inv_lower<<<(unsigned)(r0 / TB + 1), TB * TB>>>(dL.as<double>(), dU.as<double>(), mi, r0);
   DCHK(hipGetLastError());
   lap("triangular inverse");
-  k_inv_sym<<<dim3((unsigned)((mi + RB - 1) / RB), (unsigned)((mi + TB - 1) / TB)), RB>>>(dU.as<double>(),
-                                                                                        dI.as<double>(), mi, ni);
+  k_inv_sym<<<dim3((unsigned)((mi + RB - 1) / RB), (unsigned)((mi + TB - 1) / TB)), RB>>>(
+      dU.as<double>(), static_cast<double*>(inv->p), mi, ni);
   DCHK(hipGetLastError());
+  DCHK(hipDeviceSynchronize());
   lap("product");
-  DCHK(hipMemcpy(inv.data(), dI.p, nn, hipMemcpyDeviceToHost));
-  lap("download");
+  out = inv;
   return true;
 }
 

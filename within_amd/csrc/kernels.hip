@@ -2013,4 +2013,21 @@ template void launch_precond_mult<double>(hipStream_t, const DevPrecond<double>&
                                           const DevCsr<double>&, const double*, double*, PrecondWork<double>&,
                                           double*, double*, double*, const double*, int, int64_t);
 
+template <typename T>
+__global__ void k_pad_core(const double* __restrict__ src, uint32_t n, T* __restrict__ dst, uint32_t ld) {
+  const int64_t tot = (int64_t)n * ld;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / ld, j = t % ld;
+    dst[t] = j < n ? (T)src[i * n + j] : (T)0;
+  }
+}
+template <typename T>
+void launch_pad_core(hipStream_t st, const double* src, uint32_t n, T* dst, uint32_t ld) {
+  const int64_t tot = (int64_t)n * ld;
+  if (tot <= 0) return;
+  k_pad_core<T><<<(unsigned)std::min<int64_t>((tot + 255) / 256, 65536), 256, 0, st>>>(src, n, dst, ld);
+}
+template void launch_pad_core<float>(hipStream_t, const double*, uint32_t, float*, uint32_t);
+template void launch_pad_core<double>(hipStream_t, const double*, uint32_t, double*, uint32_t);
+
 }  // namespace wa

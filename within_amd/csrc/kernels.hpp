@@ -230,5 +230,8 @@ template <typename T>
 void launch_scale_col(hipStream_t st, T* Y, const T* X, const double* coef, int64_t m, int kp);
 template <typename T>
 void launch_rsub(hipStream_t st, T* R, const T* B, int64_t n);
+// dst[i * ld + j] = (T) src[i * n + j] for j < n, 0 for n <= j < ld (i < n)
+template <typename T>
+void launch_pad_core(hipStream_t st, const double* src, uint32_t n, T* dst, uint32_t ld);
 
 }  // namespace wa

@@ -613,17 +613,19 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       }
       const double t_rounds = secs();
       std::vector<double> inv;
+      std::shared_ptr<DeviceDense> dinv;
       if (device && c >= kDeviceCore) {
-        if (!device_anchored_inverse(a, c, inv)) inv.clear();
+        if (!device_anchored_inverse(a, c, dinv)) dinv.reset();
       } else {
         inv = anchored_inverse(a, c);
       }
       if (report)
         std::fprintf(stderr, "[within_amd]     approximate Cholesky: %llu rounds %.3f s, dense core %lld: %.3f s\n",
                      (unsigned long long)round, t_rounds, (long long)c, secs() - t_rounds);
-      if (!inv.empty()) {
+      if (!inv.empty() || dinv) {
         f.core = std::move(core);
         f.core_inv = std::move(inv);
+        f.core_dev = std::move(dinv);
         break;
       }
       cut = 0;
@@ -1136,6 +1138,7 @@ static void flatten(Plan& p) {
     }
     f.core_ptr.push_back((u32)f.core_idx.size());
     f.core_mat.insert(f.core_mat.end(), F.core_inv.begin(), F.core_inv.end());
+    f.core_dev.push_back(F.core_dev);
     f.max_core = std::max<i64>(f.max_core, (i64)F.core.size());
   }
   // DOF -> local entries
@@ -1505,6 +1508,30 @@ void flat_fields(IO& io, PrecondFlat& f) {
 }
 }  // namespace
 
+void materialize_cores(PrecondFlat& f) {
+  bool any = false;
+  for (const auto& d : f.core_dev) any = any || d != nullptr;
+  if (!any) return;
+  const size_t n_cores = f.core_mat_off.size();
+  std::vector<double> mat;
+  std::vector<uint64_t> off(n_cores);
+  for (size_t c = 0; c < n_cores; ++c) {
+    off[c] = mat.size();
+    const DeviceDense* d = device_core(f, c);
+    if (d) {
+      const std::vector<double> h = d->download();
+      mat.insert(mat.end(), h.begin(), h.end());
+    } else {
+      const uint64_t nc = f.core_ptr[c + 1] - f.core_ptr[c];
+      mat.insert(mat.end(), f.core_mat.begin() + (ptrdiff_t)f.core_mat_off[c],
+                 f.core_mat.begin() + (ptrdiff_t)(f.core_mat_off[c] + nc * nc));
+    }
+  }
+  f.core_mat = std::move(mat);
+  f.core_mat_off = std::move(off);
+  f.core_dev.clear();
+}
+
 std::vector<uint8_t> serialize_precond(const Plan& p) {
   if (p.precond != 1 && p.precond != 2) throw Error(E_INVALID, "solver has no preconditioner");
   Writer w;
@@ -1516,7 +1543,14 @@ std::vector<uint8_t> serialize_precond(const Plan& p) {
   w.put<uint64_t>(p.cfg.schur_seed); w.put<int32_t>(p.cfg.schur_split); w.put<int32_t>(p.cfg.dense_threshold);
   w.put<i64>(p.cfg.dense_core);
   w.vec(p.meta);
-  const PrecondFlat& f = p.pre;
+  PrecondFlat host;  // a copy with the device cores brought to the host, when there are any
+  bool dev = false;
+  for (const auto& d : p.pre.core_dev) dev = dev || d != nullptr;
+  if (dev) {
+    host = p.pre;
+    materialize_cores(host);
+  }
+  const PrecondFlat& f = dev ? host : p.pre;
   w.put<i64>(f.n_loc); w.put<i64>(f.n_red); w.put<i64>(f.n_sub);
   w.put<i64>(f.max_core); w.put<i64>(f.ac_nnz); w.put<i64>(f.c_nnz); w.put<i64>(f.dense_subs);
   flat_fields(w, const_cast<PrecondFlat&>(f));

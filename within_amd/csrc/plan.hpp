@@ -13,6 +13,7 @@
 //   * the explicit Gramian CSR (gramian/explicit.rs).
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -111,6 +112,19 @@ struct CrossTab {
   i64 n_r() const { return c.ncols; }
 };
 
+// A dense core's anchored inverse kept in device memory (FP64, n x n
+// row-major) where it was computed; host copies are made only on request
+// (serialisation, exports).  Shared by the objects of one plan.
+struct DeviceDense {
+  void* p = nullptr;
+  i64 n = 0;
+  DeviceDense() = default;
+  DeviceDense(const DeviceDense&) = delete;
+  DeviceDense& operator=(const DeviceDense&) = delete;
+  ~DeviceDense();                        // frees the device memory
+  std::vector<double> download() const;  // the n x n host copy
+};
+
 // Reduced-system factor (rounds of independent eliminations + dense core).
 struct Factor {
   i64 n = 0;
@@ -120,6 +134,7 @@ struct Factor {
   std::vector<double> col_w;
   std::vector<u32> core;
   std::vector<double> core_inv;  // |core|^2, anchored (zero last row / column)
+  std::shared_ptr<DeviceDense> core_dev;  // or the same matrix in device memory (core_inv empty)
 };
 
 struct Subdomain {
@@ -168,6 +183,9 @@ struct PrecondFlat {
   std::vector<uint64_t> core_mat_off; // n_cores
   std::vector<double> core_mat;       // concatenated row-major inverses
   std::vector<u32> core_row_core;     // per core row -> core id
+  // per core: its inverse in device memory (null: in core_mat); a device
+  // core takes no space in core_mat (its offset is that of the next core)
+  std::vector<std::shared_ptr<DeviceDense>> core_dev;
   // DOF combine: dof -> local entries (subdomain order)
   std::vector<u32> dof_ptr, dof_loc;
   i64 max_core = 0, ac_nnz = 0, c_nnz = 0, dense_subs = 0;
@@ -231,9 +249,16 @@ std::vector<Subdomain> build_subdomains(const Design& d);
 void build_local_solver(Subdomain& s, const LocalConfig& cfg);
 Csr build_gramian(const Design& d, const std::vector<FactorPass>& passes);
 std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n);
-// The same inverse evaluated on the GPU (dense.hip), bit for bit: false when
-// the anchored minor is not positive definite; throws wa::Error on a HIP
-// failure.
-bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::vector<double>& inv);
+// The same inverse evaluated on the GPU (dense.hip), bit for bit, and left in
+// device memory: false when the anchored minor is not positive definite;
+// throws wa::Error on a HIP failure.
+bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::shared_ptr<DeviceDense>& inv);
+// core c's inverse in device memory, or null
+inline const DeviceDense* device_core(const PrecondFlat& f, size_t c) {
+  return c < f.core_dev.size() ? f.core_dev[c].get() : nullptr;
+}
+// Copies device cores into core_mat (offsets rebuilt), so the arrays hold
+// every core on the host (serialisation).
+void materialize_cores(PrecondFlat& f);
 
 }  // namespace wa

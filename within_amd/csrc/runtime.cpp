@@ -521,11 +521,11 @@ void wa_solver::setup_device() {
     std::vector<size_t> big_cores;
     for (size_t c = 0; c < n_cores; ++c) {
       const uint32_t b = f.core_ptr[c], nc = f.core_ptr[c + 1] - b;
-      const double* src = f.core_mat.data() + f.core_mat_off[c];
       if (nc >= kDenseCore) {
         big_cores.push_back(c);
         std::fill(big_row.begin() + b, big_row.begin() + b + nc, 1);
       } else {
+        const double* src = f.core_mat.data() + f.core_mat_off[c];
         small_off[c] = small_mat.size();
         small_mat.insert(small_mat.end(), src, src + (size_t)nc * nc);
       }
@@ -537,11 +537,18 @@ void wa_solver::setup_device() {
         d.b = f.core_ptr[c];
         d.n = f.core_ptr[c + 1] - d.b;
         d.ld = (d.n + 31) / 32 * 32;
-        std::vector<T> m((size_t)d.n * d.ld, T(0));
-        const double* src = f.core_mat.data() + f.core_mat_off[c];
-        for (size_t i = 0; i < d.n; ++i)
-          for (size_t j = 0; j < d.n; ++j) m[i * d.ld + j] = (T)src[i * d.n + j];
-        d.M = upload(m);
+        if (const DeviceDense* dv = device_core(f, c)) {
+          // factored on the device: padded (and, for FP32, rounded) there
+          T* M = dalloc<T>((int64_t)d.n * d.ld);
+          launch_pad_core<T>(st, static_cast<const double*>(dv->p), d.n, M, d.ld);
+          d.M = M;
+        } else {
+          std::vector<T> m((size_t)d.n * d.ld, T(0));
+          const double* src = f.core_mat.data() + f.core_mat_off[c];
+          for (size_t i = 0; i < d.n; ++i)
+            for (size_t j = 0; j < d.n; ++j) m[i * d.ld + j] = (T)src[i * d.n + j];
+          d.M = upload(m);
+        }
         P.big.push_back(d);
       }
     };
@@ -3381,7 +3388,14 @@ wa_status wa_solver_subdomain_factor(const wa_solver* s, int64_t i, int64_t* nnz
   if (round_ptr) std::memcpy(round_ptr, f.round_ptr.data(), sizeof(uint32_t) * f.round_ptr.size());
   if (order) std::memcpy(order, f.order.data(), sizeof(uint32_t) * f.order.size());
   if (core) std::memcpy(core, f.core.data(), sizeof(uint32_t) * f.core.size());
-  if (core_inv) std::memcpy(core_inv, f.core_inv.data(), sizeof(double) * f.core_inv.size());
+  if (core_inv) {
+    if (f.core_dev) {
+      const std::vector<double> h = f.core_dev->download();
+      std::memcpy(core_inv, h.data(), sizeof(double) * h.size());
+    } else {
+      std::memcpy(core_inv, f.core_inv.data(), sizeof(double) * f.core_inv.size());
+    }
+  }
   return WA_OK;
 }
 
