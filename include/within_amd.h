@@ -69,8 +69,11 @@ typedef struct {
                               1 = FP32 inner Krylov + FP64 iterative refinement:
                               FP32 solves to max(tol', 1e-5) are repeated from
                               the FP64 residual until it meets tol, at most
-                              max_refinements + 8 times; converged is that final
-                              residual test.  With the explicit Gramian the
+                              max_refinements + 8 times (a column whose FP64
+                              residual falls by less than 10 % over one FP32
+                              solve is finished by FP64 refinement instead);
+                              converged is that final residual test.  With the
+                              explicit Gramian the
                               residual is b - Gx in FP64 (b = D'Wy), the same
                               vector as D'W(y - Dx) up to ~eps (|b| + |G||x|)  */
   int32_t block;           /* RHS per block-vector solve: 0 = auto (32), 1 = one

@@ -292,3 +292,30 @@ def test_single_factor_matches_the_reference_semantics(oracle_mod):
     means = np.bincount(c1[:, 0], weights=y, minlength=50) / np.maximum(np.bincount(c1[:, 0], minlength=50), 1)
     r1 = W.solve(c1, y, W.CG(tol=1e-13), preconditioner=W.Preconditioner.Off)
     assert np.abs(r1.demeaned - (y - means[c1[:, 0]])).max() <= 1e-12
+
+
+@pytest.mark.parametrize("operator", [W.OperatorRepr.Explicit, W.OperatorRepr.Implicit])
+def test_stalled_mixed_columns_finish_in_fp64(oracle_mod, monkeypatch, operator):
+    """A column whose FP64 residual stops falling under FP32 corrections is
+    finished by FP64 refinement from where it stands (SURVEY's FP32 -> FP64
+    retry).  A stall ratio of 1e-300 forces the hand-over after the first FP32
+    correction; the result must still be the oracle's projection."""
+    from within_amd import synth
+
+    p = synth.akm_panel(40000, 4000, 400, 10, seed=14, weights="lognormal")
+    codes, w = p["codes"], p["weights"]
+    Y = synth.akm_response(p, k=3)
+    ro = oracle_mod.OracleSolver(codes, w, tol=1e-10).solve_batch(Y)
+    cfg = W.CG(tol=1e-10, precision="mixed", operator=operator)
+    plain = W.Solver(codes, weights=w, config=cfg).solve_batch(Y)
+    monkeypatch.setenv("WITHIN_AMD_STALL_RATIO", "1e-300")
+    r = W.Solver(codes, weights=w, config=cfg).solve_batch(Y)
+    for c in range(Y.shape[1]):
+        assert r.converged[c] and rel(r.demeaned[:, c], ro["demeaned"][:, c]) <= 1e-8
+        # the FP64 corrections are counted with the FP32 ones
+        assert r.iterations[c] > 0
+    # without the forced stall, the mixed path never hands over on this problem
+    monkeypatch.delenv("WITHIN_AMD_STALL_RATIO")
+    again = W.Solver(codes, weights=w, config=cfg).solve_batch(Y)
+    np.testing.assert_array_equal(again.demeaned, plain.demeaned)
+    assert again.iterations == plain.iterations

@@ -419,6 +419,15 @@ static double inner_floor() {
   return v;
 }
 
+// A column of the mixed path whose FP64 residual falls by less than this
+// factor over one FP32 correction has stalled (WITHIN_AMD_STALL_RATIO
+// overrides; read on every solve)
+static double stall_ratio() {
+  const char* e = std::getenv("WITHIN_AMD_STALL_RATIO");
+  const double d = e ? std::atof(e) : 0.0;
+  return d > 0.0 && d < 1.0 ? d : 0.9;
+}
+
 static void check_device() {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1973,34 +1982,46 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
     // it the residual is D'W(y - Dx) from observation space.
     HIPCHK(hipMemsetAsync(X64, 0, vbytes, st));
     const int64_t max_outer = std::max<int64_t>(params.max_refinements, 0) + 8;
-    std::vector<int> live(kb, 1);
+    std::vector<int> live(kb, 1), fb(kb, 0);  // fb: handed to the FP64 refinement below
+    std::vector<double> rn_prev(kb, 0.0);
+    const double stall = stall_ratio();
     const bool dof_resid = has_gram;
+    // r = b - G x in FP64 (explicit Gramian) or D'W(y - Dx) from observation space
+    auto residual = [&]() {
+      if (dof_resid) {
+        hipEvent_t ex = nullptr;
+        tic(3, ex);  // counted with the observation-space work it replaces
+        launch_spmm<double>(st, g64, X64, R64, part64, kp);
+        launch_rsub<double>(st, R64, B64, m * kp);
+        toc(3, ex);
+        dof_final = true;
+      } else {
+        launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
+        rhs64(E, R64, kp);
+        fresh = true;
+      }
+      return col_norms(R64, m, kp);
+    };
     for (int64_t outer = 0; outer <= max_outer; ++outer) {
       std::vector<double> rn;
       if (outer == 0) {
         HIPCHK(hipMemcpyAsync(R64, B64, vbytes, hipMemcpyDeviceToDevice, st));
         rn = bn;
       } else {
-        if (dof_resid) {
-          hipEvent_t ex = nullptr;
-          tic(3, ex);  // counted with the observation-space work it replaces
-          launch_spmm<double>(st, g64, X64, R64, part64, kp);
-          launch_rsub<double>(st, R64, B64, m * kp);
-          toc(3, ex);
-          rn = col_norms(R64, m, kp);
-          dof_final = true;
-        } else {
-          launch_demean(st, n, Q, codes, offs, X64, Yr, E, kp);
-          rhs64(E, R64, kp);
-          fresh = true;
-          rn = col_norms(R64, m, kp);
-        }
+        rn = residual();
       }
       std::vector<double> scale(KMAX, 0.0), itol(kb, 1.0);
       std::vector<int> msk(KMAX, 0);
       bool anyc = false;
       for (int c = 0; c < kb; ++c) {
         if (live[c] && (rn[c] <= abs_tol[c] || rn[c] < F64_EPS)) live[c] = 0;
+        // an FP32 correction that hardly moved the FP64 residual: the column
+        // has stalled in FP32 and finishes in FP64
+        if (live[c] && outer > 0 && !std::isnan(rn[c]) && rn[c] > stall * rn_prev[c]) {
+          live[c] = 0;
+          fb[c] = 1;
+        }
+        rn_prev[c] = rn[c];
         if (live[c] && std::isnan(rn[c])) {
           // a NaN residual: every further pass would run out its budget, as the
           // FP64 path's initial solve and max_refinements corrections do
@@ -2014,6 +2035,9 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
           itol[c] = std::min(1.0, std::max(abs_tol[c] / rn[c], inner_floor()));
         }
       }
+      if (outer == max_outer)
+        for (int c = 0; c < kb; ++c)
+          if (live[c]) fb[c] = 1;  // the FP32 budget is spent: finish in FP64
       if (!anyc || outer == max_outer) break;
       HIPCHK(hipMemcpyAsync(s_(8), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
       launch_cast_scale<double, float>(st, R64, rhs32, s_(8), m, kp);
@@ -2025,6 +2049,35 @@ void wa_solver::solve_block(const double* Y, int64_t ldy, int c0, int kb, double
       fresh = dof_final = false;
       for (int c = 0; c < kb; ++c)
         if (live[c]) its[c] += cits[c];
+    }
+    // FP64 refinement of the columns that stalled in FP32 (or ran out of FP32
+    // passes): corrections solved in FP64 from the current x, as the FP64
+    // path's refinement loop (solver.rs:296-347), at most max_refinements + 1
+    bool anyfb = false;
+    for (int c = 0; c < kb; ++c) anyfb = anyfb || fb[c];
+    for (int64_t it = 0; anyfb && it <= std::max<int64_t>(params.max_refinements, 0); ++it) {
+      const std::vector<double> rn = (fresh || dof_final) ? col_norms(R64, m, kp) : residual();
+      std::vector<double> scale(KMAX, 0.0), ctol(kb, 1.0);
+      std::vector<int> msk(KMAX, 0);
+      bool anyc = false;
+      for (int c = 0; c < kb; ++c) {
+        if (fb[c] && (rn[c] <= abs_tol[c] || rn[c] < F64_EPS || std::isnan(rn[c]))) fb[c] = 0;
+        if (fb[c]) {
+          anyc = true;
+          scale[c] = 1.0;
+          msk[c] = 1;
+          ctol[c] = std::min(abs_tol[c] / rn[c], 1.0);
+        }
+      }
+      if (!anyc) break;
+      HIPCHK(hipMemcpyAsync(s_(8), scale.data(), sizeof(double) * KMAX, hipMemcpyHostToDevice, st));
+      launch_cast_scale<double, double>(st, R64, T64, s_(8), m, kp);
+      krylov<double>(kb, kp, T64, cx64, ctol, cconv, cits);
+      HIPCHK(hipMemcpyAsync(imask + KMAX + 1, msk.data(), sizeof(int) * KMAX, hipMemcpyHostToDevice, st));
+      launch_axpy_acc<double>(st, X64, cx64, nullptr, imask + KMAX + 1, m, kp);
+      fresh = dof_final = false;
+      for (int c = 0; c < kb; ++c)
+        if (msk[c]) its[c] += cits[c];
     }
   }
   // final residual for the final X: the observation-space D'W(y - Dx), or,
