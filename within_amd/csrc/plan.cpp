@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <thread>
 
 namespace wa {
 
@@ -1153,13 +1154,10 @@ static void flatten(Plan& p) {
 }
 
 DeviceLayout make_device_layout(const Plan& p) {
-  const Design& d = p.d;
-  const int Q = d.Q;
-  const i64 n = d.n_obs;
-  DeviceLayout L;
-  if (Q == 0) {
+  if (p.d.Q == 0) {
     // a preconditioner on its own (no design): the user numbering as is
-    L.m = d.n_dofs;
+    DeviceLayout L;
+    L.m = p.d.n_dofs;
     L.int_of.resize(L.m);
     std::iota(L.int_of.begin(), L.int_of.end(), 0u);
     L.ext_of = L.int_of;
@@ -1167,6 +1165,16 @@ DeviceLayout make_device_layout(const Plan& p) {
     L.pre = p.pre;
     return L;
   }
+  DeviceLayout L = layout_operator(p);
+  layout_precond(p, L);
+  return L;
+}
+
+DeviceLayout layout_operator(const Plan& p) {
+  const Design& d = p.d;
+  const int Q = d.Q;
+  const i64 n = d.n_obs;
+  DeviceLayout L;
   PhaseTimer t_levels("layout: levels");
   // active levels: the non-empty rows of each factor's pass
   // factor with the most levels, ordered by the factor with the next-most
@@ -1290,6 +1298,10 @@ DeviceLayout make_device_layout(const Plan& p) {
     }
   }
   t_gram.done();
+  return L;
+}
+
+void layout_precond(const Plan& p, DeviceLayout& L) {
   PhaseTimer t_pre("layout: preconditioner");
   // preconditioner: DOF indices renumbered; each subdomain's eliminated block
   // (its elimination rows) re-laid out in internal DOF order
@@ -1365,7 +1377,6 @@ DeviceLayout make_device_layout(const Plan& p) {
     for (i64 l = 0; l < nf.n_loc; ++l) nf.dof_loc[cur[nf.loc_gidx[l]]++] = (u32)l;
     L.pre = std::move(nf);
   }
-  return L;
 }
 
 void build_plan(Plan& p) {
@@ -1381,6 +1392,31 @@ void build_plan(Plan& p) {
     p.gram = build_gramian(d, p.passes);
     p.has_gram = true;
   }
+  // the operator's device layout reads only d, passes and gram: it runs on a
+  // thread of its own while the preconditioner is built below (which writes
+  // subs, pre and meta); joined on every way out of this function
+  struct Joiner {
+    std::thread t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } lay;
+  std::string lay_err;
+  int lay_code = 0;
+  if (p.layout_early && d.Q > 0) {
+    p.lay_op.reset();
+    lay.t = std::thread([&p, &lay_err, &lay_code] {
+      try {
+        p.lay_op = std::make_shared<DeviceLayout>(layout_operator(p));
+      } catch (const Error& e) {
+        lay_err = e.what();
+        lay_code = e.code;
+      } catch (const std::exception& e) {
+        lay_err = e.what();
+        lay_code = E_INVALID;
+      }
+    });
+  }
   if ((p.precond == 1 || p.precond == 2) && !p.pre_given) {
     {
       PhaseTimer t("subdomains");
@@ -1389,8 +1425,10 @@ void build_plan(Plan& p) {
     PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
-    // huge subdomains first, one at a time (their Schur sampling, rounds and
-    // dense cores use every thread), then the large ones side by side (one
+    // huge subdomains first, each on a host thread of its own with its own
+    // thread team (their Schur sampling, rounds and dense cores are parallel
+    // inside; while one subdomain's dense core is factored on the GPU the
+    // others keep the host busy), then the large ones side by side (one
     // thread each: a stack of many problems has hundreds of them), then the
     // rest.  Each build's arithmetic does not depend on its thread count.
     std::vector<size_t> serial, large, rest;
@@ -1422,7 +1460,10 @@ void build_plan(Plan& p) {
     };
     {
       PhaseTimer ts("local solvers, large subdomains");
-      for (size_t i : serial) build_one(i);
+      std::vector<std::thread> th;
+      for (size_t t = 1; t < serial.size(); ++t) th.emplace_back(build_one, serial[t]);
+      if (!serial.empty()) build_one(serial[0]);
+      for (std::thread& x : th) x.join();
     }
     {
       PhaseTimer tl("local solvers, large subdomains side by side");
@@ -1446,6 +1487,8 @@ void build_plan(Plan& p) {
       p.meta.push_back({sd.q, sd.r, sd.eliminate_q ? sd.ct.n_q() : sd.ct.n_r(), (i64)sd.f.core.size(), sd.ct.n_q(),
                         sd.ct.n_r(), (i64)sd.ct.c.indices.size()});
   }
+  if (lay.t.joinable()) lay.t.join();
+  if (lay_code) throw Error(lay_code, lay_err);
 }
 
 // ---------------------------------------------------------------------------

@@ -197,6 +197,7 @@ struct PrecondFlat {
 // dense-core size, block sizes and cross-tab nonzeros.
 struct SubMeta { int32_t q = 0, r = 0; i64 n_elim = 0, n_core = 0, n_q = 0, n_r = 0, c_nnz = 0; };
 
+struct DeviceLayout;
 struct Plan {
   Design d;
   LocalConfig cfg;
@@ -209,6 +210,11 @@ struct Plan {
   PrecondFlat pre;
   Csr gram;                   // explicit Gramian (when needed)
   bool has_gram = false;
+  // set by a caller that lays the plan out for the device: build_plan then
+  // lays out the operator (numbering, passes, Gramian) on a host thread of
+  // its own while the preconditioner is built, into lay_op
+  bool layout_early = false;
+  std::shared_ptr<DeviceLayout> lay_op;
 };
 
 // Serialized preconditioner (this library's own format, host byte order):
@@ -243,6 +249,10 @@ struct DeviceLayout {
   PrecondFlat pre;                  // local / DOF arrays in internal numbering
 };
 DeviceLayout make_device_layout(const Plan& p);
+// the same in two parts: the operator (numbering, passes, Gramian), then the
+// preconditioner arrays (p.pre) on top of it
+DeviceLayout layout_operator(const Plan& p);
+void layout_precond(const Plan& p, DeviceLayout& L);
 
 // Individual steps (exposed for tests through the C ABI).
 std::vector<Subdomain> build_subdomains(const Design& d);

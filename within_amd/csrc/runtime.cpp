@@ -451,7 +451,15 @@ void wa_solver::setup_device() {
   check_device();
   HIPCHK(hipGetDevice(&dev));
   phase("init");
-  const DeviceLayout lay = make_device_layout(plan);
+  DeviceLayout lay;
+  if (plan.lay_op) {
+    // the operator part was laid out while the preconditioner was built
+    lay = std::move(*plan.lay_op);
+    plan.lay_op.reset();
+    layout_precond(plan, lay);
+  } else {
+    lay = make_device_layout(plan);
+  }
   phase("layout");
   n = d.n_obs;
   m = lay.m;
@@ -2808,6 +2816,7 @@ static std::unique_ptr<wa_solver> create_solver(const Design& dd, const wa_param
     s->params.preconditioner = pl.precond;
     local_from_cfg(pl.cfg, s->params.local);
   }
+  pl.layout_early = !s->host_only;
   build_plan(pl);
   if (!s->host_only) s->setup_device();
   return s;
