@@ -704,6 +704,19 @@ void wa_solver::setup_device() {
             while (b1 < a1 && f.red_sub[f.lvl_rows[b1]] < s1) ++b1;
             append_schedule(sc, ptr, a0, b1, kChunk);
           }
+          if (std::getenv("WITHIN_AMD_VERBOSE")) {
+            // per level: rows, elements, longest row
+            std::fprintf(stderr, "[within_amd] pair %zu %s levels:", pairs.size(), &ptr == &f.fin_ptr ? "fwd" : "bwd");
+            for (const Segment& g : sc.segs) {
+              int64_t el = 0, mx = 0;
+              for (u32 r = g.r0; r < g.r1; ++r) {
+                el += ptr[r + 1] - ptr[r];
+                mx = std::max<int64_t>(mx, ptr[r + 1] - ptr[r]);
+              }
+              std::fprintf(stderr, " %u/%lld/%lld", g.r1 - g.r0, (long long)el, (long long)mx);
+            }
+            std::fprintf(stderr, "\n");
+          }
           return upload_sched(sc);
         };
         ps.s_fwd = pair_levels(f.fin_ptr);
