@@ -61,14 +61,22 @@ __global__ void __launch_bounds__(TB * TB) k_chol_update(double* __restrict__ L,
   const int64_t j = c0 + tx;
   const bool live = i < m && tx < w && j <= i;
   double acc = live ? L[i * m + j] : 0.0;
+  // thread (r, kk) stages one value of each operand; the next chunk's values
+  // are loaded while the current chunk is summed
+  const int r = threadIdx.x / KC, kk0 = threadIdx.x % KC;
+  double an = 0.0, bn = 0.0;
+  auto fetch = [&](int kc) {
+    const int kw = min(KC, c0 - kc);
+    an = (ib + r < m && kk0 < kw) ? L[(ib + r) * m + kc + kk0] : 0.0;
+    bn = (r < w && kk0 < kw) ? L[(int64_t)(c0 + r) * m + kc + kk0] : 0.0;
+  };
+  if (c0 > 0) fetch(0);
   for (int kc = 0; kc < c0; kc += KC) {
     const int kw = min(KC, c0 - kc);
-    {
-      const int r = threadIdx.x / KC, kk = threadIdx.x % KC;
-      As[r][kk] = (ib + r < m && kk < kw) ? L[(ib + r) * m + kc + kk] : 0.0;
-      Bs[r][kk] = (r < w && kk < kw) ? L[(int64_t)(c0 + r) * m + kc + kk] : 0.0;
-    }
+    As[r][kk0] = an;
+    Bs[r][kk0] = bn;
     __syncthreads();
+    if (kc + KC < c0) fetch(kc + KC);
     for (int kk = 0; kk < kw; ++kk) acc = acc - As[ty][kk] * Bs[tx][kk];
     __syncthreads();
   }
@@ -146,15 +154,21 @@ __global__ void __launch_bounds__(TB * TB) k_inv_lower(const double* __restrict_
   const int64_t i = r0 + ty;
   const int j = j0 + tx;
   double acc = (i == j) ? 1.0 : 0.0;
+  // the next chunk's staged values are loaded while the current one is summed
+  const int lr = threadIdx.x / KC, lk = threadIdx.x % KC, k2 = threadIdx.x / TB, jj = threadIdx.x % TB;
+  double ln = 0.0, un = 0.0;
+  auto fetch = [&](int kc) {
+    const int kw = min(KC, r0 - kc);
+    ln = (r0 + lr < m && lk < kw) ? L[(int64_t)(r0 + lr) * m + kc + lk] : 0.0;
+    un = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
+  };
+  if (j0 < r0) fetch(j0);
   for (int kc = j0; kc < r0; kc += KC) {
     const int kw = min(KC, r0 - kc);
-    {
-      const int r = threadIdx.x / KC, kk = threadIdx.x % KC;
-      Ls[r][kk] = (r0 + r < m && kk < kw) ? L[(int64_t)(r0 + r) * m + kc + kk] : 0.0;
-      const int k2 = threadIdx.x / TB, jj = threadIdx.x % TB;
-      Us[k2][jj] = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
-    }
+    Ls[lr][lk] = ln;
+    Us[k2][jj] = un;
     __syncthreads();
+    if (kc + KC < r0) fetch(kc + KC);
     for (int kk = 0; kk < kw; ++kk) acc = acc - Ls[ty][kk] * Us[kk][tx];
     __syncthreads();
   }
@@ -185,17 +199,37 @@ __global__ void __launch_bounds__(RB) k_inv_sym(const double* __restrict__ U, do
   double acc[TB];
 #pragma unroll
   for (int jj = 0; jj < TB; ++jj) acc[jj] = 0.0;
+  // the next chunk's staged values are loaded while the current one is summed
+  constexpr int NA = KC * RB / RB, NB = KC * TB / RB;  // values per thread: KC, KC / 2
+  double an[NA], bn[NB];
+  auto fetch = [&](int kc) {
+    const int kw = min(KC, m - kc);
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = t + u * RB, kk = e / RB, r = e % RB;
+      an[u] = (kk < kw && i0 + r < m) ? U[(int64_t)(kc + kk) * m + i0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = t + u * RB, kk = e / TB, jj = e % TB;
+      bn[u] = (kk < kw && j0 + jj < m) ? U[(int64_t)(kc + kk) * m + j0 + jj] : 0.0;
+    }
+  };
+  if (i0 < m) fetch(i0);
   for (int kc = i0; kc < m; kc += KC) {
     const int kw = min(KC, m - kc);
-    for (int e = t; e < KC * RB; e += RB) {
-      const int kk = e / RB, r = e % RB;
-      Ua[kk][r] = (kk < kw && i0 + r < m) ? U[(int64_t)(kc + kk) * m + i0 + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = t + u * RB;
+      Ua[e / RB][e % RB] = an[u];
     }
-    for (int e = t; e < KC * TB; e += RB) {
-      const int kk = e / TB, jj = e % TB;
-      Ub[kk][jj] = (kk < kw && j0 + jj < m) ? U[(int64_t)(kc + kk) * m + j0 + jj] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = t + u * RB;
+      Ub[e / TB][e % TB] = bn[u];
     }
     __syncthreads();
+    if (kc + KC < m) fetch(kc + KC);
     for (int kk = 0; kk < kw; ++kk) {
       const double uki = Ua[kk][t];
 #pragma unroll

@@ -1381,6 +1381,31 @@ void layout_precond(const Plan& p, DeviceLayout& L) {
 
 void build_plan(Plan& p) {
   const Design& d = p.d;
+  // the subdomains (cross-tabulations and components) read only d: built on a
+  // thread of their own while the operator passes and the Gramian are built
+  const bool want_subs = (p.precond == 1 || p.precond == 2) && !p.pre_given;
+  struct SubsJoiner {
+    std::thread t;
+    ~SubsJoiner() {
+      if (t.joinable()) t.join();
+    }
+  } subs_thread;
+  std::string subs_err;
+  int subs_code = 0;
+  if (want_subs) {
+    subs_thread.t = std::thread([&p, &subs_err, &subs_code] {
+      try {
+        PhaseTimer t("subdomains");
+        p.subs = build_subdomains(p.d);
+      } catch (const Error& e) {
+        subs_err = e.what();
+        subs_code = e.code;
+      } catch (const std::exception& e) {
+        subs_err = e.what();
+        subs_code = E_INVALID;
+      }
+    });
+  }
   p.passes.resize(d.Q);
   {
     PhaseTimer t("operator passes");
@@ -1417,11 +1442,9 @@ void build_plan(Plan& p) {
       }
     });
   }
-  if ((p.precond == 1 || p.precond == 2) && !p.pre_given) {
-    {
-      PhaseTimer t("subdomains");
-      p.subs = build_subdomains(d);
-    }
+  if (want_subs) {
+    subs_thread.t.join();
+    if (subs_code) throw Error(subs_code, subs_err);
     PhaseTimer t("local solvers");
     std::vector<std::string> err(p.subs.size());
     std::vector<int> code(p.subs.size(), 0);
