@@ -16,7 +16,10 @@
 namespace wa {
 // This CPU build factors every dense core on the host (the GPU path is never
 // taken: cfg.device is false), so no DeviceDense is ever made.
-bool device_anchored_inverse(const std::vector<double>&, i64, std::shared_ptr<DeviceDense>&) { return false; }
+bool device_anchored_inverse(i64, const u32*, const u32*, const double*, const double*,
+                             std::shared_ptr<DeviceDense>&) {
+  return false;
+}
 DeviceDense::~DeviceDense() {}
 std::vector<double> DeviceDense::download() const { return {}; }
 }  // namespace wa

@@ -2,7 +2,8 @@
 // factor on the GPU (operator/local_solver.rs:207-300 semantics), used at
 // setup for large cores.
 //
-// The arithmetic of every entry is the one fixed by anchored_inverse in
+// The core's Laplacian arrives as CSR (the host assembles no dense matrix);
+// the arithmetic of every entry is the one fixed by anchored_inverse in
 // plan.cpp (and by the oracle's plain loops):
 //   l_ij = (a_ij - sum_{k<j} l_ik l_jk) / l_jj,  l_jj = sqrt(a_jj - sum_{k<j} l_jk^2)
 //   U_ij = (delta_ij - sum_{k=j}^{i-1} l_ik U_kj) / l_ii         (U = L^-1)
@@ -248,14 +249,20 @@ __global__ void __launch_bounds__(RB) k_inv_sym(const double* __restrict__ U, do
     }
 }
 
-// L[i][j] = A[i][j] for j <= i < m (the anchored minor's lower triangle of the
-// n x n matrix A), zero above the diagonal
-__global__ void k_minor_lower(const double* __restrict__ A, int64_t n, double* __restrict__ L, int64_t m) {
-  const int64_t tot = m * m;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t / m, j = t % m;
-    L[t] = j <= i ? A[i * n + j] : 0.0;
+// The anchored minor's lower triangle (rows and columns [0, m), m = n - 1) of
+// the n x n Laplacian given as CSR with positive off-diagonal weights and its
+// diagonal: L[i][j] = -w_ij for j < i, L[i][i] = diag_i; L is zeroed first.
+// One workgroup per row.
+__global__ void k_minor_from_csr(const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
+                                 const double* __restrict__ w, const double* __restrict__ diag,
+                                 double* __restrict__ L, int64_t m) {
+  const int64_t i = blockIdx.x;
+  if (i >= m) return;
+  for (uint32_t e = ptr[i] + threadIdx.x; e < ptr[i + 1]; e += blockDim.x) {
+    const uint32_t j = col[e];
+    if (j < i) L[i * m + j] = -w[e];
   }
+  if (threadIdx.x == 0) L[i * m + i] = diag[i];
 }
 
 struct DevBuf {
@@ -280,7 +287,8 @@ std::vector<double> DeviceDense::download() const {
   return h;
 }
 
-bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::shared_ptr<DeviceDense>& out) {
+bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double* w, const double* diag,
+                             std::shared_ptr<DeviceDense>& out) {
   const bool report = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
   auto t0 = std::chrono::steady_clock::now();
   auto lap = [&](const char* what) {
@@ -310,11 +318,19 @@ bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::shared_pt
   const size_t mm = (size_t)m * m * sizeof(double);
   DevBuf dL(mm), dU(mm), dF(sizeof(int));
   {
-    // the matrix goes up once; its anchored minor's lower triangle is taken on the device
-    DevBuf dA(nn);
-    DCHK(hipMemcpy(dA.p, a.data(), nn, hipMemcpyHostToDevice));
-    k_minor_lower<<<(unsigned)((m * m + 255) / 256 > 65536 ? 65536 : (m * m + 255) / 256), 256>>>(
-        dA.as<double>(), n, dL.as<double>(), m);
+    // the sparse Laplacian goes up; the minor's dense lower triangle is laid out on the device
+    const size_t nnz = ptr[n];
+    DevBuf dP(sizeof(u32) * (n + 1)), dC(sizeof(u32) * std::max<size_t>(nnz, 1)),
+        dW(sizeof(double) * std::max<size_t>(nnz, 1)), dD(sizeof(double) * n);
+    DCHK(hipMemcpy(dP.p, ptr, sizeof(u32) * (n + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+      DCHK(hipMemcpy(dC.p, col, sizeof(u32) * nnz, hipMemcpyHostToDevice));
+      DCHK(hipMemcpy(dW.p, w, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    DCHK(hipMemcpy(dD.p, diag, sizeof(double) * n, hipMemcpyHostToDevice));
+    DCHK(hipMemset(dL.p, 0, mm));
+    k_minor_from_csr<<<(unsigned)m, 256>>>(dP.as<u32>(), dC.as<u32>(), dW.as<double>(), dD.as<double>(),
+                                            dL.as<double>(), m);
     DCHK(hipGetLastError());
     DCHK(hipDeviceSynchronize());
   }

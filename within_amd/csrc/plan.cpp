@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <parallel/algorithm>
 #include <thread>
 
 namespace wa {
@@ -603,21 +604,34 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       i64 c = (i64)core.size();
       std::vector<i64> at(n, -1);
       for (i64 i = 0; i < c; ++i) at[core[i]] = i;
-      std::vector<double> a(c * c, 0.0);
+      // the core's Laplacian: its rows' neighbours (core numbering) and weights,
+      // and its diagonal, the sum of the row's weights in adjacency order
+      std::vector<u32> cptr(c + 1, 0);
+      for (i64 i = 0; i < c; ++i) cptr[i + 1] = cptr[i] + (u32)adj[core[i]].size();
+      std::vector<u32> ccol(cptr[c]);
+      std::vector<double> cw(cptr[c]), cdiag(c);
+#pragma omp parallel for schedule(static) if (c >= 1024)
       for (i64 i = 0; i < c; ++i) {
         double sum = 0.0;
+        u32 o = cptr[i];
         for (const Nbr& kv : adj[core[i]]) {
           sum += kv.second;
-          a[i * c + at[kv.first]] = -kv.second;
+          ccol[o] = (u32)at[kv.first];
+          cw[o++] = kv.second;
         }
-        a[i * c + i] = sum;
+        cdiag[i] = sum;
       }
       const double t_rounds = secs();
       std::vector<double> inv;
       std::shared_ptr<DeviceDense> dinv;
       if (device && c >= kDeviceCore) {
-        if (!device_anchored_inverse(a, c, dinv)) dinv.reset();
+        if (!device_anchored_inverse(c, cptr.data(), ccol.data(), cw.data(), cdiag.data(), dinv)) dinv.reset();
       } else {
+        std::vector<double> a((size_t)c * c, 0.0);
+        for (i64 i = 0; i < c; ++i) {
+          for (u32 o = cptr[i]; o < cptr[i + 1]; ++o) a[i * c + ccol[o]] = -cw[o];
+          a[i * c + i] = cdiag[i];
+        }
         inv = anchored_inverse(a, c);
       }
       if (report)
@@ -636,7 +650,8 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
     order.clear();
     for (i64 v = 0; v < n; ++v)
       if (live[v]) { key[v] = round_key(seed, round, (uint64_t)v); order.push_back((u32)v); }
-    std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+    // a total order (ids break ties): the parallel sort gives std::sort's result
+    __gnu_parallel::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
       size_t da = adj[a].size(), db = adj[b].size();
       return da != db ? da < db : (key[a] != key[b] ? key[a] < key[b] : a < b);
     });

@@ -260,9 +260,11 @@ void build_local_solver(Subdomain& s, const LocalConfig& cfg);
 Csr build_gramian(const Design& d, const std::vector<FactorPass>& passes);
 std::vector<double> anchored_inverse(const std::vector<double>& a, i64 n);
 // The same inverse evaluated on the GPU (dense.hip), bit for bit, and left in
-// device memory: false when the anchored minor is not positive definite;
-// throws wa::Error on a HIP failure.
-bool device_anchored_inverse(const std::vector<double>& a, i64 n, std::shared_ptr<DeviceDense>& inv);
+// device memory, of the n x n Laplacian given as CSR (row i: neighbours col
+// with positive weights w, diagonal diag_i): false when the anchored minor is
+// not positive definite; throws wa::Error on a HIP failure.
+bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double* w, const double* diag,
+                             std::shared_ptr<DeviceDense>& inv);
 // core c's inverse in device memory, or null
 inline const DeviceDense* device_core(const PrecondFlat& f, size_t c) {
   return c < f.core_dev.size() ? f.core_dev[c].get() : nullptr;
