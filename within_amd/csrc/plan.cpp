@@ -153,7 +153,7 @@ static void counting_sort(std::vector<u32>& idx, const u32* key, i64 nkeys) {
 static FactorPass build_pass(const Design& d, int q) {
   FactorPass fp;
   fp.q = q;
-  i64 n = d.n_obs;
+  const i64 n = d.n_obs, L = d.levels[q];
   std::vector<u32> idx(n);
   std::iota(idx.begin(), idx.end(), 0u);
   // secondary key: the largest other factor (gather locality inside a level)
@@ -161,26 +161,26 @@ static FactorPass build_pass(const Design& d, int q) {
   for (int r = 0; r < d.Q; ++r)
     if (r != q && (sec < 0 || d.levels[r] > d.levels[sec])) sec = r;
   if (sec >= 0) counting_sort(idx, d.codes[sec].data(), d.levels[sec]);
-  counting_sort(idx, d.codes[q].data(), d.levels[q]);
-  fp.obs = idx;
-  fp.row_ptr.assign(d.levels[q] + 1, 0);
+  counting_sort(idx, d.codes[q].data(), L);
+  fp.row_ptr.assign(L + 1, 0);
   for (i64 i = 0; i < n; ++i) fp.row_ptr[d.codes[q][i] + 1]++;
-  for (i64 j = 0; j < d.levels[q]; ++j) fp.row_ptr[j + 1] += fp.row_ptr[j];
-  int Qm = d.Q - 1;
+  for (i64 j = 0; j < L; ++j) fp.row_ptr[j + 1] += fp.row_ptr[j];
+  const int Qm = d.Q - 1;
   fp.other.resize((size_t)n * Qm);
+  if (!d.weights.empty()) fp.w.resize(n);
+#pragma omp parallel for schedule(static)
   for (i64 t = 0; t < n; ++t) {
-    u32 o = idx[t];
+    const u32 o = idx[t];
     int s = 0;
     for (int r = 0; r < d.Q; ++r)
       if (r != q) fp.other[(size_t)t * Qm + s++] = (u32)(d.offset[r] + d.codes[r][o]);
+    if (!d.weights.empty()) fp.w[t] = d.weights[o];
   }
-  if (!d.weights.empty()) {
-    fp.w.resize(n);
-    for (i64 t = 0; t < n; ++t) fp.w[t] = d.weights[idx[t]];
-  }
-  fp.diag.assign(d.levels[q], 0.0);
-  for (i64 j = 0; j < d.levels[q]; ++j)
+  fp.diag.assign(L, 0.0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (i64 j = 0; j < L; ++j)
     for (u32 t = fp.row_ptr[j]; t < fp.row_ptr[j + 1]; ++t) fp.diag[j] += d.w(idx[t]);
+  fp.obs = std::move(idx);
   return fp;
 }
 
@@ -1097,49 +1097,94 @@ static void flatten(Plan& p) {
       f.keep_rowsum[R0 + j] = rs;
     }
   }
-  // factor levels: round t of every subdomain, then the core gather level
-  std::vector<std::vector<std::pair<u32, double>>> in_edges(f.n_red);
+  // factor levels: round t of every subdomain, then the core gather level.
+  // In-edges of each reduced row as CSR, in (subdomain, elimination order,
+  // column) order; then the level rows in order, their forward (in-edge) and
+  // backward (out-edge) lists sized, offset and filled in parallel.
+  std::vector<u32> in_ptr(f.n_red + 1, 0);
   for (i64 si = 0; si < f.n_sub; ++si) {
     const Factor& F = subs[si].f;
-    u32 R0 = f.red_ptr[si];
+    const u32 R0 = f.red_ptr[si];
     for (u32 k = 0; k < F.round_ptr.back(); ++k) {
-      u32 a = F.order[k];
+      const u32 a = F.order[k];
       if (!(F.pivot[a] > 0.0)) continue;
-      for (u32 p = F.col_ptr[a]; p < F.col_ptr[a + 1]; ++p)
-        in_edges[R0 + F.col_idx[p]].push_back({R0 + a, F.col_w[p] / F.pivot[a]});
-      f.inv_pivot[R0 + a] = 1.0 / F.pivot[a];
+      for (u32 p = F.col_ptr[a]; p < F.col_ptr[a + 1]; ++p) in_ptr[R0 + F.col_idx[p] + 1]++;
     }
-    f.ac_nnz += (i64)F.col_idx.size();
   }
+  for (i64 r = 0; r < f.n_red; ++r) in_ptr[r + 1] += in_ptr[r];
+  std::vector<u32> in_src(in_ptr[f.n_red]);
+  std::vector<double> in_w(in_ptr[f.n_red]);
+  {
+    std::vector<u32> cur(in_ptr.begin(), in_ptr.end() - 1);
+    for (i64 si = 0; si < f.n_sub; ++si) {
+      const Factor& F = subs[si].f;
+      const u32 R0 = f.red_ptr[si];
+      for (u32 k = 0; k < F.round_ptr.back(); ++k) {
+        const u32 a = F.order[k];
+        if (!(F.pivot[a] > 0.0)) continue;
+        for (u32 p = F.col_ptr[a]; p < F.col_ptr[a + 1]; ++p) {
+          const u32 at = cur[R0 + F.col_idx[p]]++;
+          in_src[at] = R0 + a;
+          in_w[at] = F.col_w[p] / F.pivot[a];
+        }
+        f.inv_pivot[R0 + a] = 1.0 / F.pivot[a];
+      }
+      f.ac_nnz += (i64)F.col_idx.size();
+    }
+  }
+  // the level rows: (level, subdomain, elimination order); per row its
+  // subdomain and whether it pulls its later neighbours on the way back
+  // (eliminated rows do; core rows get their values from the dense solve)
+  std::vector<u32> row_sub;
+  std::vector<char> row_out;
   f.lvl_ptr.assign(1, 0);
-  f.fin_ptr.assign(1, 0);
-  f.bout_ptr.assign(1, 0);
   for (i64 t = 0; t <= max_rounds; ++t) {
     for (i64 si = 0; si < f.n_sub; ++si) {
       const Factor& F = subs[si].f;
-      u32 R0 = f.red_ptr[si];
-      std::vector<u32> rows;
+      const u32 R0 = f.red_ptr[si];
+      auto add = [&](u32 v) {
+        f.lvl_rows.push_back(R0 + v);
+        row_sub.push_back((u32)si);
+        row_out.push_back(t < max_rounds && F.pivot[v] > 0.0 ? 1 : 0);
+      };
       if (t < (i64)F.round_ptr.size() - 1) {
-        for (u32 k = F.round_ptr[t]; k < F.round_ptr[t + 1]; ++k) rows.push_back(F.order[k]);
+        for (u32 k = F.round_ptr[t]; k < F.round_ptr[t + 1]; ++k) add(F.order[k]);
       } else if (t == max_rounds) {
-        rows = F.core;  // last level: core vertices gather their in-edges
-      }
-      for (u32 v : rows) {
-        u32 g = R0 + v;
-        f.lvl_rows.push_back(g);
-        for (const auto& pe : in_edges[g]) { f.fin_idx.push_back(pe.first); f.fin_val.push_back(pe.second); }
-        f.fin_ptr.push_back((u32)f.fin_idx.size());
-        // eliminated rows pull their later neighbours on the way back; core
-        // rows get their values from the dense solve instead
-        if (t < max_rounds && F.pivot[v] > 0.0)
-          for (u32 p = F.col_ptr[v]; p < F.col_ptr[v + 1]; ++p) {
-            f.bout_idx.push_back(R0 + F.col_idx[p]);
-            f.bout_val.push_back(F.col_w[p] / F.pivot[v]);
-          }
-        f.bout_ptr.push_back((u32)f.bout_idx.size());
+        for (u32 v : F.core) add(v);  // last level: core vertices gather their in-edges
       }
     }
     f.lvl_ptr.push_back((u32)f.lvl_rows.size());
+  }
+  const i64 nrows = (i64)f.lvl_rows.size();
+  f.fin_ptr.assign(nrows + 1, 0);
+  f.bout_ptr.assign(nrows + 1, 0);
+  for (i64 k = 0; k < nrows; ++k) {
+    const u32 g = f.lvl_rows[k];
+    const Factor& F = subs[row_sub[k]].f;
+    const u32 v = g - f.red_ptr[row_sub[k]];
+    f.fin_ptr[k + 1] = f.fin_ptr[k] + (in_ptr[g + 1] - in_ptr[g]);
+    f.bout_ptr[k + 1] = f.bout_ptr[k] + (row_out[k] ? F.col_ptr[v + 1] - F.col_ptr[v] : 0);
+  }
+  f.fin_idx.resize(f.fin_ptr[nrows]);
+  f.fin_val.resize(f.fin_ptr[nrows]);
+  f.bout_idx.resize(f.bout_ptr[nrows]);
+  f.bout_val.resize(f.bout_ptr[nrows]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (i64 k = 0; k < nrows; ++k) {
+    const u32 g = f.lvl_rows[k];
+    const u32 si = row_sub[k], R0 = f.red_ptr[si], v = g - R0;
+    const Factor& F = subs[si].f;
+    u32 o = f.fin_ptr[k];
+    for (u32 e = in_ptr[g]; e < in_ptr[g + 1]; ++e, ++o) {
+      f.fin_idx[o] = in_src[e];
+      f.fin_val[o] = in_w[e];
+    }
+    if (!row_out[k]) continue;
+    o = f.bout_ptr[k];
+    for (u32 p = F.col_ptr[v]; p < F.col_ptr[v + 1]; ++p, ++o) {
+      f.bout_idx[o] = R0 + F.col_idx[p];
+      f.bout_val[o] = F.col_w[p] / F.pivot[v];
+    }
   }
   // dense cores
   f.core_ptr.assign(1, 0);
@@ -1424,8 +1469,25 @@ void build_plan(Plan& p) {
   p.passes.resize(d.Q);
   {
     PhaseTimer t("operator passes");
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int q = 0; q < d.Q; ++q) p.passes[q] = build_pass(d, q);
+    // the passes side by side, each on a host thread with its own team
+    std::vector<std::thread> th;
+    std::vector<std::string> perr(d.Q);
+    for (int q = 1; q < d.Q; ++q)
+      th.emplace_back([&p, &d, &perr, q] {
+        try {
+          p.passes[q] = build_pass(d, q);
+        } catch (const std::exception& e) {
+          perr[q] = e.what();
+        }
+      });
+    try {
+      if (d.Q > 0) p.passes[0] = build_pass(d, 0);
+    } catch (const std::exception& e) {
+      perr[0] = e.what();
+    }
+    for (std::thread& x : th) x.join();
+    for (int q = 0; q < d.Q; ++q)
+      if (!perr[q].empty()) throw Error(E_INVALID, perr[q]);
   }
   if (p.explicit_op || p.precond == 2) {
     PhaseTimer t("explicit gramian");
