@@ -52,36 +52,65 @@ struct HipFail : Error {
   } while (0)
 
 // Left-looking update of the column tile [c0, c0 + w) for rows >= c0:
-// L_ij -= sum_{k<c0} L_ik L_jk, k ascending, one thread per entry (the partial
-// sums are stored and the tile kernels continue them from k = c0).
-__global__ void __launch_bounds__(TB * TB) k_chol_update(double* __restrict__ L, int m, int c0, int w) {
-  __shared__ double As[TB][KC + 1];
-  __shared__ double Bs[TB][KC + 1];
-  const int tx = threadIdx.x % TB, ty = threadIdx.x / TB;
-  const int64_t ib = c0 + (int64_t)blockIdx.x * TB, i = ib + ty;
-  const int64_t j = c0 + tx;
-  const bool live = i < m && tx < w && j <= i;
-  double acc = live ? L[i * m + j] : 0.0;
-  // thread (r, kk) stages one value of each operand; the next chunk's values
-  // are loaded while the current chunk is summed
-  const int r = threadIdx.x / KC, kk0 = threadIdx.x % KC;
-  double an = 0.0, bn = 0.0;
+// L_ij -= sum_{k<c0} L_ik L_jk, k ascending (the partial sums are stored and
+// the tile kernels continue them from k = c0).  A workgroup takes 32 rows,
+// each thread a 2 x 2 block of entries whose sums it runs term by term; the
+// operands of a 32-wide k chunk are staged in LDS k-major, and the next
+// chunk's are loaded while one is summed.
+constexpr int UR = 32;  // rows per update workgroup
+__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ L, int m, int c0, int w) {
+  __shared__ double As[KC][UR + 2];
+  __shared__ double Bs[KC][TB + 2];
+  const int t = threadIdx.x, tr = t / 16, tc = t % 16;
+  const int64_t ib = c0 + (int64_t)blockIdx.x * UR;
+  double acc[2][2];
+  bool live[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int64_t i = ib + 2 * tr + x;
+      const int jl = 2 * tc + y;
+      live[x][y] = i < m && jl < w && c0 + jl <= i;
+      acc[x][y] = live[x][y] ? L[i * m + c0 + jl] : 0.0;
+    }
+  constexpr int NS = UR * KC / 256;
+  double an[NS], bn[NS];
   auto fetch = [&](int kc) {
     const int kw = min(KC, c0 - kc);
-    an = (ib + r < m && kk0 < kw) ? L[(ib + r) * m + kc + kk0] : 0.0;
-    bn = (r < w && kk0 < kw) ? L[(int64_t)(c0 + r) * m + kc + kk0] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u, r = e / KC, kk = e % KC;
+      an[u] = (ib + r < m && kk < kw) ? L[(ib + r) * m + kc + kk] : 0.0;
+      bn[u] = (r < w && kk < kw) ? L[(int64_t)(c0 + r) * m + kc + kk] : 0.0;
+    }
   };
   if (c0 > 0) fetch(0);
   for (int kc = 0; kc < c0; kc += KC) {
     const int kw = min(KC, c0 - kc);
-    As[r][kk0] = an;
-    Bs[r][kk0] = bn;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u;
+      As[e % KC][e / KC] = an[u];
+      Bs[e % KC][e / KC] = bn[u];
+    }
     __syncthreads();
     if (kc + KC < c0) fetch(kc + KC);
-    for (int kk = 0; kk < kw; ++kk) acc = acc - As[ty][kk] * Bs[tx][kk];
+    for (int kk = 0; kk < kw; ++kk) {
+      const double a0 = As[kk][2 * tr], a1 = As[kk][2 * tr + 1];
+      const double b0 = Bs[kk][2 * tc], b1 = Bs[kk][2 * tc + 1];
+      acc[0][0] = acc[0][0] - a0 * b0;
+      acc[0][1] = acc[0][1] - a0 * b1;
+      acc[1][0] = acc[1][0] - a1 * b0;
+      acc[1][1] = acc[1][1] - a1 * b1;
+    }
     __syncthreads();
   }
-  if (live) L[i * m + j] = acc;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+      if (live[x][y]) L[(ib + 2 * tr + x) * m + c0 + 2 * tc + y] = acc[x][y];
 }
 
 // Diagonal tile [c0, c0 + w) after its update: rows c0 + t, t < w, finished
@@ -142,50 +171,81 @@ __global__ void __launch_bounds__(RB) k_chol_below(double* __restrict__ L, int m
     for (int jj = 0; jj < w; ++jj) L[i * m + c0 + jj] = r_[jj];
 }
 
-// U rows [r0, r0 + TB) of column tile j0 = blockIdx.x * TB (<= r0): one
-// thread per entry for k in [j0, r0), then the tile's own rows in order.
-__global__ void __launch_bounds__(TB * TB) k_inv_lower(const double* __restrict__ L, double* __restrict__ U, int m,
-                                                       int r0) {
-  __shared__ double Ls[TB][KC + 1];
-  __shared__ double Us[KC][TB + 1];
+// U rows [r0, r0 + TB) of column tile j0 = blockIdx.x * TB (<= r0): each
+// thread a 2 x 2 block of entries for k in [j0, r0) (term by term, operands
+// staged k-major in LDS, the next chunk loaded while one is summed), then the
+// tile's own rows in order, one thread per column.
+__global__ void __launch_bounds__(256) k_inv_lower(const double* __restrict__ L, double* __restrict__ U, int m,
+                                                   int r0) {
+  __shared__ double Ls[KC][TB + 2];
+  __shared__ double Us[KC][TB + 2];
   __shared__ double S[TB][TB + 1];
   __shared__ double Lt[TB][TB + 1];
-  const int tx = threadIdx.x % TB, ty = threadIdx.x / TB;
+  const int t = threadIdx.x, tr = t / 16, tc = t % 16;
   const int j0 = blockIdx.x * TB;
-  const int64_t i = r0 + ty;
-  const int j = j0 + tx;
-  double acc = (i == j) ? 1.0 : 0.0;
-  // the next chunk's staged values are loaded while the current one is summed
-  const int lr = threadIdx.x / KC, lk = threadIdx.x % KC, k2 = threadIdx.x / TB, jj = threadIdx.x % TB;
-  double ln = 0.0, un = 0.0;
+  double acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = (r0 + 2 * tr + x == j0 + 2 * tc + y) ? 1.0 : 0.0;
+  constexpr int NS = TB * KC / 256;
+  double ln[NS], un[NS];
   auto fetch = [&](int kc) {
     const int kw = min(KC, r0 - kc);
-    ln = (r0 + lr < m && lk < kw) ? L[(int64_t)(r0 + lr) * m + kc + lk] : 0.0;
-    un = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u;
+      const int lr = e / KC, lk = e % KC, k2 = e / TB, jj = e % TB;
+      ln[u] = (r0 + lr < m && lk < kw) ? L[(int64_t)(r0 + lr) * m + kc + lk] : 0.0;
+      un[u] = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
+    }
   };
   if (j0 < r0) fetch(j0);
   for (int kc = j0; kc < r0; kc += KC) {
     const int kw = min(KC, r0 - kc);
-    Ls[lr][lk] = ln;
-    Us[k2][jj] = un;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u;
+      Ls[e % KC][e / KC] = ln[u];
+      Us[e / TB][e % TB] = un[u];
+    }
     __syncthreads();
     if (kc + KC < r0) fetch(kc + KC);
-    for (int kk = 0; kk < kw; ++kk) acc = acc - Ls[ty][kk] * Us[kk][tx];
+    for (int kk = 0; kk < kw; ++kk) {
+      const double a0 = Ls[kk][2 * tr], a1 = Ls[kk][2 * tr + 1];
+      const double b0 = Us[kk][2 * tc], b1 = Us[kk][2 * tc + 1];
+      acc[0][0] = acc[0][0] - a0 * b0;
+      acc[0][1] = acc[0][1] - a0 * b1;
+      acc[1][0] = acc[1][0] - a1 * b0;
+      acc[1][1] = acc[1][1] - a1 * b1;
+    }
     __syncthreads();
   }
-  S[ty][tx] = acc;
-  Lt[ty][tx] = (r0 + ty < m && tx <= ty) ? L[(int64_t)(r0 + ty) * m + r0 + tx] : 1.0;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) S[2 * tr + x][2 * tc + y] = acc[x][y];
+  for (int e = t; e < TB * TB; e += 256) {
+    const int r = e / TB, c = e % TB;
+    Lt[r][c] = (r0 + r < m && c <= r) ? L[(int64_t)(r0 + r) * m + r0 + c] : 1.0;
+  }
   __syncthreads();
   // k in [r0, i): one thread per column, the column kept in S
-  if (ty == 0) {
+  if (t < TB) {
+    const int j = j0 + t;
     for (int s = 0; s < TB; ++s) {
-      double x = S[s][tx];
-      for (int k = 0; k < s; ++k) x = x - Lt[s][k] * S[k][tx];
-      S[s][tx] = (r0 + s >= j) ? x / Lt[s][s] : 0.0;
+      double x = S[s][t];
+      for (int k = 0; k < s; ++k) x = x - Lt[s][k] * S[k][t];
+      S[s][t] = (r0 + s >= j) ? x / Lt[s][s] : 0.0;
     }
   }
   __syncthreads();
-  if (i < m && j < m) U[i * m + j] = S[ty][tx];
+  for (int e = t; e < TB * TB; e += 256) {
+    const int r = e / TB, c = e % TB;
+    const int64_t i = r0 + r;
+    const int j = j0 + c;
+    if (i < m && j < m) U[i * m + j] = S[r][c];
+  }
 }
 
 // inv rows [i0, i0 + RB) x columns [j0, j0 + TB), lower triangle, mirrored.
@@ -340,7 +400,7 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
   const int mi = (int)m, ni = (int)n;
   for (int c0 = 0; c0 < mi; c0 += TB) {
     const int w = std::min(TB, mi - c0), c1 = c0 + w;
-    if (c0 > 0) k_chol_update<<<(unsigned)((mi - c0 + TB - 1) / TB), TB * TB>>>(dL.as<double>(), mi, c0, w);
+    if (c0 > 0) k_chol_update<<<(unsigned)((mi - c0 + UR - 1) / UR), 256>>>(dL.as<double>(), mi, c0, w);
     k_chol_diag<<<1, 64>>>(dL.as<double>(), mi, c0, w, dF.as<int>());
     if (c1 < mi) k_chol_below<<<(unsigned)((mi - c1 + RB - 1) / RB), RB>>>(dL.as<double>(), mi, c0, w, c1);
   }
@@ -350,7 +410,7 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
   if (failed) return false;
   lap("cholesky");
   for (int r0 = 0; r0 < mi; r0 += TB)
-    k_inv_lower<<<(unsigned)(r0 / TB + 1), TB * TB>>>(dL.as<double>(), dU.as<double>(), mi, r0);
+    k_inv_lower<<<(unsigned)(r0 / TB + 1), 256>>>(dL.as<double>(), dU.as<double>(), mi, r0);
   DCHK(hipGetLastError());
   lap("triangular inverse");
   k_inv_sym<<<dim3((unsigned)((mi + RB - 1) / RB), (unsigned)((mi + TB - 1) / TB)), RB>>>(
