@@ -1489,14 +1489,17 @@ void build_plan(Plan& p) {
     for (int q = 0; q < d.Q; ++q)
       if (!perr[q].empty()) throw Error(E_INVALID, perr[q]);
   }
-  if (p.explicit_op || p.precond == 2) {
+  const bool want_gram = p.explicit_op || p.precond == 2;
+  auto build_gram = [&p] {
     PhaseTimer t("explicit gramian");
-    p.gram = build_gramian(d, p.passes);
+    p.gram = build_gramian(p.d, p.passes);
     p.has_gram = true;
-  }
-  // the operator's device layout reads only d, passes and gram: it runs on a
-  // thread of its own while the preconditioner is built below (which writes
-  // subs, pre and meta); joined on every way out of this function
+  };
+  // the Gramian and the operator's device layout read only d, passes and gram:
+  // when the caller lays the plan out early they run on a thread of their own
+  // while the preconditioner is built below (which writes subs, pre and
+  // meta); joined on every way out of this function
+  if (want_gram && !(p.layout_early && d.Q > 0)) build_gram();
   struct Joiner {
     std::thread t;
     ~Joiner() {
@@ -1507,8 +1510,9 @@ void build_plan(Plan& p) {
   int lay_code = 0;
   if (p.layout_early && d.Q > 0) {
     p.lay_op.reset();
-    lay.t = std::thread([&p, &lay_err, &lay_code] {
+    lay.t = std::thread([&p, &lay_err, &lay_code, want_gram, &build_gram] {
       try {
+        if (want_gram) build_gram();
         p.lay_op = std::make_shared<DeviceLayout>(layout_operator(p));
       } catch (const Error& e) {
         lay_err = e.what();
