@@ -13,6 +13,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -452,11 +453,32 @@ void wa_solver::setup_device() {
   HIPCHK(hipGetDevice(&dev));
   phase("init");
   DeviceLayout lay;
+  // the preconditioner's arrays are laid out (host work, writing lay.pre only)
+  // on a thread of their own while the operator's are uploaded; joined before
+  // the preconditioner's upload and on every way out
+  struct Joiner {
+    std::thread t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } pre_lay;
+  std::string pre_err;
+  int pre_code = 0;
   if (plan.lay_op) {
     // the operator part was laid out while the preconditioner was built
     lay = std::move(*plan.lay_op);
     plan.lay_op.reset();
-    layout_precond(plan, lay);
+    pre_lay.t = std::thread([this, &lay, &pre_err, &pre_code] {
+      try {
+        layout_precond(plan, lay);
+      } catch (const Error& e) {
+        pre_err = e.what();
+        pre_code = e.code;
+      } catch (const std::exception& e) {
+        pre_err = e.what();
+        pre_code = WA_ERR_INVALID_ARGUMENT;
+      }
+    });
   } else {
     lay = make_device_layout(plan);
   }
@@ -527,6 +549,9 @@ void wa_solver::setup_device() {
     has_gram = true;
   }
   phase("gramian");
+  if (pre_lay.t.joinable()) pre_lay.t.join();
+  if (pre_code) throw Error(pre_code, pre_err);
+  phase("preconditioner layout");
   if (!plan.meta.empty()) {
     const PrecondFlat& f = lay.pre;
     // dense cores: the large ones are applied by the dense kernels from
