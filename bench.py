@@ -68,17 +68,22 @@ def device_Y(p, k, torch, seed=21):
     return Y
 
 
-# The local solver of every leg (measured path, plain FP64 baseline, FP64
-# reference, CPU oracle): Rust LocalSolverConfig::default() (config.rs:96-104,
-# approximate Cholesky split 1) with the sampled Schur complement at split 2
-# (ApproxSchurConfig(0, 2)): 27 instead of 36 iterations on this workload, and
-# faster per block once the exact dense core covers the last 16k vertices.
-SCHUR_SPLIT = 2
+# The local solver of every leg of the 20M headline (measured path, plain FP64
+# baseline, FP64 reference, CPU oracle): Rust LocalSolverConfig::default()
+# (config.rs:96-104, approximate Cholesky split 1) with the sampled Schur
+# complement at split 3 (ApproxSchurConfig(0, 3)), and on the GPU an exact dense
+# core for the last 20480 vertices of the reduced system.  Measured per 32-RHS
+# block: split 1 takes 36 iterations, split 2 25-28 (73.6 ms with a 16384 core),
+# split 3 22-25 (70.3 ms with a 20480 core), split 4 21-23 (73.6 ms at best).
+SCHUR_SPLIT = 3
+DENSE_CORE = 20480
+# the learned workload's local solver: split 2 and the library's 16384 core
+LEARNED_SCHUR_SPLIT = 2
 
 
-def bench_precond(W):
-    """Additive Schwarz, SchurComplement(ApproxCholConfig(0, 1), ApproxSchurConfig(0, SCHUR_SPLIT))."""
-    return W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, SCHUR_SPLIT)))
+def bench_precond(W, split=SCHUR_SPLIT):
+    """Additive Schwarz, SchurComplement(ApproxCholConfig(0, 1), ApproxSchurConfig(0, split))."""
+    return W.AdditiveSchwarz(W.SchurComplement(W.ApproxCholConfig(0, 1), W.ApproxSchurConfig(0, split)))
 
 
 class Run:
@@ -287,7 +292,7 @@ def bench_learned(args, torch):
     Y = torch.randn((k, n), generator=g, device="cuda", dtype=torch.float64)  # column-major n x k
     cfg = W.CG(tol=args.tol, precision="mixed", operator=W.OperatorRepr.Explicit)
     t0 = time.perf_counter()
-    sv = W.Solver(codes, weights=w, config=cfg, preconditioner=bench_precond(W))
+    sv = W.Solver(codes, weights=w, config=cfg, preconditioner=bench_precond(W, LEARNED_SCHUR_SPLIT))
     setup = time.perf_counter() - t0
     if not args.no_autotune:
         sv.autotune(k=k)
@@ -357,7 +362,7 @@ def main():
                     help="Krylov tolerance for both the measured path and the plain baseline")
     ap.add_argument("--operator", choices=["implicit", "explicit"], default="explicit")
     ap.add_argument("--dense-core", type=int, default=0,
-                    help="reduced-system size factored densely (0 = library default)")
+                    help="reduced-system size factored densely (0 = 20480 for the 20M workload)")
     ap.add_argument("--profile", action="store_true",
                     help="headline solves only (no reference, baseline or CPU legs): for counter runs")
     ap.add_argument("--workload", choices=["akm20m", "many", "learned"], default="akm20m",
@@ -409,7 +414,7 @@ def main():
     # --- headline: FP32 block CG + FP64 refinement, k RHS per block -------------
     log("bench: setup mixed-precision block solver ...")
     opr = W.OperatorRepr.Explicit if args.operator == "explicit" else W.OperatorRepr.Implicit
-    fast = Run(p, W.CG(precision="mixed", tol=args.tol, operator=opr), args.dense_core)
+    fast = Run(p, W.CG(precision="mixed", tol=args.tol, operator=opr), args.dense_core or DENSE_CORE)
     m = fast.m
     log(f"bench: setup {fast.setup_s:.1f}s  n_dofs={m}  info={fast.s.info().tolist()}")
     tuned = None
@@ -568,7 +573,7 @@ def main():
                                f"k={k} (Y=[y,X])", "n_obs": n, "n_dofs": m, "active_dofs": m_dev, "k": k,
                    "preconditioner": f"additive Schwarz, SchurComplement(ApproxCholConfig(0, 1), "
                                      f"ApproxSchurConfig(0, {SCHUR_SPLIT})), exact dense core of <= "
-                                     f"{args.dense_core or 16384} vertices",
+                                     f"{args.dense_core or DENSE_CORE} vertices",
                    "scale": args.scale},
         "plain_fp64_k1_rhs_per_s": round(base_rate, 4),
         "fp64_block_rhs_per_s": round(fp64_block_rate, 3),
