@@ -52,13 +52,15 @@ struct HipFail : Error {
   } while (0)
 
 // Left-looking update of the column tile [c0, c0 + w) for rows >= c0:
-// L_ij -= sum_{k<c0} L_ik L_jk, k ascending (the partial sums are stored and
-// the tile kernels continue them from k = c0).  A workgroup takes 32 rows,
+// L_ij -= sum_{ka<=k<c0} L_ik L_jk, k ascending, continuing the stored partial
+// sum (k < ka came from the bulk updates; the tile kernels continue from
+// k = c0).  A workgroup takes 32 rows,
 // each thread a 2 x 2 block of entries whose sums it runs term by term; the
 // operands of a 32-wide k chunk are staged in LDS k-major, and the next
 // chunk's are loaded while one is summed.
 constexpr int UR = 32;  // rows per update workgroup
-__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ L, int m, int c0, int w) {
+constexpr int kSwitch = 1024;  // columns between bulk updates (a multiple of 64)
+__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ L, int m, int c0, int w, int ka) {
   __shared__ double As[KC][UR + 2];
   __shared__ double Bs[KC][TB + 2];
   const int t = threadIdx.x, tr = t / 16, tc = t % 16;
@@ -85,8 +87,8 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ L, int
       bn[u] = (r < w && kk < kw) ? L[(int64_t)(c0 + r) * m + kc + kk] : 0.0;
     }
   };
-  if (c0 > 0) fetch(0);
-  for (int kc = 0; kc < c0; kc += KC) {
+  if (ka < c0) fetch(ka);
+  for (int kc = ka; kc < c0; kc += KC) {
     const int kw = min(KC, c0 - kc);
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
@@ -171,12 +173,13 @@ __global__ void __launch_bounds__(RB) k_chol_below(double* __restrict__ L, int m
     for (int jj = 0; jj < w; ++jj) L[i * m + c0 + jj] = r_[jj];
 }
 
-// U rows [r0, r0 + TB) of column tile j0 = blockIdx.x * TB (<= r0): each
-// thread a 2 x 2 block of entries for k in [j0, r0) (term by term, operands
-// staged k-major in LDS, the next chunk loaded while one is summed), then the
-// tile's own rows in order, one thread per column.
+// U rows [r0, r0 + TB) of column tile j0 = blockIdx.x * TB (<= r0), from the
+// partial sums stored in U (delta_ij less the terms k < ka of the bulk
+// updates): each thread a 2 x 2 block of entries for k in [max(j0, ka), r0)
+// (term by term, operands staged k-major in LDS, the next chunk loaded while
+// one is summed), then the tile's own rows in order, one thread per column.
 __global__ void __launch_bounds__(256) k_inv_lower(const double* __restrict__ L, double* __restrict__ U, int m,
-                                                   int r0) {
+                                                   int r0, int ka) {
   __shared__ double Ls[KC][TB + 2];
   __shared__ double Us[KC][TB + 2];
   __shared__ double S[TB][TB + 1];
@@ -187,7 +190,10 @@ __global__ void __launch_bounds__(256) k_inv_lower(const double* __restrict__ L,
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] = (r0 + 2 * tr + x == j0 + 2 * tc + y) ? 1.0 : 0.0;
+    for (int y = 0; y < 2; ++y) {
+      const int i = r0 + 2 * tr + x, j = j0 + 2 * tc + y;
+      acc[x][y] = (i < m && j < m) ? U[(int64_t)i * m + j] : 0.0;
+    }
   constexpr int NS = TB * KC / 256;
   double ln[NS], un[NS];
   auto fetch = [&](int kc) {
@@ -200,8 +206,9 @@ __global__ void __launch_bounds__(256) k_inv_lower(const double* __restrict__ L,
       un[u] = (k2 < kw && j0 + jj < m) ? U[(int64_t)(kc + k2) * m + j0 + jj] : 0.0;
     }
   };
-  if (j0 < r0) fetch(j0);
-  for (int kc = j0; kc < r0; kc += KC) {
+  const int k0 = max(j0, ka);
+  if (k0 < r0) fetch(k0);
+  for (int kc = k0; kc < r0; kc += KC) {
     const int kw = min(KC, r0 - kc);
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
@@ -248,65 +255,110 @@ __global__ void __launch_bounds__(256) k_inv_lower(const double* __restrict__ L,
   }
 }
 
-// inv rows [i0, i0 + RB) x columns [j0, j0 + TB), lower triangle, mirrored.
-__global__ void __launch_bounds__(RB) k_inv_sym(const double* __restrict__ U, double* __restrict__ inv, int m,
-                                                int n) {
-  __shared__ double Ua[KC][RB + 1];
-  __shared__ double Ub[KC][TB + 1];
-  const int t = threadIdx.x;
-  const int i0 = blockIdx.x * RB, j0 = blockIdx.y * TB;
-  if (j0 >= min(i0 + RB, m)) return;
-  const int64_t i = i0 + t;
-  double acc[TB];
+// Bulk sums over 64 x 64 tiles, each thread a 4 x 4 block of entries whose
+// terms it adds one by one, k ascending (operands of a 16-wide k chunk staged
+// in LDS, the next chunk loaded while one is summed; partial sums stored in
+// the output are continued and stored back, which is exact):
+//   kChol: L_ij -= sum_{ka<=k<kb} L_ik L_jk over tiles i, j >= (i_lo, j_lo), j <= i
+//   kTri:  U_ij -= sum_{ka<=k<kb} L_ik U_kj over rows >= i_lo, columns >= j_lo
+//          (terms with k < j are exact zeros of U)
+//   kSym:  inv_ij = inv_ji = sum_{k>=i0} U_ki U_kj, j <= i, i0 the tile's first row
+//          (terms with k < i are exact zeros of U)
+enum { kChol = 0, kTri = 1, kSym = 2 };
+constexpr int GT = 64, GK = 16;
+template <int MODE>
+__global__ void __launch_bounds__(256) k_tile_terms(const double* __restrict__ A, const double* __restrict__ B,
+                                                    double* __restrict__ C, int m, int ldc, int i_lo, int j_lo,
+                                                    int ka, int kb) {
+  __shared__ __align__(16) double As[GK][GT + 2];
+  __shared__ __align__(16) double Bs[GK][GT + 2];
+  const int t = threadIdx.x, tr = t / 16, tc = t % 16;
+  const int i0 = i_lo + blockIdx.x * GT, j0 = j_lo + blockIdx.y * GT;
+  if (MODE != kTri && j0 > i0 + GT - 1) return;
+  if (MODE == kSym) ka = i0, kb = m;
+  double acc[4][4];
+  bool live[4][4];
 #pragma unroll
-  for (int jj = 0; jj < TB; ++jj) acc[jj] = 0.0;
-  // the next chunk's staged values are loaded while the current one is summed
-  constexpr int NA = KC * RB / RB, NB = KC * TB / RB;  // values per thread: KC, KC / 2
-  double an[NA], bn[NB];
-  auto fetch = [&](int kc) {
-    const int kw = min(KC, m - kc);
+  for (int x = 0; x < 4; ++x)
 #pragma unroll
-    for (int u = 0; u < NA; ++u) {
-      const int e = t + u * RB, kk = e / RB, r = e % RB;
-      an[u] = (kk < kw && i0 + r < m) ? U[(int64_t)(kc + kk) * m + i0 + r] : 0.0;
+    for (int y = 0; y < 4; ++y) {
+      const int i = i0 + 4 * tr + x, j = j0 + 4 * tc + y;
+      live[x][y] = i < m && j < m && (MODE == kTri || j <= i);
+      acc[x][y] = (MODE != kSym && live[x][y]) ? C[(int64_t)i * ldc + j] : 0.0;
     }
+  constexpr int NS = GT * GK / 256;
+  double an[NS], bn[NS];
+  auto fetch = [&](int kc) {
+    const int kw = min(GK, kb - kc);
 #pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int e = t + u * RB, kk = e / TB, jj = e % TB;
-      bn[u] = (kk < kw && j0 + jj < m) ? U[(int64_t)(kc + kk) * m + j0 + jj] : 0.0;
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u;
+      if (MODE == kSym) {
+        const int kk = e / GT, r = e % GT;
+        an[u] = (kk < kw && i0 + r < m) ? A[(int64_t)(kc + kk) * m + i0 + r] : 0.0;
+      } else {
+        const int r = e / GK, kk = e % GK;
+        an[u] = (kk < kw && i0 + r < m) ? A[(int64_t)(i0 + r) * m + kc + kk] : 0.0;
+      }
+      if (MODE == kChol) {
+        const int r = e / GK, kk = e % GK;
+        bn[u] = (kk < kw && j0 + r < m) ? B[(int64_t)(j0 + r) * m + kc + kk] : 0.0;
+      } else {
+        const int kk = e / GT, c = e % GT;
+        bn[u] = (kk < kw && j0 + c < m) ? B[(int64_t)(kc + kk) * m + j0 + c] : 0.0;
+      }
     }
   };
-  if (i0 < m) fetch(i0);
-  for (int kc = i0; kc < m; kc += KC) {
-    const int kw = min(KC, m - kc);
+  if (ka < kb) fetch(ka);
+  for (int kc = ka; kc < kb; kc += GK) {
+    const int kw = min(GK, kb - kc);
 #pragma unroll
-    for (int u = 0; u < NA; ++u) {
-      const int e = t + u * RB;
-      Ua[e / RB][e % RB] = an[u];
-    }
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int e = t + u * RB;
-      Ub[e / TB][e % TB] = bn[u];
+    for (int u = 0; u < NS; ++u) {
+      const int e = t + 256 * u;
+      if (MODE == kSym)
+        As[e / GT][e % GT] = an[u];
+      else
+        As[e % GK][e / GK] = an[u];
+      if (MODE == kChol)
+        Bs[e % GK][e / GK] = bn[u];
+      else
+        Bs[e / GT][e % GT] = bn[u];
     }
     __syncthreads();
-    if (kc + KC < m) fetch(kc + KC);
+    if (kc + GK < kb) fetch(kc + GK);
     for (int kk = 0; kk < kw; ++kk) {
-      const double uki = Ua[kk][t];
+      double a[4], b[4];
 #pragma unroll
-      for (int jj = 0; jj < TB; ++jj) acc[jj] = acc[jj] + uki * Ub[kk][jj];
+      for (int x = 0; x < 4; ++x) a[x] = As[kk][4 * tr + x];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) b[y] = Bs[kk][4 * tc + y];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          if (MODE == kSym)
+            acc[x][y] = acc[x][y] + a[x] * b[y];
+          else
+            acc[x][y] = acc[x][y] - a[x] * b[y];
+        }
     }
     __syncthreads();
   }
-  if (i < m)
 #pragma unroll
-    for (int jj = 0; jj < TB; ++jj) {
-      const int64_t j = j0 + jj;
-      if (j <= i) {
-        inv[i * n + j] = acc[jj];
-        inv[j * n + i] = acc[jj];
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      if (live[x][y]) {
+        const int64_t i = i0 + 4 * tr + x, j = j0 + 4 * tc + y;
+        C[i * ldc + j] = acc[x][y];
+        if (MODE == kSym) C[j * ldc + i] = acc[x][y];
       }
-    }
+}
+
+// U = I on the diagonal (U is zeroed first): the starting partial sums delta_ij
+__global__ void k_unit_diag(double* __restrict__ U, int64_t m) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    U[i * m + i] = 1.0;
 }
 
 // The anchored minor's lower triangle (rows and columns [0, m), m = n - 1) of
@@ -395,12 +447,23 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
     DCHK(hipDeviceSynchronize());
   }
   DCHK(hipMemset(dU.p, 0, mm));
+  k_unit_diag<<<(unsigned)std::min<i64>((m + 255) / 256, 1024), 256>>>(dU.as<double>(), m);
   DCHK(hipMemset(dF.p, 0, sizeof(int)));
   lap("upload");
   const int mi = (int)m, ni = (int)n;
+  // every kSwitch columns the terms of the finished columns since the last
+  // switch are summed into the whole trailing part at once; each tile's own
+  // update then sums only the columns since the last switch
+  auto tiles = [](int len) { return (unsigned)((len + GT - 1) / GT); };
+  int ka = 0;
   for (int c0 = 0; c0 < mi; c0 += TB) {
     const int w = std::min(TB, mi - c0), c1 = c0 + w;
-    if (c0 > 0) k_chol_update<<<(unsigned)((mi - c0 + UR - 1) / UR), 256>>>(dL.as<double>(), mi, c0, w);
+    if (c0 > 0 && c0 % kSwitch == 0) {
+      k_tile_terms<kChol><<<dim3(tiles(mi - c0), tiles(mi - c0)), 256>>>(dL.as<double>(), dL.as<double>(),
+                                                                         dL.as<double>(), mi, mi, c0, c0, ka, c0);
+      ka = c0;
+    }
+    if (c0 > ka) k_chol_update<<<(unsigned)((mi - c0 + UR - 1) / UR), 256>>>(dL.as<double>(), mi, c0, w, ka);
     k_chol_diag<<<1, 64>>>(dL.as<double>(), mi, c0, w, dF.as<int>());
     if (c1 < mi) k_chol_below<<<(unsigned)((mi - c1 + RB - 1) / RB), RB>>>(dL.as<double>(), mi, c0, w, c1);
   }
@@ -409,12 +472,19 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
   DCHK(hipMemcpy(&failed, dF.p, sizeof(int), hipMemcpyDeviceToHost));
   if (failed) return false;
   lap("cholesky");
-  for (int r0 = 0; r0 < mi; r0 += TB)
-    k_inv_lower<<<(unsigned)(r0 / TB + 1), 256>>>(dL.as<double>(), dU.as<double>(), mi, r0);
+  ka = 0;
+  for (int r0 = 0; r0 < mi; r0 += TB) {
+    if (r0 > 0 && r0 % kSwitch == 0) {
+      k_tile_terms<kTri><<<dim3(tiles(mi - r0), tiles(r0)), 256>>>(dL.as<double>(), dU.as<double>(),
+                                                                   dU.as<double>(), mi, mi, r0, 0, ka, r0);
+      ka = r0;
+    }
+    k_inv_lower<<<(unsigned)(r0 / TB + 1), 256>>>(dL.as<double>(), dU.as<double>(), mi, r0, ka);
+  }
   DCHK(hipGetLastError());
   lap("triangular inverse");
-  k_inv_sym<<<dim3((unsigned)((mi + RB - 1) / RB), (unsigned)((mi + TB - 1) / TB)), RB>>>(
-      dU.as<double>(), static_cast<double*>(inv->p), mi, ni);
+  k_tile_terms<kSym><<<dim3(tiles(mi), tiles(mi)), 256>>>(dU.as<double>(), dU.as<double>(),
+                                                          static_cast<double*>(inv->p), mi, ni, 0, 0, 0, 0);
   DCHK(hipGetLastError());
   DCHK(hipDeviceSynchronize());
   lap("product");
