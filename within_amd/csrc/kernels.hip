@@ -1484,17 +1484,22 @@ __global__ void __launch_bounds__(kBlock) k_core_tiled(const T* __restrict__ M, 
   }
 }
 
-// FP32, 32 columns: the same tiles and staging on the matrix cores.  Each
-// wave owns 32 of the tile's 128 rows and all 32 columns, accumulated by
+// FP32, 32 columns: the same tiles on the matrix cores.  Each wave owns 32 of
+// the tile's 128 rows and all 32 columns, accumulated by
 // v_mfma_f32_32x32x2_f32 over pairs of k.  Operand layout (checked on gfx950):
 // lane l supplies A[row l % 32][k l / 32] and B[k l / 32][column l % 32];
 // result register r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// The core inverse is exactly symmetric, so the k-major tile Ms[k][row] =
+// M[row][k] = M[k][row] is staged from rows k of M: 16-byte loads of
+// contiguous row segments and 16-byte LDS stores; the row pitch (160 floats,
+// 32 banks apart) keeps the two half-waves' operand reads on distinct banks.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 __global__ void __launch_bounds__(kBlock) k_core_mfma(const float* __restrict__ M, const float* __restrict__ Zc,
                                                       float* __restrict__ P, uint32_t n, uint32_t ld, uint32_t ks) {
-  constexpr int KP = 32, NV = 4, TK = 32, TR = kCoreRows, VPR = TK / NV, MV = TR * VPR / kBlock;
+  constexpr int KP = 32, NV = 4, TK = 32, TR = kCoreRows, VPK = TR / NV, MV = TK * VPK / kBlock;
+  constexpr int MP = TR + 32;
   static_assert(TR == 4 * 32 && kBlock == 256, "four waves of 32 rows");
-  __shared__ float Ms[TK][TR + NV];
+  __shared__ __align__(16) float Ms[TK][MP];
   __shared__ __align__(16) float Zs[TK * KP];
   const int tid = threadIdx.x, wv = tid / kWave, lane = tid & (kWave - 1);
   const uint32_t r0 = blockIdx.x * TR;
@@ -1505,9 +1510,9 @@ __global__ void __launch_bounds__(kBlock) k_core_mfma(const float* __restrict__ 
 #pragma unroll
     for (int u = 0; u < MV; ++u) {
       const int v = tid + u * kBlock;
-      const int row = v / VPR, kv = (v % VPR) * NV;
-      if (r0 + row < n) {
-        mr[u] = vload<float, NV>(M + (size_t)(r0 + row) * ld + k0 + kv);
+      const int kk = v / VPK, rv = (v % VPK) * NV;
+      if (k0 + kk < n && r0 + rv < ld) {
+        mr[u] = vload<float, NV>(M + (size_t)(k0 + kk) * ld + r0 + rv);
       } else {
 #pragma unroll
         for (int c = 0; c < NV; ++c) mr[u].v[c] = 0;
@@ -1520,9 +1525,8 @@ __global__ void __launch_bounds__(kBlock) k_core_mfma(const float* __restrict__ 
 #pragma unroll
     for (int u = 0; u < MV; ++u) {
       const int v = tid + u * kBlock;
-      const int row = v / VPR, kv = (v % VPR) * NV;
-#pragma unroll
-      for (int c = 0; c < NV; ++c) Ms[kv + c][row] = mr[u].v[c];
+      const int kk = v / VPK, rv = (v % VPK) * NV;
+      *reinterpret_cast<VecT<float, NV>*>(&Ms[kk][rv]) = mr[u];
     }
     for (int v = tid; v < TK * KP / NV; v += kBlock)
       *reinterpret_cast<VecT<float, NV>*>(&Zs[v * NV]) = vload<float, NV>(Zc + (size_t)k0 * KP + v * NV);
