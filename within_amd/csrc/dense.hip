@@ -17,12 +17,16 @@
 // unchanged.
 //
 // Stage 1 is left-looking by 32-column tiles: the tile's columns receive the
-// contributions of every earlier column (one thread per entry), then the
-// diagonal tile is finished column by column and the rows below it are
-// substituted against it (one thread per row).  Stage 2 runs row tiles in
-// order, each column tile of the row tile in its own workgroup (one thread
-// per entry, then the tile's own rows in order).  Stage 3 evaluates the whole
-// lower triangle at once.  Partial sums written to memory and read back are
+// contributions of the earlier columns, then the diagonal tile is finished
+// column by column and the rows below it are substituted against it (one
+// thread per row).  Stage 2 runs row tiles in order, each column tile of the
+// row tile in its own workgroup, then the tile's own rows in order.  In both,
+// every 1024 columns (rows) a bulk launch adds the terms of the columns (rows)
+// finished since the previous one to the whole trailing part, so a tile's own
+// update only sums the terms since the last bulk launch: the late tiles, with
+// few rows left, no longer run long serial sums in few workgroups.  Stage 3
+// evaluates the whole lower triangle at once.  Every entry still receives its
+// terms in ascending k; partial sums written to memory and read back are
 // exact, so splitting a sum across kernels keeps its rounding.
 #include <hip/hip_runtime.h>
 
