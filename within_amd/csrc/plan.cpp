@@ -595,7 +595,14 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
   std::vector<u32> pick, order, touched;
   std::vector<std::vector<Edge>> fills;
   std::vector<i64> acnt(n, 0), abeg(n, 0);  // per touched vertex: its fill additions
-  std::vector<Nbr> adds;
+  // a fill addition to a vertex, tagged with the edge's place in the round's
+  // emission sequence (picked vertices in order, each one's edges in order)
+  struct FillAdd {
+    u32 nbr, seq;
+    double w;
+  };
+  std::vector<FillAdd> adds;
+  std::vector<u32> fill_at;  // per picked vertex: its first edge's place
   while (left > 0) {
     if (left <= cut) {
       std::vector<u32> core;
@@ -677,16 +684,29 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       if (nb.size() >= 2) sample_clique_tree(nb, dv, seed + (uint64_t)v, split, fills[t]);
     }
     // neighbours of the set lose their edge to it and gain the fill, added
-    // per entry in (v ascending, emission) order
+    // per entry in (v ascending, emission) order: the additions are scattered
+    // in parallel and each vertex's are then sorted by (neighbour, place in the
+    // emission sequence), which is the sequential order of the additions
+    const bool par = np >= 1024;
     touched.clear();
-    for (i64 t = 0; t < np; ++t) {
-      const u32 v = pick[t];
-      for (const Nbr& q : adj[v])
-        if (dirty[q.first] != round + 1) { dirty[q.first] = round + 1; touched.push_back(q.first); }
+#pragma omp parallel if (par)
+    {
+      std::vector<u32> mine;
+#pragma omp for schedule(dynamic, 64) nowait
+      for (i64 t = 0; t < np; ++t)
+        for (const Nbr& q : adj[pick[t]])
+          if (__atomic_exchange_n(&dirty[q.first], round + 1, __ATOMIC_RELAXED) != round + 1) mine.push_back(q.first);
+#pragma omp critical
+      touched.insert(touched.end(), mine.begin(), mine.end());
     }
-    for (u32 u : touched) acnt[u] = 0;
+    const i64 nt = (i64)touched.size();
+    for (i64 k = 0; k < nt; ++k) acnt[touched[k]] = 0;
+#pragma omp parallel for schedule(dynamic, 64) if (par)
     for (i64 t = 0; t < np; ++t)
-      for (const Edge& e : fills[t]) { acnt[e.lo]++; acnt[e.hi]++; }
+      for (const Edge& e : fills[t]) {
+        __atomic_fetch_add(&acnt[e.lo], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&acnt[e.hi], 1, __ATOMIC_RELAXED);
+      }
     i64 run = 0;
     for (u32 u : touched) {  // acnt becomes the write cursor of u's range
       abeg[u] = run;
@@ -694,11 +714,18 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       acnt[u] = abeg[u];
     }
     adds.resize(run);
-    for (i64 t = 0; t < np; ++t)
+    fill_at.resize(np + 1);
+    fill_at[0] = 0;
+    for (i64 t = 0; t < np; ++t) fill_at[t + 1] = fill_at[t] + (u32)fills[t].size();
+#pragma omp parallel for schedule(dynamic, 64) if (par)
+    for (i64 t = 0; t < np; ++t) {
+      u32 sq = fill_at[t];
       for (const Edge& e : fills[t]) {
-        adds[acnt[e.lo]++] = {e.hi, e.w};
-        adds[acnt[e.hi]++] = {e.lo, e.w};
+        adds[__atomic_fetch_add(&acnt[e.lo], 1, __ATOMIC_RELAXED)] = {e.hi, sq, e.w};
+        adds[__atomic_fetch_add(&acnt[e.hi], 1, __ATOMIC_RELAXED)] = {e.lo, sq, e.w};
+        ++sq;
       }
+    }
     for (i64 t = 0; t < np; ++t) {
       const u32 v = pick[t];
       cols[v] = std::move(adj[v]);
@@ -706,31 +733,39 @@ static Factor factor_laplacian(const Csr& lap, uint64_t seed, int split, i64 den
       live[v] = 0;
       f.order.push_back(v);
     }
-    const i64 nt = (i64)touched.size();
-#pragma omp parallel for schedule(dynamic, 64) if (nt >= 1024)
-    for (i64 k = 0; k < nt; ++k) {
-      const u32 u = touched[k];
-      Nbr* a0 = adds.data() + abeg[u];
-      Nbr* a1 = adds.data() + acnt[u];
-      std::stable_sort(a0, a1, [](const Nbr& x, const Nbr& y) { return x.first < y.first; });
-      const std::vector<Nbr>& old = adj[u];
+#pragma omp parallel if (nt >= 1024)
+    {
       std::vector<Nbr> merged;
-      merged.reserve(old.size() + (size_t)(a1 - a0));
-      size_t i = 0;
-      const Nbr* p = a0;
-      while (i < old.size() || p < a1) {
-        if (i < old.size() && !live[old[i].first]) { ++i; continue; }
-        if (p == a1 || (i < old.size() && old[i].first < p->first)) {
-          merged.push_back(old[i++]);
-          continue;
+      auto before = [](const FillAdd& x, const FillAdd& y) { return x.nbr != y.nbr ? x.nbr < y.nbr : x.seq < y.seq; };
+#pragma omp for schedule(dynamic, 64)
+      for (i64 k = 0; k < nt; ++k) {
+        const u32 u = touched[k];
+        FillAdd* a0 = adds.data() + abeg[u];
+        FillAdd* a1 = adds.data() + acnt[u];
+        if (a1 - a0 <= 16) {
+          for (FillAdd* x = a0 + 1; x < a1; ++x)
+            for (FillAdd* y = x; y > a0 && before(*y, *(y - 1)); --y) std::swap(*y, *(y - 1));
+        } else {
+          std::sort(a0, a1, before);
         }
-        const u32 j = p->first;
-        double acc = 0.0;
-        if (i < old.size() && old[i].first == j) acc = old[i++].second;
-        for (; p < a1 && p->first == j; ++p) acc += p->second;
-        merged.push_back({j, acc});
+        const std::vector<Nbr>& old = adj[u];
+        merged.clear();
+        size_t i = 0;
+        const FillAdd* p = a0;
+        while (i < old.size() || p < a1) {
+          if (i < old.size() && !live[old[i].first]) { ++i; continue; }
+          if (p == a1 || (i < old.size() && old[i].first < p->nbr)) {
+            merged.push_back(old[i++]);
+            continue;
+          }
+          const u32 j = p->nbr;
+          double acc = 0.0;
+          if (i < old.size() && old[i].first == j) acc = old[i++].second;
+          for (; p < a1 && p->nbr == j; ++p) acc += p->w;
+          merged.push_back({j, acc});
+        }
+        adj[u].assign(merged.begin(), merged.end());
       }
-      adj[u] = std::move(merged);
     }
     left -= np;
     f.round_ptr.push_back((u32)f.order.size());
