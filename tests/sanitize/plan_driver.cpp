@@ -21,6 +21,7 @@ bool device_anchored_inverse(i64, const u32*, const u32*, const double*, const d
   return false;
 }
 DeviceDense::~DeviceDense() {}
+void DeviceDense::ready() const {}
 std::vector<double> DeviceDense::download() const { return {}; }
 }  // namespace wa
 

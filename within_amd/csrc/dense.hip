@@ -387,6 +387,11 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  void* release() {
+    void* r = p;
+    p = nullptr;
+    return r;
+  }
   template <typename T>
   T* as() const { return (T*)p; }
 };
@@ -394,10 +399,27 @@ struct DevBuf {
 }  // namespace
 
 DeviceDense::~DeviceDense() {
+  try {
+    ready();
+  } catch (const std::exception&) {
+  }
   if (p) (void)hipFree(p);
 }
 
+void DeviceDense::ready() const {
+  std::lock_guard<std::mutex> g(mu);
+  if (!pending) return;
+  hipStream_t s = static_cast<hipStream_t>(pending);
+  pending = nullptr;
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipStreamDestroy(s);
+  for (void* b : scratch) (void)hipFree(b);
+  scratch.clear();
+  if (e != hipSuccess) throw HipFail(std::string("HIP: ") + hipGetErrorString(e));
+}
+
 std::vector<double> DeviceDense::download() const {
+  ready();
   std::vector<double> h((size_t)n * n);
   if (!h.empty()) DCHK(hipMemcpy(h.data(), p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
   return h;
@@ -421,18 +443,40 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
   DCHK(hipGetDeviceProperties(&prop, dev));
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     throw HipFail(std::string("within_amd requires a gfx950 (MI355X) device, found ") + prop.gcnArchName);
+  // everything runs on a stream of its own (not ordered with the null stream,
+  // so the caller's uploads proceed meanwhile); the stream is synchronised
+  // and destroyed on every way out except the last, where the inverse takes it
+  struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    }
+  } ds;
+  DCHK(hipStreamCreateWithFlags(&ds.s, hipStreamNonBlocking));
+  const hipStream_t st = ds.s;
   const size_t nn = (size_t)n * n * sizeof(double);
   auto inv = std::make_shared<DeviceDense>();
   inv->n = n;
   DCHK(hipMalloc(&inv->p, std::max<size_t>(nn, sizeof(double))));
-  DCHK(hipMemset(inv->p, 0, std::max<size_t>(nn, sizeof(double))));
+  DCHK(hipMemsetAsync(inv->p, 0, std::max<size_t>(nn, sizeof(double)), st));
   const i64 m = n - 1;
   if (m <= 0) {
+    DCHK(hipStreamSynchronize(st));
     out = inv;
     return true;
   }
   const size_t mm = (size_t)m * m * sizeof(double);
   DevBuf dL(mm), dU(mm), dF(sizeof(int));
+  // on a way out with work in flight, it finishes before any buffer is freed
+  struct SyncOnExit {
+    hipStream_t* s;
+    ~SyncOnExit() {
+      if (*s) (void)hipStreamSynchronize(*s);
+    }
+  } sync_exit{&ds.s};
   {
     // the sparse Laplacian goes up; the minor's dense lower triangle is laid out on the device
     const size_t nnz = ptr[n];
@@ -444,15 +488,15 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
       DCHK(hipMemcpy(dW.p, w, sizeof(double) * nnz, hipMemcpyHostToDevice));
     }
     DCHK(hipMemcpy(dD.p, diag, sizeof(double) * n, hipMemcpyHostToDevice));
-    DCHK(hipMemset(dL.p, 0, mm));
-    k_minor_from_csr<<<(unsigned)m, 256>>>(dP.as<u32>(), dC.as<u32>(), dW.as<double>(), dD.as<double>(),
-                                            dL.as<double>(), m);
+    DCHK(hipMemsetAsync(dL.p, 0, mm, st));
+    k_minor_from_csr<<<(unsigned)m, 256, 0, st>>>(dP.as<u32>(), dC.as<u32>(), dW.as<double>(), dD.as<double>(),
+                                                   dL.as<double>(), m);
     DCHK(hipGetLastError());
-    DCHK(hipDeviceSynchronize());
+    DCHK(hipStreamSynchronize(st));  // before the CSR buffers are freed
   }
-  DCHK(hipMemset(dU.p, 0, mm));
-  k_unit_diag<<<(unsigned)std::min<i64>((m + 255) / 256, 1024), 256>>>(dU.as<double>(), m);
-  DCHK(hipMemset(dF.p, 0, sizeof(int)));
+  DCHK(hipMemsetAsync(dU.p, 0, mm, st));
+  k_unit_diag<<<(unsigned)std::min<i64>((m + 255) / 256, 1024), 256, 0, st>>>(dU.as<double>(), m);
+  DCHK(hipMemsetAsync(dF.p, 0, sizeof(int), st));
   lap("upload");
   const int mi = (int)m, ni = (int)n;
   // every kSwitch columns the terms of the finished columns since the last
@@ -463,35 +507,43 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
   for (int c0 = 0; c0 < mi; c0 += TB) {
     const int w = std::min(TB, mi - c0), c1 = c0 + w;
     if (c0 > 0 && c0 % kSwitch == 0) {
-      k_tile_terms<kChol><<<dim3(tiles(mi - c0), tiles(mi - c0)), 256>>>(dL.as<double>(), dL.as<double>(),
-                                                                         dL.as<double>(), mi, mi, c0, c0, ka, c0);
+      k_tile_terms<kChol><<<dim3(tiles(mi - c0), tiles(mi - c0)), 256, 0, st>>>(
+          dL.as<double>(), dL.as<double>(), dL.as<double>(), mi, mi, c0, c0, ka, c0);
       ka = c0;
     }
-    if (c0 > ka) k_chol_update<<<(unsigned)((mi - c0 + UR - 1) / UR), 256>>>(dL.as<double>(), mi, c0, w, ka);
-    k_chol_diag<<<1, 64>>>(dL.as<double>(), mi, c0, w, dF.as<int>());
-    if (c1 < mi) k_chol_below<<<(unsigned)((mi - c1 + RB - 1) / RB), RB>>>(dL.as<double>(), mi, c0, w, c1);
+    if (c0 > ka) k_chol_update<<<(unsigned)((mi - c0 + UR - 1) / UR), 256, 0, st>>>(dL.as<double>(), mi, c0, w, ka);
+    k_chol_diag<<<1, 64, 0, st>>>(dL.as<double>(), mi, c0, w, dF.as<int>());
+    if (c1 < mi) k_chol_below<<<(unsigned)((mi - c1 + RB - 1) / RB), RB, 0, st>>>(dL.as<double>(), mi, c0, w, c1);
   }
   DCHK(hipGetLastError());
   int failed = 0;
-  DCHK(hipMemcpy(&failed, dF.p, sizeof(int), hipMemcpyDeviceToHost));
+  DCHK(hipMemcpyAsync(&failed, dF.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
   if (failed) return false;
   lap("cholesky");
   ka = 0;
   for (int r0 = 0; r0 < mi; r0 += TB) {
     if (r0 > 0 && r0 % kSwitch == 0) {
-      k_tile_terms<kTri><<<dim3(tiles(mi - r0), tiles(r0)), 256>>>(dL.as<double>(), dU.as<double>(),
-                                                                   dU.as<double>(), mi, mi, r0, 0, ka, r0);
+      k_tile_terms<kTri><<<dim3(tiles(mi - r0), tiles(r0)), 256, 0, st>>>(dL.as<double>(), dU.as<double>(),
+                                                                          dU.as<double>(), mi, mi, r0, 0, ka, r0);
       ka = r0;
     }
-    k_inv_lower<<<(unsigned)(r0 / TB + 1), 256>>>(dL.as<double>(), dU.as<double>(), mi, r0, ka);
+    k_inv_lower<<<(unsigned)(r0 / TB + 1), 256, 0, st>>>(dL.as<double>(), dU.as<double>(), mi, r0, ka);
   }
   DCHK(hipGetLastError());
   lap("triangular inverse");
-  k_tile_terms<kSym><<<dim3(tiles(mi), tiles(mi)), 256>>>(dU.as<double>(), dU.as<double>(),
-                                                          static_cast<double*>(inv->p), mi, ni, 0, 0, 0, 0);
+  k_tile_terms<kSym><<<dim3(tiles(mi), tiles(mi)), 256, 0, st>>>(dU.as<double>(), dU.as<double>(),
+                                                                 static_cast<double*>(inv->p), mi, ni, 0, 0, 0, 0);
   DCHK(hipGetLastError());
-  DCHK(hipDeviceSynchronize());
   lap("product");
+  // the triangular inverse and the product finish behind the caller's back:
+  // the inverse takes the stream and the two scratch matrices (ready() waits)
+  {
+    std::lock_guard<std::mutex> g(inv->mu);
+    inv->scratch = {dL.release(), dU.release()};
+    inv->pending = ds.s;
+    ds.s = nullptr;
+  }
   out = inv;
   return true;
 }

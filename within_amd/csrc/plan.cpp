@@ -1396,6 +1396,84 @@ DeviceLayout layout_operator(const Plan& p) {
   return L;
 }
 
+// a deep copy of a flattened preconditioner, its arrays copied side by side
+static void copy_flat(const PrecondFlat& f, PrecondFlat& c) {
+  static_assert(sizeof(PrecondFlat) == 7 * sizeof(i64) + 33 * sizeof(std::vector<u32>),
+                "copy_flat copies every member of PrecondFlat");
+  c.n_loc = f.n_loc, c.n_red = f.n_red, c.n_sub = f.n_sub;
+  c.max_core = f.max_core, c.ac_nnz = f.ac_nnz, c.c_nnz = f.c_nnz, c.dense_subs = f.dense_subs;
+#pragma omp parallel
+#pragma omp single
+  {
+#pragma omp task
+    c.loc_gidx = f.loc_gidx;
+#pragma omp task
+    c.loc_scale = f.loc_scale;
+#pragma omp task
+    c.loc_ptr = f.loc_ptr;
+#pragma omp task
+    c.red_ptr = f.red_ptr;
+#pragma omp task
+    c.loc_sub = f.loc_sub;
+#pragma omp task
+    c.red_sub = f.red_sub;
+#pragma omp task
+    c.red_loc = f.red_loc;
+#pragma omp task
+    c.elim_loc = f.elim_loc;
+#pragma omp task
+    c.elim_inv = f.elim_inv;
+#pragma omp task
+    c.elim_ptr = f.elim_ptr;
+#pragma omp task
+    c.elim_idx = f.elim_idx;
+#pragma omp task
+    c.elim_val = f.elim_val;
+#pragma omp task
+    c.keep_ptr = f.keep_ptr;
+#pragma omp task
+    c.keep_idx = f.keep_idx;
+#pragma omp task
+    c.keep_val = f.keep_val;
+#pragma omp task
+    c.keep_rowsum = f.keep_rowsum;
+#pragma omp task
+    c.lvl_ptr = f.lvl_ptr;
+#pragma omp task
+    c.lvl_rows = f.lvl_rows;
+#pragma omp task
+    c.fin_ptr = f.fin_ptr;
+#pragma omp task
+    c.fin_idx = f.fin_idx;
+#pragma omp task
+    c.fin_val = f.fin_val;
+#pragma omp task
+    c.bout_ptr = f.bout_ptr;
+#pragma omp task
+    c.bout_idx = f.bout_idx;
+#pragma omp task
+    c.bout_val = f.bout_val;
+#pragma omp task
+    c.inv_pivot = f.inv_pivot;
+#pragma omp task
+    c.core_ptr = f.core_ptr;
+#pragma omp task
+    c.core_idx = f.core_idx;
+#pragma omp task
+    c.core_mat_off = f.core_mat_off;
+#pragma omp task
+    c.core_mat = f.core_mat;
+#pragma omp task
+    c.core_row_core = f.core_row_core;
+#pragma omp task
+    c.core_dev = f.core_dev;
+#pragma omp task
+    c.dof_ptr = f.dof_ptr;
+#pragma omp task
+    c.dof_loc = f.dof_loc;
+  }
+}
+
 void layout_precond(const Plan& p, DeviceLayout& L) {
   PhaseTimer t_pre("layout: preconditioner");
   // preconditioner: DOF indices renumbered; each subdomain's eliminated block
@@ -1407,7 +1485,8 @@ void layout_precond(const Plan& p, DeviceLayout& L) {
     for (u32 g : f.loc_gidx)
       if (L.int_of[g] == UINT32_MAX)
         throw Error(E_INVALID, "the preconditioner does not match the design (it covers an unobserved level)");
-    PrecondFlat nf = f;
+    PrecondFlat nf;
+    copy_flat(f, nf);
     const i64 ne = (i64)f.elim_loc.size();
     std::vector<u32> lp(f.n_loc);
     std::iota(lp.begin(), lp.end(), 0u);

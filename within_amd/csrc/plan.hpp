@@ -16,6 +16,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <mutex>
 #include <vector>
 
 namespace wa {
@@ -114,15 +115,23 @@ struct CrossTab {
 
 // A dense core's anchored inverse kept in device memory (FP64, n x n
 // row-major) where it was computed; host copies are made only on request
-// (serialisation, exports).  Shared by the objects of one plan.
+// (serialisation, exports).  Shared by the objects of one plan.  Its last
+// kernels may still run on a stream of their own when it is handed over, so
+// every reader of p calls ready() first (download() does): the first call
+// waits for them and frees their scratch.
 struct DeviceDense {
   void* p = nullptr;
   i64 n = 0;
   DeviceDense() = default;
   DeviceDense(const DeviceDense&) = delete;
   DeviceDense& operator=(const DeviceDense&) = delete;
-  ~DeviceDense();                        // frees the device memory
+  ~DeviceDense();                        // waits for the kernels, frees the device memory
+  void ready() const;                    // blocks until p holds the inverse (throws on a device error)
   std::vector<double> download() const;  // the n x n host copy
+  // the computation still in flight: its stream (a hipStream_t) and scratch
+  mutable void* pending = nullptr;
+  mutable std::vector<void*> scratch;
+  mutable std::mutex mu;
 };
 
 // Reduced-system factor (rounds of independent eliminations + dense core).

@@ -580,7 +580,9 @@ void wa_solver::setup_device() {
         d.n = f.core_ptr[c + 1] - d.b;
         d.ld = (d.n + 31) / 32 * 32;
         if (const DeviceDense* dv = device_core(f, c)) {
-          // factored on the device: padded (and, for FP32, rounded) there
+          // factored on the device: padded (and, for FP32, rounded) there,
+          // once the inverse's own kernels are done
+          dv->ready();
           T* M = dalloc<T>((int64_t)d.n * d.ld);
           launch_pad_core<T>(st, static_cast<const double*>(dv->p), d.n, M, d.ld);
           d.M = M;
