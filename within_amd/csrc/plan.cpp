@@ -48,15 +48,21 @@ Design make_design(const u32* codes, i64 n_obs, int Q, i64 ld, const double* wei
   d.n_obs = n_obs;
   d.Q = Q;
   d.codes.resize(Q);
-  i64 off = 0;
+  // the factors side by side: each copied and its largest code found
+  std::vector<u32> mx(Q, 0);
+#pragma omp parallel for schedule(dynamic, 1) if (n_obs >= 65536)
   for (int q = 0; q < Q; ++q) {
     const u32* col = codes + (i64)q * ld;
     d.codes[q].assign(col, col + n_obs);
-    u32 mx = 0;
-    for (i64 i = 0; i < n_obs; ++i) mx = std::max(mx, col[i]);
-    d.levels.push_back((i64)mx + 1);
+    u32 m = 0;
+    for (i64 i = 0; i < n_obs; ++i) m = std::max(m, col[i]);
+    mx[q] = m;
+  }
+  i64 off = 0;
+  for (int q = 0; q < Q; ++q) {
+    d.levels.push_back((i64)mx[q] + 1);
     d.offset.push_back(off);
-    off += (i64)mx + 1;
+    off += (i64)mx[q] + 1;
   }
   if (off >= (i64)UINT32_MAX) throw Error(E_OVERFLOW, "Overflow: n_dofs exceeds u32 indexing");
   d.n_dofs = off;
