@@ -2032,6 +2032,15 @@ void launch_pad_core(hipStream_t st, const double* src, uint32_t n, T* dst, uint
   k_pad_core<T><<<(unsigned)std::min<int64_t>((tot + 255) / 256, 65536), 256, 0, st>>>(src, n, dst, ld);
 }
 template void launch_pad_core<float>(hipStream_t, const double*, uint32_t, float*, uint32_t);
+
+__global__ void k_narrow(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+void launch_narrow(hipStream_t st, const double* src, float* dst, int64_t n) {
+  if (n <= 0) return;
+  k_narrow<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65536), 256, 0, st>>>(src, dst, n);
+}
 template void launch_pad_core<double>(hipStream_t, const double*, uint32_t, double*, uint32_t);
 
 }  // namespace wa

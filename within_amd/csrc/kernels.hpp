@@ -233,5 +233,7 @@ void launch_rsub(hipStream_t st, T* R, const T* B, int64_t n);
 // dst[i * ld + j] = (T) src[i * n + j] for j < n, 0 for n <= j < ld (i < n)
 template <typename T>
 void launch_pad_core(hipStream_t st, const double* src, uint32_t n, T* dst, uint32_t ld);
+// dst[i] = (float) src[i], i < n (the host cast's rounding: to nearest even)
+void launch_narrow(hipStream_t st, const double* src, float* dst, int64_t n);
 
 }  // namespace wa

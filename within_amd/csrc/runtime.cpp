@@ -277,10 +277,11 @@ struct wa_solver {
     if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return p;
   }
-  template <typename T>
-  T* upload_as(const std::vector<double>& v) {
-    std::vector<T> c(v.begin(), v.end());
-    return upload(c);
+  // an FP32 copy, made on the device, of n doubles already uploaded
+  float* narrowed(const double* d, int64_t n) {
+    float* p = dalloc<float>(n);
+    launch_narrow(st, d, p, n);
+    return p;
   }
   // schedule shape on stderr when WITHIN_AMD_VERBOSE is set
   static void sched_report(const char* name, const Schedule& s, const std::vector<uint32_t>& ptr) {
@@ -512,10 +513,10 @@ void wa_solver::setup_device() {
     a.obs = b.obs = upload(fp.obs);
     if (!fp.w.empty()) {
       a.ew = upload(fp.w);
-      b.ew = upload_as<float>(fp.w);
+      b.ew = narrowed(a.ew, (int64_t)fp.w.size());
     }
     a.diag = upload(fp.diag);
-    b.diag = upload_as<float>(fp.diag);
+    b.diag = narrowed(a.diag, (int64_t)fp.diag.size());
     a.qm = b.qm = Q - 1;
     a.off = b.off = (uint32_t)lay.off[fp.q];
     a.levels = b.levels = lay.levels[fp.q];
@@ -536,7 +537,7 @@ void wa_solver::setup_device() {
     g64.ptr = g32.ptr = upload(g.indptr);
     g64.idx = g32.idx = upload(g.indices);
     g64.val = upload(g.data);
-    g32.val = upload_as<float>(g.data);
+    g32.val = narrowed(g64.val, (int64_t)g.data.size());
     g64.nrows = g32.nrows = g.nrows;
     g64.nnz = g32.nnz = (int64_t)g.indices.size();
     {
@@ -598,26 +599,35 @@ void wa_solver::setup_device() {
     };
     upload_big(pre64, double());
     upload_big(pre32, float());
-    auto fill = [&](auto& P, auto tag) {
-      using T = decltype(tag);
+    auto sizes = [&](auto& P) {
       P.n_loc = f.n_loc;
       P.n_red = f.n_red;
       P.n_sub = f.n_sub;
       P.n_elim = (int64_t)f.elim_loc.size();
       P.n_levels = (int64_t)f.lvl_ptr.size() - 1;
       P.n_core_rows = (int64_t)f.core_idx.size();
-      P.loc_scale = upload_as<T>(f.loc_scale);
-      P.elim_inv = upload_as<T>(f.elim_inv);
-      P.elim_val = upload_as<T>(f.elim_val);
-      P.keep_val = upload_as<T>(f.keep_val);
-      P.keep_rowsum = upload_as<T>(f.keep_rowsum);
-      P.fin_val = upload_as<T>(f.fin_val);
-      P.bout_val = upload_as<T>(f.bout_val);
-      P.inv_pivot = upload_as<T>(f.inv_pivot);
-      P.core_mat = upload_as<T>(small_mat);
     };
-    fill(pre64, double());
-    fill(pre32, float());
+    sizes(pre64);
+    sizes(pre32);
+    // the FP64 values go up; their FP32 copies are made on the device
+    pre64.loc_scale = upload(f.loc_scale);
+    pre64.elim_inv = upload(f.elim_inv);
+    pre64.elim_val = upload(f.elim_val);
+    pre64.keep_val = upload(f.keep_val);
+    pre64.keep_rowsum = upload(f.keep_rowsum);
+    pre64.fin_val = upload(f.fin_val);
+    pre64.bout_val = upload(f.bout_val);
+    pre64.inv_pivot = upload(f.inv_pivot);
+    pre64.core_mat = upload(small_mat);
+    pre32.loc_scale = narrowed(pre64.loc_scale, (int64_t)f.loc_scale.size());
+    pre32.elim_inv = narrowed(pre64.elim_inv, (int64_t)f.elim_inv.size());
+    pre32.elim_val = narrowed(pre64.elim_val, (int64_t)f.elim_val.size());
+    pre32.keep_val = narrowed(pre64.keep_val, (int64_t)f.keep_val.size());
+    pre32.keep_rowsum = narrowed(pre64.keep_rowsum, (int64_t)f.keep_rowsum.size());
+    pre32.fin_val = narrowed(pre64.fin_val, (int64_t)f.fin_val.size());
+    pre32.bout_val = narrowed(pre64.bout_val, (int64_t)f.bout_val.size());
+    pre32.inv_pivot = narrowed(pre64.inv_pivot, (int64_t)f.inv_pivot.size());
+    pre32.core_mat = narrowed(pre64.core_mat, (int64_t)small_mat.size());
     // integer arrays shared by both precisions
     const uint32_t* loc_gidx = upload(f.loc_gidx);
     const uint32_t* loc_ptr = upload(f.loc_ptr);
