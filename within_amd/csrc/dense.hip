@@ -28,6 +28,11 @@
 // evaluates the whole lower triangle at once.  Every entry still receives its
 // terms in ascending k; partial sums written to memory and read back are
 // exact, so splitting a sum across kernels keeps its rounding.
+//
+// Everything runs on a stream of its own.  Once the Cholesky is known to have
+// succeeded, the triangular inverse and the product are left running: the
+// returned inverse holds the stream and the scratch, and DeviceDense::ready()
+// waits for them before the matrix is read.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -58,9 +63,9 @@ struct HipFail : Error {
 // Left-looking update of the column tile [c0, c0 + w) for rows >= c0:
 // L_ij -= sum_{ka<=k<c0} L_ik L_jk, k ascending, continuing the stored partial
 // sum (k < ka came from the bulk updates; the tile kernels continue from
-// k = c0).  A workgroup takes 32 rows,
-// each thread a 2 x 2 block of entries whose sums it runs term by term; the
-// operands of a 32-wide k chunk are staged in LDS k-major, and the next
+// k = c0).  A workgroup takes 32 rows, each thread a 2 x 2 block of entries
+// whose sums it runs term by term; the operands of a 32-wide k chunk are
+// staged in LDS k-major, and the next
 // chunk's are loaded while one is summed.
 constexpr int UR = 32;  // rows per update workgroup
 constexpr int kSwitch = 1024;  // columns between bulk updates (a multiple of 64)
