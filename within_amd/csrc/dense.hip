@@ -487,12 +487,14 @@ bool device_anchored_inverse(i64 n, const u32* ptr, const u32* col, const double
     const size_t nnz = ptr[n];
     DevBuf dP(sizeof(u32) * (n + 1)), dC(sizeof(u32) * std::max<size_t>(nnz, 1)),
         dW(sizeof(double) * std::max<size_t>(nnz, 1)), dD(sizeof(double) * n);
-    DCHK(hipMemcpy(dP.p, ptr, sizeof(u32) * (n + 1), hipMemcpyHostToDevice));
+    // on the core's stream, so the layout kernel is ordered after them (the
+    // host arrays outlive the synchronisation below)
+    DCHK(hipMemcpyAsync(dP.p, ptr, sizeof(u32) * (n + 1), hipMemcpyHostToDevice, st));
     if (nnz) {
-      DCHK(hipMemcpy(dC.p, col, sizeof(u32) * nnz, hipMemcpyHostToDevice));
-      DCHK(hipMemcpy(dW.p, w, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      DCHK(hipMemcpyAsync(dC.p, col, sizeof(u32) * nnz, hipMemcpyHostToDevice, st));
+      DCHK(hipMemcpyAsync(dW.p, w, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
     }
-    DCHK(hipMemcpy(dD.p, diag, sizeof(double) * n, hipMemcpyHostToDevice));
+    DCHK(hipMemcpyAsync(dD.p, diag, sizeof(double) * n, hipMemcpyHostToDevice, st));
     DCHK(hipMemsetAsync(dL.p, 0, mm, st));
     k_minor_from_csr<<<(unsigned)m, 256, 0, st>>>(dP.as<u32>(), dC.as<u32>(), dW.as<double>(), dD.as<double>(),
                                                    dL.as<double>(), m);

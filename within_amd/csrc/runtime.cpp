@@ -181,6 +181,7 @@ struct wa_solver {
   size_t lead = 0;
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_upload = nullptr;  // the null stream's uploads, before device casts on st
   double* part2 = nullptr;                 // st2's split-row partials
   // the lead pair's trailing levels as one launch per direction (engine.hpp
   // k_level_flow): counters of the forward and backward runs, the previous
@@ -277,9 +278,13 @@ struct wa_solver {
     if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return p;
   }
-  // an FP32 copy, made on the device, of n doubles already uploaded
+  // an FP32 copy, made on the device, of n doubles already uploaded (on the
+  // null stream: st waits for that stream's work first)
   float* narrowed(const double* d, int64_t n) {
     float* p = dalloc<float>(n);
+    if (!ev_upload) HIPCHK(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev_upload, nullptr));
+    HIPCHK(hipStreamWaitEvent(st, ev_upload, 0));
     launch_narrow(st, d, p, n);
     return p;
   }
@@ -321,6 +326,7 @@ struct wa_solver {
     if (hbuf) (void)hipHostFree(hbuf);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_upload) (void)hipEventDestroy(ev_upload);
     if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
